@@ -1,0 +1,100 @@
+/*
+ * blst_hip.h -- C ABI of the MI355X (gfx950) backend for BLS12-381 G1
+ * multi-scalar multiplication.
+ *
+ * The first group of entry points is the drop-in boundary: the same names,
+ * argument meaning, scratch sizes and fail-closed behaviour as the scalar
+ * backend's C API, so a caller linked against this library instead of the
+ * reference's MSM objects needs no source change.  The second group exposes
+ * what the device path adds on top (device-resident inputs, batches of MSMs,
+ * the serving loop, error reporting); it has no counterpart in the reference.
+ *
+ * All pointers are plain host pointers unless the name says "device".  No
+ * entry point throws; every failure writes the point at infinity (all-zero
+ * blst_p1), as the reference's early returns do, and records a message that
+ * blst_hip_last_error() returns.
+ */
+#ifndef BLST_HIP_H
+#define BLST_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef BLST_TYPES_H
+/* Layout-identical restatement of bindings/blst_types.h:40-42, 95-131 for
+ * callers that do not include the reference headers. */
+typedef uint8_t blst_byte;
+typedef uint64_t blst_limb_t;
+typedef struct { blst_limb_t l[384 / 8 / sizeof(blst_limb_t)]; } blst_fp;
+typedef struct { blst_fp x, y, z; } blst_p1;
+typedef struct { blst_fp x, y; } blst_p1_affine;
+#endif
+
+/* ------------------------------------------------------------------ drop-in */
+
+/* Replaces blst_p1s_mult_pippenger_scratch_sizeof
+ * (reference bindings/blst_scalar.h:535, blst/c_api.cpp:1371-1374):
+ * 192 * 2^(window-1) bytes for the reference's window heuristic, 0 for no points. */
+size_t blst_p1s_mult_pippenger_scratch_sizeof(size_t npoints);
+
+/* Replaces blst_p1s_mult_pippenger_window_scratch_sizeof
+ * (bindings/blst_scalar.h:537, blst/c_api.cpp:1376-1380): 0 unless window in [1, 24]. */
+size_t blst_p1s_mult_pippenger_window_scratch_sizeof(size_t window);
+
+/* Replaces blst_p1s_mult_pippenger (bindings/blst_scalar.h:539,
+ * blst/c_api.cpp:1382-1418).  Same contract: descriptor arrays with later
+ * NULL entries meaning contiguous continuation; npoints == 0 or nbits == 0,
+ * nbits > 256, NULL heads, or npoints > 1 with NULL scratch -> infinity;
+ * npoints == 1 needs no scratch.  The caller's scratch is accepted and not
+ * written: bucket memory lives on the GPU. */
+void blst_p1s_mult_pippenger(blst_p1 *ret, const blst_p1_affine *const points[],
+                             size_t npoints, const blst_byte *const scalars[],
+                             size_t nbits, blst_limb_t *scratch);
+
+/* Replaces blst_p1s_mult_pippenger_window (bindings/blst_scalar.h:603,
+ * blst/c_api.cpp:1546-1583): explicit window in [1, 24], else infinity. */
+void blst_p1s_mult_pippenger_window(blst_p1 *ret, const blst_p1_affine *const points[],
+                                    size_t npoints, const blst_byte *const scalars[],
+                                    size_t nbits, blst_limb_t *scratch, size_t window);
+
+/* Replaces blst_p1s_tile_pippenger (bindings/blst_scalar.h:605,
+ * blst/c_api.cpp:1585-1611, 883-925): the signed Booth digit window starting
+ * at bit0, sum_i d_i * P_i, without the window combination. */
+void blst_p1s_tile_pippenger(blst_p1 *ret, const blst_p1_affine *const points[],
+                             size_t npoints, const blst_byte *const scalars[],
+                             size_t nbits, blst_limb_t *scratch, size_t bit0,
+                             size_t window);
+
+/* ------------------------------------------------------- device extensions */
+
+/* Last error message of the calling thread ("" when the last call succeeded). */
+const char *blst_hip_last_error(void);
+
+/* 1 when a gfx9 device is present and the kernels load, else 0. */
+int blst_hip_available(void);
+
+/* MSM over inputs already resident in device memory: device_points is npoints
+ * contiguous blst_p1_affine, device_scalars npoints * ceil(nbits/8) bytes.
+ * window 0 selects the tuned window.  Returns 0 on success. */
+int blst_hip_msm_device(blst_p1 *ret, const void *device_points, const void *device_scalars,
+                        size_t npoints, size_t nbits, size_t window);
+
+/* Test hook: the signed Booth digits the device computes for window size
+ * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
+ * written to out[npoints] (host).  Returns 0 on success. */
+int blst_hip_booth_digits(int32_t *out, const blst_byte *scalars, size_t npoints,
+                          size_t nbits, size_t bit0, size_t window);
+
+/* Timing of the phases of the last MSM on this thread, milliseconds:
+ * [digits, sort, accumulate, partials, reduce, fold, total]. */
+void blst_hip_last_phase_ms(double out[7]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLST_HIP_H */

@@ -1,0 +1,240 @@
+"""TEST INFRASTRUCTURE ONLY: ctypes binding of the oracle (liboracle.so).
+
+The oracle is a CPU restatement of category-blst's G1 MSM path (see
+blst_oracle.c for the file:line each function follows).  Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
+and only as the checker; the product library never touches it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+_sz = ctypes.c_size_t
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", HERE, "-B"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run make -C oracle")
+        L = ctypes.CDLL(LIB_PATH)
+        sigs = {
+            "or_fp_mul": [_u64p, _u64p, _u64p],
+            "or_fp_add": [_u64p, _u64p, _u64p],
+            "or_fp_sub": [_u64p, _u64p, _u64p],
+            "or_fp_neg": [_u64p, _u64p],
+            "or_fp_to_mont": [_u64p, _u64p],
+            "or_fp_from_mont": [_u64p, _u64p],
+            "or_fp_inv": [_u64p, _u64p],
+            "or_p1_generator": [_u64p],
+            "or_p1_from_affine": [_u64p, _u64p],
+            "or_p1_add": [_u64p, _u64p, _u64p],
+            "or_p1_double": [_u64p, _u64p],
+            "or_p1_mult": [_u64p, _u64p, _u8p, _sz],
+            "or_p1_to_affine": [_u64p, _u64p],
+            "or_p1_compress": [_u8p, _u64p],
+            "or_p1s_to_affine": [_u64p, _u64p, _sz],
+            "or_msm_pippenger": [_u64p, _u64p, _u8p, _sz, _sz, _sz],
+            "or_msm_pippenger_mt": [_u64p, _u64p, _u8p, _sz, _sz, _sz, ctypes.c_int],
+            "or_msm_tile": [_u64p, _u64p, _u8p, _sz, _sz, _sz, _sz],
+            "or_msm_naive": [_u64p, _u64p, _u8p, _sz, _sz],
+            "or_gen_scalars": [_u8p, _sz, ctypes.c_uint64, ctypes.c_int],
+            "or_gen_points": [_u64p, _sz, ctypes.c_uint64, ctypes.c_int],
+            "or_gen_srs": [_u64p, _sz, ctypes.c_uint64, ctypes.c_int],
+            "or_apply_edge_cases": [_u64p, _u8p, _sz, ctypes.c_uint64],
+        }
+        for name, args in sigs.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = None
+        for name, args, res in [
+            ("or_fp_sgn0_pty_mont", [_u64p], ctypes.c_uint64),
+            ("or_p1_is_equal", [_u64p, _u64p], ctypes.c_int),
+            ("or_p1_affine_on_curve", [_u64p], ctypes.c_int),
+            ("or_pippenger_window_size", [_sz], _sz),
+            ("or_pippenger_scratch_sizeof", [_sz], _sz),
+            ("or_pippenger_window_scratch_sizeof", [_sz], _sz),
+            ("or_booth_digit", [_u8p, _sz, _sz, _sz], ctypes.c_uint64),
+        ]:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def u64p(a):
+    assert a.dtype == np.uint64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_u64p)
+
+
+def u8p(a):
+    assert a.dtype == np.uint8 and a.flags.c_contiguous
+    return a.ctypes.data_as(_u8p)
+
+
+# ---- field helpers on python ints (values, not Montgomery) -----------------
+
+def limbs(x, n=6):
+    return np.array([(x >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(n)], dtype=np.uint64)
+
+
+def value(a):
+    return sum(int(v) << (64 * i) for i, v in enumerate(a))
+
+
+def fp_mul(a, b):
+    r = np.zeros(6, np.uint64)
+    lib().or_fp_mul(u64p(r), u64p(np.ascontiguousarray(a)), u64p(np.ascontiguousarray(b)))
+    return r
+
+
+def fp_binop(name, a, b):
+    r = np.zeros(6, np.uint64)
+    getattr(lib(), name)(u64p(r), u64p(np.ascontiguousarray(a)), u64p(np.ascontiguousarray(b)))
+    return r
+
+
+def fp_unop(name, a):
+    r = np.zeros(6, np.uint64)
+    getattr(lib(), name)(u64p(r), u64p(np.ascontiguousarray(a)))
+    return r
+
+
+# ---- points ----------------------------------------------------------------
+
+def generator_affine():
+    g = np.zeros(12, np.uint64)
+    lib().or_p1_generator(u64p(g))
+    return g
+
+
+def from_affine(a):
+    r = np.zeros(18, np.uint64)
+    lib().or_p1_from_affine(u64p(r), u64p(np.ascontiguousarray(a, dtype=np.uint64)))
+    return r
+
+
+def p1_add(a, b):
+    r = np.zeros(18, np.uint64)
+    lib().or_p1_add(u64p(r), u64p(np.ascontiguousarray(a)), u64p(np.ascontiguousarray(b)))
+    return r
+
+
+def p1_double(a):
+    r = np.zeros(18, np.uint64)
+    lib().or_p1_double(u64p(r), u64p(np.ascontiguousarray(a)))
+    return r
+
+
+def p1_mult(p, scalar_bytes, nbits):
+    r = np.zeros(18, np.uint64)
+    s = np.frombuffer(bytes(scalar_bytes), dtype=np.uint8).copy()
+    lib().or_p1_mult(u64p(r), u64p(np.ascontiguousarray(p)), u8p(s), nbits)
+    return r
+
+
+def to_affine(p):
+    r = np.zeros(12, np.uint64)
+    lib().or_p1_to_affine(u64p(r), u64p(np.ascontiguousarray(p)))
+    return r
+
+
+def compress(p):
+    out = np.zeros(48, np.uint8)
+    lib().or_p1_compress(u8p(out), u64p(np.ascontiguousarray(p, dtype=np.uint64)))
+    return out.tobytes()
+
+
+def is_equal(a, b):
+    return bool(lib().or_p1_is_equal(u64p(np.ascontiguousarray(a)), u64p(np.ascontiguousarray(b))))
+
+
+def on_curve(a):
+    return bool(lib().or_p1_affine_on_curve(u64p(np.ascontiguousarray(a, dtype=np.uint64))))
+
+
+# ---- MSM -------------------------------------------------------------------
+
+def window_size(n):
+    return lib().or_pippenger_window_size(n)
+
+
+def scratch_sizeof(n):
+    return lib().or_pippenger_scratch_sizeof(n)
+
+
+def window_scratch_sizeof(w):
+    return lib().or_pippenger_window_scratch_sizeof(w)
+
+
+def booth_digit(scalar_bytes, nbits, bit0, window):
+    s = np.frombuffer(bytes(scalar_bytes), dtype=np.uint8).copy()
+    return lib().or_booth_digit(u8p(s), nbits, bit0, window)
+
+
+def msm_pippenger(points, scalars, n, nbits, window=0, threads=1):
+    """points: (n,12) uint64 affine Montgomery; scalars: (n, nbytes) uint8."""
+    r = np.zeros(18, np.uint64)
+    pts = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
+    if threads > 1:
+        lib().or_msm_pippenger_mt(u64p(r), u64p(pts), u8p(sc), n, nbits, window, threads)
+    else:
+        lib().or_msm_pippenger(u64p(r), u64p(pts), u8p(sc), n, nbits, window)
+    return r
+
+
+def msm_tile(points, scalars, n, nbits, bit0, window):
+    r = np.zeros(18, np.uint64)
+    pts = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
+    lib().or_msm_tile(u64p(r), u64p(pts), u8p(sc), n, nbits, bit0, window)
+    return r
+
+
+def msm_naive(points, scalars, n, nbits):
+    r = np.zeros(18, np.uint64)
+    pts = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
+    lib().or_msm_naive(u64p(r), u64p(pts), u8p(sc), n, nbits)
+    return r
+
+
+# ---- synthetic inputs ------------------------------------------------------
+
+def gen_scalars(n, seed, low64_pct=0):
+    out = np.zeros((n, 32), np.uint8)
+    lib().or_gen_scalars(u8p(out.reshape(-1)), n, seed, low64_pct)
+    return out
+
+
+def gen_points(n, seed, threads=8):
+    out = np.zeros((n, 12), np.uint64)
+    lib().or_gen_points(u64p(out.reshape(-1)), n, seed, threads)
+    return out
+
+
+def gen_srs(n, seed, threads=8):
+    out = np.zeros((n, 12), np.uint64)
+    lib().or_gen_srs(u64p(out.reshape(-1)), n, seed, threads)
+    return out
+
+
+def apply_edge_cases(points, scalars, seed):
+    n = points.shape[0]
+    assert points.flags.c_contiguous and scalars.flags.c_contiguous and scalars.shape[1] == 32
+    lib().or_apply_edge_cases(u64p(points.reshape(-1)), u8p(scalars.reshape(-1)), n, seed)

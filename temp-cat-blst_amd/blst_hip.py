@@ -1,0 +1,154 @@
+"""ctypes binding of libblst_hip.so, the gfx950 G1 MSM backend.
+
+The product is the C ABI in include/blst_hip.h; this module only forwards
+numpy buffers to it for tests and benchmarks.  It never falls back to a CPU
+implementation: if the library or a GPU is missing, calls raise.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libblst_hip.so")
+
+# symbol -> (argtypes, restype); the full exported surface of include/blst_hip.h
+_vp = ctypes.c_void_p
+_sz = ctypes.c_size_t
+SIGNATURES = {
+    "blst_p1s_mult_pippenger_scratch_sizeof": ([_sz], _sz),
+    "blst_p1s_mult_pippenger_window_scratch_sizeof": ([_sz], _sz),
+    "blst_p1s_mult_pippenger": ([_vp, _vp, _sz, _vp, _sz, _vp], None),
+    "blst_p1s_mult_pippenger_window": ([_vp, _vp, _sz, _vp, _sz, _vp, _sz], None),
+    "blst_p1s_tile_pippenger": ([_vp, _vp, _sz, _vp, _sz, _vp, _sz, _sz], None),
+    "blst_hip_last_error": ([], ctypes.c_char_p),
+    "blst_hip_available": ([], ctypes.c_int),
+    "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
+    "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
+    "blst_hip_last_phase_ms": ([_vp], None),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library (no GPU call is made here)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libblst_hip.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def last_error():
+    return load().blst_hip_last_error().decode()
+
+
+def available():
+    return bool(load().blst_hip_available())
+
+
+def scratch_sizeof(n):
+    return load().blst_p1s_mult_pippenger_scratch_sizeof(n)
+
+
+def window_scratch_sizeof(w):
+    return load().blst_p1s_mult_pippenger_window_scratch_sizeof(w)
+
+
+class Descriptors:
+    """Pointer arrays in the reference's descriptor convention.
+
+    mode "contiguous": {base, NULL} (the Rust binding's form);
+    mode "exact": n explicit pointers, no sentinel;
+    mode "mixed": explicit pointers for the first k entries, then NULL.
+    """
+
+    def __init__(self, points, scalars, mode="contiguous", k=1):
+        self.points = np.ascontiguousarray(points, dtype=np.uint64)
+        self.scalars = np.ascontiguousarray(scalars, dtype=np.uint8)
+        n = self.points.shape[0]
+        nb = self.scalars.shape[1]
+        pbase, sbase = self.points.ctypes.data, self.scalars.ctypes.data
+        if mode == "contiguous":
+            # entries past the first NULL are never read
+            pp, ss = [pbase, 0], [sbase, 0]
+        elif mode == "exact":
+            pp = [pbase + 96 * i for i in range(n)]
+            ss = [sbase + nb * i for i in range(n)]
+        elif mode == "mixed":
+            pp = [pbase + 96 * i if i < k else 0 for i in range(n)]
+            ss = [sbase + nb * i if i < k else 0 for i in range(n)]
+        else:
+            raise ValueError(mode)
+        self.pp = np.array(pp, dtype=np.uint64)
+        self.ss = np.array(ss, dtype=np.uint64)
+
+
+def p1s_mult_pippenger(points, scalars, nbits, window=0, scratch=True, mode="contiguous", k=1):
+    """G1 MSM through the drop-in entry point; returns the 18-limb blst_p1."""
+    L = load()
+    n = points.shape[0]
+    d = Descriptors(points, scalars, mode, k)
+    ret = np.zeros(18, np.uint64)
+    sz = (window_scratch_sizeof(window) if window else scratch_sizeof(n)) or 8
+    scr = np.zeros((sz + 7) // 8, np.uint64) if scratch else None
+    sp = _ptr(scr) if scratch else None
+    if window:
+        L.blst_p1s_mult_pippenger_window(_ptr(ret), _ptr(d.pp), n, _ptr(d.ss), nbits, sp, window)
+    else:
+        L.blst_p1s_mult_pippenger(_ptr(ret), _ptr(d.pp), n, _ptr(d.ss), nbits, sp)
+    err = last_error()
+    if err and n > 0:
+        raise RuntimeError("blst_p1s_mult_pippenger: " + err)
+    return ret
+
+
+def p1s_tile_pippenger(points, scalars, nbits, bit0, window):
+    L = load()
+    n = points.shape[0]
+    d = Descriptors(points, scalars, "contiguous")
+    ret = np.zeros(18, np.uint64)
+    scr = np.zeros((window_scratch_sizeof(window) + 7) // 8 + 1, np.uint64)
+    L.blst_p1s_tile_pippenger(_ptr(ret), _ptr(d.pp), n, _ptr(d.ss), nbits, _ptr(scr), bit0, window)
+    err = last_error()
+    if err:
+        raise RuntimeError("blst_p1s_tile_pippenger: " + err)
+    return ret
+
+
+def booth_digits(scalars, nbits, bit0, window):
+    L = load()
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8)
+    out = np.zeros(sc.shape[0], np.int32)
+    rc = L.blst_hip_booth_digits(_ptr(out), _ptr(sc), sc.shape[0], nbits, bit0, window)
+    if rc:
+        raise RuntimeError("blst_hip_booth_digits: " + last_error())
+    return out
+
+
+def msm_device(d_points_ptr, d_scalars_ptr, n, nbits, window=0):
+    """MSM over device-resident inputs given as raw device addresses."""
+    L = load()
+    ret = np.zeros(18, np.uint64)
+    rc = L.blst_hip_msm_device(_ptr(ret), ctypes.c_void_p(d_points_ptr), ctypes.c_void_p(d_scalars_ptr),
+                               n, nbits, window)
+    if rc:
+        raise RuntimeError("blst_hip_msm_device: " + last_error())
+    return ret
+
+
+def last_phase_ms():
+    out = np.zeros(7, np.float64)
+    load().blst_hip_last_phase_ms(_ptr(out))
+    return dict(zip(["digits", "sort", "accumulate", "partials", "reduce", "fold", "total"], out.tolist()))

@@ -1,0 +1,165 @@
+// G1 point arithmetic on gfx950 for y^2 = x^3 + 4 over Fp.
+//
+// Formats (all field elements Montgomery, twelve 32-bit limbs):
+//   aff   (x, y); the all-zero pair is the point at infinity, as in
+//         blst_p1_affine (bindings/blst_types.h:95-152).
+//   xyzz  (X, Y, ZZ, ZZZ) with x = X/ZZ, y = Y/ZZZ; infinity is ZZ = ZZZ = 0.
+//         This is the bucket representation of the reference
+//         (blst/scalar/curve/point.h:347-384).
+//   jac   (X, Y, Z) Jacobian, the blst_p1 layout; infinity is Z = 0.
+//
+// Formula sources are the EFD entries the reference names
+// (blst/scalar/curve/point.cpp:1933-1936, 2021-2025): madd-2008-s and
+// mdbl-2008-s-1 for XYZZ+affine, add-2008-s and dbl-2008-s-1 for XYZZ+XYZZ.
+// The exceptional inputs (equal points, opposite points, infinity) take a
+// branch; they are rare in an MSM and keep the common path straight-line.
+#pragma once
+#include "fp.hip.h"
+
+namespace bls {
+
+struct aff { fp x, y; };
+struct xyzz { fp x, y, zz, zzz; };
+struct jac { fp x, y, z; };
+
+__device__ __forceinline__ void xyzz_set_inf(xyzz& p) {
+    fp_zero(p.x); fp_zero(p.y); fp_zero(p.zz); fp_zero(p.zzz);
+}
+
+__device__ __forceinline__ bool xyzz_is_inf(const xyzz& p) { return fp_is_zero(p.zz); }
+
+__device__ __forceinline__ bool aff_is_inf(const aff& a) {
+    return fp_is_zero(a.x) && fp_is_zero(a.y);
+}
+
+__device__ __forceinline__ void xyzz_from_aff(xyzz& p, const aff& a) {
+    p.x = a.x; p.y = a.y; fp_one(p.zz); fp_one(p.zzz);
+}
+
+// mdbl-2008-s-1: 2*(x, y) into XYZZ
+__device__ __noinline__ void xyzz_mdbl(xyzz& r, const aff& a) {
+    fp U, V, W, S, M, t;
+    fp_dbl(U, a.y);          // U = 2y
+    fp_sqr(V, U);            // V = U^2
+    fp_mul(W, U, V);         // W = U*V
+    fp_mul(S, a.x, V);       // S = x*V
+    fp_sqr(M, a.x);
+    fp_dbl(t, M);
+    fp_add(M, M, t);         // M = 3x^2
+    fp_sqr(r.x, M);
+    fp_dbl(t, S);
+    fp_sub(r.x, r.x, t);     // X3 = M^2 - 2S
+    fp_sub(t, S, r.x);
+    fp_mul(t, t, M);
+    fp_mul(r.y, W, a.y);
+    fp_sub(r.y, t, r.y);     // Y3 = M(S - X3) - W*y
+    r.zz = V;
+    r.zzz = W;
+}
+
+// dbl-2008-s-1: 2*P in XYZZ
+__device__ __forceinline__ void xyzz_dbl_inl(xyzz& r, const xyzz& p) {
+    if (xyzz_is_inf(p)) { r = p; return; }
+    fp U, V, W, S, M, t;
+    fp_dbl(U, p.y);
+    fp_sqr(V, U);
+    fp_mul(W, U, V);
+    fp_mul(S, p.x, V);
+    fp_sqr(M, p.x);
+    fp_dbl(t, M);
+    fp_add(M, M, t);         // M = 3X^2 (a = 0)
+    xyzz o;
+    fp_sqr(o.x, M);
+    fp_dbl(t, S);
+    fp_sub(o.x, o.x, t);
+    fp_sub(t, S, o.x);
+    fp_mul(t, t, M);
+    fp_mul(o.y, W, p.y);
+    fp_sub(o.y, t, o.y);
+    fp_mul(o.zz, V, p.zz);
+    fp_mul(o.zzz, W, p.zzz);
+    r = o;
+}
+
+// out-of-line doubling for the rare equal-points branch of the adders
+__device__ __noinline__ void xyzz_dbl(xyzz& r, const xyzz& p) { xyzz_dbl_inl(r, p); }
+
+// acc += a (a affine, not infinity): madd-2008-s, 8M + 2S
+__device__ __forceinline__ void xyzz_madd(xyzz& acc, const aff& a) {
+    if (xyzz_is_inf(acc)) { xyzz_from_aff(acc, a); return; }
+    fp P, R;
+    fp_mul(P, a.x, acc.zz);
+    fp_mul(R, a.y, acc.zzz);
+    fp_sub(P, P, acc.x);     // P = U2 - X1
+    fp_sub(R, R, acc.y);     // R = S2 - Y1
+    if (fp_is_zero(P)) {
+        if (fp_is_zero(R)) xyzz_mdbl(acc, a);
+        else xyzz_set_inf(acc);
+        return;
+    }
+    fp PP, PPP, Q, t;
+    fp_sqr(PP, P);
+    fp_mul(PPP, PP, P);
+    fp_mul(Q, acc.x, PP);
+    fp_sqr(t, R);
+    fp_sub(t, t, PPP);
+    fp_sub(t, t, Q);
+    fp_sub(acc.x, t, Q);     // X3 = R^2 - PPP - 2Q
+    fp_sub(Q, Q, acc.x);
+    fp_mul(Q, Q, R);
+    fp_mul(t, acc.y, PPP);
+    fp_sub(acc.y, Q, t);     // Y3 = R(Q - X3) - Y1*PPP
+    fp_mul(acc.zz, acc.zz, PP);
+    fp_mul(acc.zzz, acc.zzz, PPP);
+}
+
+// r = p + q: add-2008-s, 12M + 2S
+__device__ __forceinline__ void xyzz_add(xyzz& r, const xyzz& p, const xyzz& q) {
+    if (xyzz_is_inf(q)) { r = p; return; }
+    if (xyzz_is_inf(p)) { r = q; return; }
+    fp U1, S1, P, R;
+    fp_mul(U1, p.x, q.zz);
+    fp_mul(S1, p.y, q.zzz);
+    fp_mul(P, q.x, p.zz);
+    fp_mul(R, q.y, p.zzz);
+    fp_sub(P, P, U1);
+    fp_sub(R, R, S1);
+    if (fp_is_zero(P)) {
+        if (fp_is_zero(R)) xyzz_dbl(r, p);
+        else xyzz_set_inf(r);
+        return;
+    }
+    fp PP, PPP, Q, t;
+    xyzz o;
+    fp_sqr(PP, P);
+    fp_mul(PPP, PP, P);
+    fp_mul(Q, U1, PP);
+    fp_sqr(t, R);
+    fp_sub(t, t, PPP);
+    fp_sub(t, t, Q);
+    fp_sub(o.x, t, Q);
+    fp_sub(Q, Q, o.x);
+    fp_mul(Q, Q, R);
+    fp_mul(t, S1, PPP);
+    fp_sub(o.y, Q, t);
+    fp_mul(t, p.zz, q.zz);
+    fp_mul(o.zz, t, PP);
+    fp_mul(t, p.zzz, q.zzz);
+    fp_mul(o.zzz, t, PPP);
+    r = o;
+}
+
+__device__ __forceinline__ void xyzz_neg(xyzz& r, const xyzz& p) {
+    r = p;
+    fp_neg(r.y, p.y);
+}
+
+// Jacobian (X, Y, Z) with Z = ZZ:  X = x*ZZ^2 = X'*ZZ, Y = y*ZZ^3 = Y'*ZZZ
+// (Pointxyzz::to_Jacobian, point.cpp:2109-2114)
+__device__ __forceinline__ void xyzz_to_jac(jac& r, const xyzz& p) {
+    fp_mul(r.x, p.x, p.zz);
+    fp_mul(r.y, p.y, p.zzz);
+    r.z = p.zz;
+}
+
+}  // namespace bls

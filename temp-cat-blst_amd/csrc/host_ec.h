@@ -1,0 +1,177 @@
+// Host-side BLS12-381 G1 arithmetic for the latency-bound tail of an MSM:
+// folding the per-window sums W_j into sum_j 2^(c*j) W_j (c*D doublings in a
+// row, which a single CPU core finishes in tens of microseconds while one GPU
+// lane would need milliseconds), and the 144-byte blst_p1 output.
+//
+// Montgomery arithmetic on six 64-bit limbs, R = 2^384, the layout of
+// blst_fp (bindings/blst_types.h:95-152).  Formulas: add-2008-s and
+// dbl-2008-s-1 in XYZZ coordinates, Pointxyzz::to_Jacobian
+// (blst/scalar/curve/point.cpp:1939-2001, 2109-2114).
+#pragma once
+#include <stdint.h>
+#include <string.h>
+
+namespace hostec {
+
+typedef unsigned __int128 u128;
+
+struct fp6 { uint64_t l[6]; };
+struct xyzz6 { fp6 x, y, zz, zzz; };
+struct jac6 { fp6 x, y, z; };
+
+static const uint64_t kP[6] = {0xb9feffffffffaaabULL, 0x1eabfffeb153ffffULL, 0x6730d2a0f6b0f624ULL,
+                               0x64774b84f38512bfULL, 0x4b1ba7b6434bacd7ULL, 0x1a0111ea397fe69aULL};
+static const uint64_t kN0 = 0x89f3fffcfffcfffdULL;
+
+inline bool is_zero(const fp6& a) {
+    return (a.l[0] | a.l[1] | a.l[2] | a.l[3] | a.l[4] | a.l[5]) == 0;
+}
+
+inline void sub_p_if_ge(fp6& r, const uint64_t t[7]) {
+    uint64_t d[6], br = 0;
+    for (int j = 0; j < 6; j++) {
+        u128 s = (u128)t[j] - kP[j] - br;
+        d[j] = (uint64_t)s;
+        br = (uint64_t)(s >> 64) & 1;
+    }
+    const bool keep = t[6] < br;
+    for (int j = 0; j < 6; j++) r.l[j] = keep ? t[j] : d[j];
+}
+
+inline void mul(fp6& r, const fp6& a, const fp6& b) {
+    uint64_t t[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 6; i++) {
+        uint64_t c = 0;
+        for (int j = 0; j < 6; j++) {
+            u128 s = (u128)a.l[j] * b.l[i] + t[j] + c;
+            t[j] = (uint64_t)s;
+            c = (uint64_t)(s >> 64);
+        }
+        u128 s = (u128)t[6] + c;
+        t[6] = (uint64_t)s;
+        const uint64_t hi = (uint64_t)(s >> 64);
+        const uint64_t m = t[0] * kN0;
+        s = (u128)m * kP[0] + t[0];
+        c = (uint64_t)(s >> 64);
+        for (int j = 1; j < 6; j++) {
+            s = (u128)m * kP[j] + t[j] + c;
+            t[j - 1] = (uint64_t)s;
+            c = (uint64_t)(s >> 64);
+        }
+        s = (u128)t[6] + c;
+        t[5] = (uint64_t)s;
+        t[6] = hi + (uint64_t)(s >> 64);
+    }
+    sub_p_if_ge(r, t);
+}
+
+inline void add(fp6& r, const fp6& a, const fp6& b) {
+    uint64_t t[7], c = 0;
+    for (int j = 0; j < 6; j++) {
+        u128 s = (u128)a.l[j] + b.l[j] + c;
+        t[j] = (uint64_t)s;
+        c = (uint64_t)(s >> 64);
+    }
+    t[6] = c;
+    sub_p_if_ge(r, t);
+}
+
+inline void sub(fp6& r, const fp6& a, const fp6& b) {
+    uint64_t d[6], br = 0;
+    for (int j = 0; j < 6; j++) {
+        u128 s = (u128)a.l[j] - b.l[j] - br;
+        d[j] = (uint64_t)s;
+        br = (uint64_t)(s >> 64) & 1;
+    }
+    const uint64_t mask = 0 - br;
+    uint64_t c = 0;
+    for (int j = 0; j < 6; j++) {
+        u128 s = (u128)d[j] + (kP[j] & mask) + c;
+        r.l[j] = (uint64_t)s;
+        c = (uint64_t)(s >> 64);
+    }
+}
+
+inline void set_inf(xyzz6& p) { memset(&p, 0, sizeof(p)); }
+inline bool is_inf(const xyzz6& p) { return is_zero(p.zz); }
+
+inline void dbl(xyzz6& r, const xyzz6& p) {
+    if (is_inf(p)) { r = p; return; }
+    fp6 U, V, W, S, M, t;
+    add(U, p.y, p.y);
+    mul(V, U, U);
+    mul(W, U, V);
+    mul(S, p.x, V);
+    mul(M, p.x, p.x);
+    add(t, M, M);
+    add(M, M, t);
+    xyzz6 o;
+    mul(o.x, M, M);
+    add(t, S, S);
+    sub(o.x, o.x, t);
+    sub(t, S, o.x);
+    mul(t, t, M);
+    mul(o.y, W, p.y);
+    sub(o.y, t, o.y);
+    mul(o.zz, V, p.zz);
+    mul(o.zzz, W, p.zzz);
+    r = o;
+}
+
+inline void addp(xyzz6& r, const xyzz6& p, const xyzz6& q) {
+    if (is_inf(q)) { r = p; return; }
+    if (is_inf(p)) { r = q; return; }
+    fp6 U1, S1, P, R;
+    mul(U1, p.x, q.zz);
+    mul(S1, p.y, q.zzz);
+    mul(P, q.x, p.zz);
+    mul(R, q.y, p.zzz);
+    sub(P, P, U1);
+    sub(R, R, S1);
+    if (is_zero(P)) {
+        if (is_zero(R)) dbl(r, p);
+        else set_inf(r);
+        return;
+    }
+    fp6 PP, PPP, Q, t;
+    xyzz6 o;
+    mul(PP, P, P);
+    mul(PPP, PP, P);
+    mul(Q, U1, PP);
+    mul(t, R, R);
+    sub(t, t, PPP);
+    sub(t, t, Q);
+    sub(o.x, t, Q);
+    sub(Q, Q, o.x);
+    mul(Q, Q, R);
+    mul(t, S1, PPP);
+    sub(o.y, Q, t);
+    mul(t, p.zz, q.zz);
+    mul(o.zz, t, PP);
+    mul(t, p.zzz, q.zzz);
+    mul(o.zzz, t, PPP);
+    r = o;
+}
+
+// Pointxyzz::to_Jacobian; infinity maps to the all-zero blst_p1 the
+// reference's early returns produce (c_api.cpp:1390-1408).
+inline void to_jac(jac6& r, const xyzz6& p) {
+    if (is_inf(p)) { memset(&r, 0, sizeof(r)); return; }
+    mul(r.x, p.x, p.zz);
+    mul(r.y, p.y, p.zzz);
+    r.z = p.zz;
+}
+
+// sum_j 2^(c*j) w[j], Horner from the top window
+inline void fold_windows(xyzz6& out, const xyzz6* w, int nwin, int c) {
+    xyzz6 acc;
+    set_inf(acc);
+    for (int j = nwin - 1; j >= 0; j--) {
+        if (j != nwin - 1)
+            for (int k = 0; k < c; k++) dbl(acc, acc);
+        addp(acc, acc, w[j]);
+    }
+    out = acc;
+}
+
+}  // namespace hostec
