@@ -1,0 +1,217 @@
+#!/usr/bin/env python3
+"""Benchmark of the gfx950 G1 MSM backend.
+
+One step = one complete G1 MSM of n = 2^20 points (window digits, sort,
+bucket accumulation, bucket reduction, window fold; the 144-byte blst_p1
+result on the host) over inputs already resident in HBM.  Workload: P_i =
+[k_i]G with k_i uniform in Fr, uniform Fr scalars (nbits = 255), with the
+edge-case mix 1% zero / 1% one / 1% r-1 scalars and 0.5% repeated points, all
+from seeded xorshift64* streams.
+
+Prints one JSON line.  The roofline object prices the dominant kernel
+(bucket accumulation) against the VALU 32-bit integer multiply-add peak of
+MI355X; the cpu_baseline object times the oracle (a C port of the reference
+path) on the host, single-threaded, on the same inputs, and its compressed
+output must equal the GPU's byte for byte.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "temp-cat-blst_amd"))
+
+import numpy as np  # noqa: E402
+
+METRIC = "G1 MSM points/s @2^20 (bit-exact); batched vs plain GPU throughput + p99 latency"
+SEED = 0x9E3779B97F4A7C15
+
+# MI355X: 256 CUs x 64 lanes x 2.4 GHz, v_mad_u64_u32 issuing at the full VALU
+# rate (one 32x32->64 multiply-add per lane per clock)
+PEAK_INT_MAD_PER_S = 256 * 64 * 2.4e9
+INT_MAD_PER_FP_MUL = 300   # 12x12 a*b + 12x12 m*p + 12 m_i, Montgomery on 32-bit limbs
+
+
+def algorithmic_fp_muls(n):
+    """W(n) = 6 n ceil(256/c) + 28 ceil(256/c) 2^(c-1), minimised over c in [4, 21].
+
+    6 = one batch-affine bucket addition, 28 = the two XYZZ additions per
+    bucket of the running sum.  Returns (W, c*, accumulation share)."""
+    best = None
+    for c in range(4, 22):
+        d = math.ceil(256 / c)
+        w = 6 * n * d + 28 * d * 2 ** (c - 1)
+        if best is None or w < best[0]:
+            best = (w, c, 6 * n * d)
+    return best
+
+
+def load_traffic():
+    """HBM bytes per k_accumulate launch, from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary (copy-kernel calibrated)."""
+    path = os.path.join(ROOT, "profiles", "msm_2pow20_hbm_traffic.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("hbm_bytes_per_launch k_accumulate:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(points, scalars, n, gpu_compressed):
+    """The oracle (C port of blst_p1s_mult_pippenger) on the host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    t = time.perf_counter()
+    r = oracle.msm_pippenger(points, scalars, n, 255, threads=1)
+    dt = time.perf_counter() - t
+    cpu_bytes = oracle.compress(r)
+    threads = min(16, os.cpu_count() or 1)
+    t = time.perf_counter()
+    r_mt = oracle.msm_pippenger(points, scalars, n, 255, threads=threads)
+    dt_mt = time.perf_counter() - t
+    assert oracle.compress(r_mt) == cpu_bytes
+    return {
+        "value": n / dt,
+        "unit": "points/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": "one full MSM of the same n=2^20 inputs, oracle Pippenger (reference window %d), 1 thread, %.2f s"
+        % (oracle.window_size(n), dt),
+        "all_cores": {"value": n / dt_mt, "threads": threads, "seconds": dt_mt},
+        "bit_exact_vs_gpu": cpu_bytes == gpu_compressed,
+        "compressed": cpu_bytes.hex(),
+    }
+
+
+def copy_calibration(nbytes, reps=3):
+    """Device-to-device copies of a known size: the HIP runtime's blit kernel
+    reads and writes nbytes each, so FETCH_SIZE / WRITE_SIZE of these
+    dispatches calibrate the counters against true HBM bytes."""
+    import ctypes
+    import blst_hip as bh
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    src = bh.DeviceBuffer(np.zeros(nbytes, np.uint8))
+    dst = bh.DeviceBuffer(np.zeros(nbytes, np.uint8))
+    for _ in range(reps):
+        rc = hip.hipMemcpy(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), ctypes.c_size_t(nbytes), 3)
+        assert rc == 0, rc
+    hip.hipDeviceSynchronize()
+    src.close()
+    dst.close()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--window", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run device-to-device copies of BYTES (HBM counter calibration)")
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the MSM backend runs on a single device "
+              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    import blst_hip as bh
+
+    if not bh.available():
+        raise SystemExit("no gfx950 device visible")
+    n = args.n
+    t = time.perf_counter()
+    points = bh.synth_points(n, SEED ^ 1)
+    scalars = bh.synth_scalars(n, SEED ^ 2)
+    bh.synth_edge_cases(points, scalars, SEED ^ 3)
+    gen_s = time.perf_counter() - t
+    print("inputs ready in %.1f s" % gen_s, file=sys.stderr, flush=True)
+    dp = bh.DeviceBuffer(points)
+    ds = bh.DeviceBuffer(scalars)
+
+    r = None
+    for _ in range(args.warmup):
+        r = bh.msm_device(dp.ptr, ds.ptr, n, 255, args.window)
+    acc_ms, phases = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        r = bh.msm_device(dp.ptr, ds.ptr, n, 255, args.window)
+        ph = bh.last_phase_ms()
+        acc_ms.append(ph["accumulate"])
+        phases.append(ph)
+    t1 = time.perf_counter()
+    elapsed = t1 - t0
+    ms_per_step = elapsed / args.steps * 1e3
+    value = n * args.steps / elapsed
+    print("timed %d steps: %.3f ms/step" % (args.steps, ms_per_step), file=sys.stderr, flush=True)
+
+    W, cstar, W_acc = algorithmic_fp_muls(n)
+    acc_avg_s = float(np.mean(acc_ms)) / 1e3
+    achieved = W_acc * INT_MAD_PER_FP_MUL / acc_avg_s
+    phase_avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "points/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {
+            "workload": "plain G1 MSM, n=2^20 affine points [k_i]G, uniform Fr scalars (nbits=255) "
+                        "with 1% zero / 1% one / 1% r-1 scalars and 0.5% repeated points; inputs resident in HBM",
+            "n": n,
+            "nbits": 255,
+            "window": args.window or None,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "valu",
+            "kernel": "k_accumulate",
+            "achieved": achieved / 1e12,
+            "peak": PEAK_INT_MAD_PER_S / 1e12,
+            "unit": "TOP/s",
+            "frac": achieved / PEAK_INT_MAD_PER_S,
+            "traffic": load_traffic(),
+            "algorithmic_work": "6*n*ceil(256/c*) Fp-mul x 300 int32 mad, c*=%d: %.4g mad per launch"
+                                % (cstar, W_acc * INT_MAD_PER_FP_MUL),
+            "avg_launch_ms": acc_avg_s * 1e3,
+        },
+        "msm_roofline": {
+            "W_fp_mul": W,
+            "c_star": cstar,
+            "fp_mul_per_s": W / (ms_per_step / 1e3),
+            "frac": W * INT_MAD_PER_FP_MUL / (ms_per_step / 1e3) / PEAK_INT_MAD_PER_S,
+        },
+        "phases_ms": phase_avg,
+    }
+    gpu_bytes = bh.compress(r)
+    out["compressed"] = gpu_bytes.hex()
+    if not args.no_cpu:
+        cb = cpu_baseline(points, scalars, n, gpu_bytes)
+        out["cpu_baseline"] = cb
+        out["bit_exact"] = cb["bit_exact_vs_gpu"]
+    dp.close()
+    ds.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -83,11 +83,37 @@ int blst_hip_available(void);
 int blst_hip_msm_device(blst_p1 *ret, const void *device_points, const void *device_scalars,
                         size_t npoints, size_t nbits, size_t window);
 
+/* Device memory for callers without their own allocator (benchmarks,
+ * resident SRS bases): copy `bytes` from host memory into a new device
+ * buffer on the current device.  Returns NULL on failure. */
+void *blst_hip_device_upload(const void *host, size_t bytes);
+void blst_hip_device_free(void *device_ptr);
+
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
  * written to out[npoints] (host).  Returns 0 on success. */
 int blst_hip_booth_digits(int32_t *out, const blst_byte *scalars, size_t npoints,
                           size_t nbits, size_t bit0, size_t window);
+
+/* [k_i]G for each scalar (generator multiples, affine, all-zero for k = 0),
+ * computed on the device from a comb table of G: the synthetic bases of the
+ * benchmark workloads and KZG setup points [tau^i]G.  Returns 0 on success. */
+int blst_hip_p1s_mult_generator(blst_p1_affine *out, const blst_byte *scalars, size_t npoints,
+                                size_t nbits);
+
+/* Seeded workload generators (xorshift64*): n uniform elements of Fr as
+ * 32-byte little-endian scalars (low64_pct percent keep only their low 64
+ * bits); the KZG exponents tau^i mod r with tau drawn from the seed; and the
+ * edge-case mix (1% zero, 1% one, 1% r-1 scalars, 0.5% repeated points)
+ * applied in place at seeded distinct positions. */
+void blst_hip_synth_scalars(blst_byte *out32, size_t n, uint64_t seed, int low64_pct);
+void blst_hip_synth_srs_scalars(blst_byte *out32, size_t n, uint64_t seed);
+void blst_hip_synth_edge_cases(void *points96, blst_byte *scalars32, size_t n, uint64_t seed);
+
+/* 48-byte ZCash compressed encoding of a G1 Jacobian point, the format of
+ * blst_p1_compress (bindings/blst_scalar.h:609, blst/c_api.cpp:1618-1621),
+ * computed on the host by this library. */
+void blst_hip_p1_compress(blst_byte out[48], const blst_p1 *in);
 
 /* Timing of the phases of the last MSM on this thread, milliseconds:
  * [digits, sort, accumulate, partials, reduce, fold, total]. */

@@ -26,6 +26,13 @@ SIGNATURES = {
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_last_phase_ms": ([_vp], None),
+    "blst_hip_device_upload": ([_vp, _sz], _vp),
+    "blst_hip_device_free": ([_vp], None),
+    "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
+    "blst_hip_synth_scalars": ([_vp, _sz, ctypes.c_uint64, ctypes.c_int], None),
+    "blst_hip_synth_srs_scalars": ([_vp, _sz, ctypes.c_uint64], None),
+    "blst_hip_synth_edge_cases": ([_vp, _vp, _sz, ctypes.c_uint64], None),
+    "blst_hip_p1_compress": ([_vp, _vp], None),
 }
 
 _lib = None
@@ -146,6 +153,62 @@ def msm_device(d_points_ptr, d_scalars_ptr, n, nbits, window=0):
     if rc:
         raise RuntimeError("blst_hip_msm_device: " + last_error())
     return ret
+
+
+def compress(p1):
+    """48-byte compressed encoding of an 18-limb blst_p1 (library code)."""
+    p = np.ascontiguousarray(p1, dtype=np.uint64)
+    out = np.zeros(48, np.uint8)
+    load().blst_hip_p1_compress(_ptr(out), _ptr(p))
+    return out.tobytes()
+
+
+def generator_multiples(scalars, nbits=255):
+    """[k_i]G on the device, returned as (n, 12) uint64 affine limbs."""
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8)
+    out = np.zeros((sc.shape[0], 12), np.uint64)
+    rc = load().blst_hip_p1s_mult_generator(_ptr(out), _ptr(sc), sc.shape[0], nbits)
+    if rc:
+        raise RuntimeError("blst_hip_p1s_mult_generator: " + last_error())
+    return out
+
+
+def synth_scalars(n, seed, low64_pct=0):
+    out = np.zeros((n, 32), np.uint8)
+    load().blst_hip_synth_scalars(_ptr(out), n, seed, low64_pct)
+    return out
+
+
+def synth_srs_scalars(n, seed):
+    out = np.zeros((n, 32), np.uint8)
+    load().blst_hip_synth_srs_scalars(_ptr(out), n, seed)
+    return out
+
+
+def synth_edge_cases(points, scalars, seed):
+    assert points.flags.c_contiguous and scalars.flags.c_contiguous
+    load().blst_hip_synth_edge_cases(_ptr(points), _ptr(scalars), points.shape[0], seed)
+
+
+def synth_points(n, seed):
+    """P_i = [k_i]G with k_i uniform in Fr from the seeded stream."""
+    return generator_multiples(synth_scalars(n, seed))
+
+
+class DeviceBuffer:
+    """A device copy of a host numpy array (freed on close)."""
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host)
+        self.nbytes = host.nbytes
+        self.ptr = load().blst_hip_device_upload(_ptr(host), host.nbytes)
+        if not self.ptr:
+            raise RuntimeError("blst_hip_device_upload: " + last_error())
+
+    def close(self):
+        if self.ptr:
+            load().blst_hip_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
 
 
 def last_phase_ms():

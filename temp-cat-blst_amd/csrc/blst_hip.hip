@@ -58,6 +58,8 @@ struct Ctx {
     hipStream_t stream = nullptr;
     std::mutex mu;
     DevBuf pts, scl, ent, rank, hist, off, skey, sval, buckets, slot_key, slots, H, scan_tmp, dbg;
+    DevBuf gtable, gen_tmp, gen_out;     // comb table of G, fixed-base scratch
+    bool gtable_ready = false;
     hipEvent_t ev[8] = {};
 };
 
@@ -195,9 +197,8 @@ void run_device(Ctx& cx, const Plan& pl, const void* d_points, const void* d_sca
                        cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off,
                        cx.skey.as<uint32_t>(), cx.sval.as<uint32_t>());
     HIPCK(hipGetLastError());
-    HIPCK(hipEventRecord(cx.ev[2], st));
-
     HIPCK(hipMemsetAsync(cx.buckets.p, 0, (size_t)nkeys * sizeof(xyzz), st));
+    HIPCK(hipEventRecord(cx.ev[2], st));
     hipLaunchKernelGGL(k_accumulate, dim3(div_up(nchunks, 256)), dim3(256), 0, st,
                        cx.skey.as<const uint32_t>(), cx.sval.as<const uint32_t>(), off + nkeys,
                        (const uint4*)d_points, cx.buckets.as<xyzz>(), cx.slot_key.as<uint32_t>(),
@@ -318,6 +319,36 @@ int host_msm(blst_p1* ret, const blst_p1_affine* const points[], size_t npoints,
     g_phase[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     g_err.clear();
     return 0;
+}
+
+// Comb table T[m][j-1] = [j * 2^(8m)] G, m < 32, j < 256, affine, on the
+// device; built once per context on the host (8160 additions) with one
+// batched inversion.
+void ensure_gtable(Ctx& cx) {
+    if (cx.gtable_ready) return;
+    static const uint64_t GX[6] = {0x5cb38790fd530c16ULL, 0x7817fc679976fff5ULL, 0x154f95c7143ba1c1ULL,
+                                   0xf0ae6acdf3d0e747ULL, 0xedce6ecc21dbf440ULL, 0x120177419e0bfb75ULL};
+    static const uint64_t GY[6] = {0xbaac93d50ce72271ULL, 0x8c22631a7918fd8eULL, 0xdd595f13570725ceULL,
+                                   0x51ac582950405194ULL, 0x0e1c8c3fad0059c0ULL, 0x0bbc3efc5008a26aULL};
+    hostec::xyzz6 base;
+    std::memcpy(base.x.l, GX, 48);
+    std::memcpy(base.y.l, GY, 48);
+    base.zz = hostec::one();
+    base.zzz = hostec::one();
+    std::vector<hostec::xyzz6> T(32 * 255);
+    for (int m = 0; m < 32; m++) {
+        hostec::xyzz6 acc = base;
+        for (int j = 1; j < 256; j++) {
+            if (j > 1) hostec::addp(acc, acc, base);
+            T[m * 255 + (j - 1)] = acc;
+        }
+        for (int k = 0; k < 8; k++) hostec::dbl(base, base);
+    }
+    std::vector<hostec::aff6> A(T.size());
+    hostec::batch_to_affine(A.data(), T.data(), T.size());
+    cx.gtable.ensure(A.size() * sizeof(hostec::aff6));
+    HIPCK(hipMemcpy(cx.gtable.p, A.data(), A.size() * sizeof(hostec::aff6), hipMemcpyHostToDevice));
+    cx.gtable_ready = true;
 }
 
 bool heads_ok(const void* const* xs, const void* const* ys, size_t n) {
@@ -457,6 +488,62 @@ int blst_hip_booth_digits(int32_t* out, const blst_byte* scalars, size_t npoints
     }
     g_err.clear();
     return 0;
+}
+
+void* blst_hip_device_upload(const void* host, size_t bytes) {
+    void* d = nullptr;
+    if (!host || bytes == 0) { g_err = "invalid arguments"; return nullptr; }
+    if (hipMalloc(&d, bytes) != hipSuccess) { g_err = "hipMalloc failed"; return nullptr; }
+    if (hipMemcpy(d, host, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        g_err = "hipMemcpy failed";
+        return nullptr;
+    }
+    g_err.clear();
+    return d;
+}
+
+void blst_hip_device_free(void* device_ptr) {
+    if (device_ptr) (void)hipFree(device_ptr);
+}
+
+int blst_hip_p1s_mult_generator(blst_p1_affine* out, const blst_byte* scalars, size_t npoints,
+                                size_t nbits) {
+    if (npoints == 0) { g_err.clear(); return 0; }
+    if (!out || !scalars || nbits == 0 || nbits > 256 || npoints >= 0x7fffffffULL) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        ensure_gtable(cx);
+        const size_t nbytes = (nbits + 7) / 8;
+        const uint32_t Q = 64;
+        cx.scl.ensure(npoints * nbytes + 16);
+        cx.gen_tmp.ensure(npoints * sizeof(xyzz));
+        cx.gen_out.ensure(npoints * sizeof(aff));
+        HIPCK(hipMemcpyAsync(cx.scl.p, scalars, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
+        hipLaunchKernelGGL(k_fixed_base, dim3(div_up(npoints, 256)), dim3(256), 0, cx.stream,
+                           cx.scl.as<const uint8_t>(), (uint32_t)npoints, (uint32_t)nbytes, (uint32_t)nbits,
+                           cx.gtable.as<const uint4>(), cx.gen_tmp.as<xyzz>());
+        hipLaunchKernelGGL(k_to_affine, dim3(div_up(div_up(npoints, Q), 256)), dim3(256), 0, cx.stream,
+                           cx.gen_tmp.as<const xyzz>(), cx.gen_out.as<aff>(), (uint32_t)npoints, Q);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(out, cx.gen_out.p, npoints * sizeof(aff), hipMemcpyDeviceToHost, cx.stream));
+        HIPCK(hipStreamSynchronize(cx.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+void blst_hip_p1_compress(blst_byte out[48], const blst_p1* in) {
+    hostec::jac6 j;
+    std::memcpy(&j, in, sizeof(j));
+    hostec::compress(out, j);
 }
 
 void blst_hip_last_phase_ms(double out[7]) {

@@ -173,4 +173,19 @@ __device__ __forceinline__ void fp_mul(fp& r, const fp& a, const fp& b) {
 
 __device__ __forceinline__ void fp_sqr(fp& r, const fp& a) { fp_mul(r, a, a); }
 
+// a^(p-2) = a^-1 (Fermat), left-to-right over the fixed exponent bits; the
+// branch is uniform across the wave.  0 maps to 0.
+__device__ __noinline__ void fp_inv(fp& r, const fp& a) {
+    constexpr uint32_t E[12] = {BLS_P0 - 2, BLS_P1, BLS_P2, BLS_P3, BLS_P4, BLS_P5,
+                                BLS_P6, BLS_P7, BLS_P8, BLS_P9, BLS_P10, BLS_P11};
+    fp acc;
+    fp_one(acc);
+    const fp base = a;
+    for (int bit = 380; bit >= 0; bit--) {
+        fp_sqr(acc, acc);
+        if ((E[bit >> 5] >> (bit & 31)) & 1u) fp_mul(acc, acc, base);
+    }
+    r = acc;
+}
+
 }  // namespace bls

@@ -92,6 +92,25 @@ inline void sub(fp6& r, const fp6& a, const fp6& b) {
     }
 }
 
+// Montgomery one, R mod p
+static const uint64_t kOne[6] = {0x760900000002fffdULL, 0xebf4000bc40c0002ULL, 0x5f48985753c758baULL,
+                                 0x77ce585370525745ULL, 0x5c071a97a256ec6dULL, 0x15f65ec3fa80e493ULL};
+
+inline fp6 one() { fp6 r; memcpy(r.l, kOne, 48); return r; }
+
+// a^(p-2) by left-to-right square-and-multiply; 0 maps to 0
+inline void inv(fp6& r, const fp6& a) {
+    uint64_t e[6];
+    memcpy(e, kP, 48);
+    e[0] -= 2;
+    fp6 acc = one();
+    for (int bit = 380; bit >= 0; bit--) {
+        mul(acc, acc, acc);
+        if ((e[bit / 64] >> (bit % 64)) & 1) mul(acc, acc, a);
+    }
+    r = acc;
+}
+
 inline void set_inf(xyzz6& p) { memset(&p, 0, sizeof(p)); }
 inline bool is_inf(const xyzz6& p) { return is_zero(p.zz); }
 
@@ -160,6 +179,71 @@ inline void to_jac(jac6& r, const xyzz6& p) {
     mul(r.x, p.x, p.zz);
     mul(r.y, p.y, p.zzz);
     r.z = p.zz;
+}
+
+// XYZZ -> affine for n points with one inversion (Montgomery's trick over
+// ZZ*ZZZ = Z^5); infinity becomes the all-zero affine pair.
+struct aff6 { fp6 x, y; };
+inline void batch_to_affine(aff6* out, const xyzz6* in, size_t n) {
+    fp6* pre = new fp6[n + 1];
+    fp6 acc = one();
+    for (size_t i = 0; i < n; i++) {
+        pre[i] = acc;
+        if (!is_inf(in[i])) {
+            fp6 t;
+            mul(t, in[i].zz, in[i].zzz);
+            mul(acc, acc, t);
+        }
+    }
+    fp6 ia;
+    inv(ia, acc);
+    for (size_t i = n; i-- > 0;) {
+        if (is_inf(in[i])) { memset(&out[i], 0, sizeof(aff6)); continue; }
+        fp6 t, it, izz, izzz;
+        mul(t, in[i].zz, in[i].zzz);
+        mul(it, ia, pre[i]);          // 1 / (ZZ*ZZZ)
+        mul(ia, ia, t);
+        mul(izz, it, in[i].zzz);      // 1 / ZZ
+        mul(izzz, it, in[i].zz);      // 1 / ZZZ
+        mul(out[i].x, in[i].x, izz);
+        mul(out[i].y, in[i].y, izzz);
+    }
+    delete[] pre;
+}
+
+// ZCash compressed encoding of a Jacobian point (the blst_p1_compress
+// format, blst/scalar/curve/point.cpp:817-902): 48 big-endian bytes of x,
+// 0x80 = compressed, 0x40 = infinity, 0x20 = y > (p-1)/2.
+inline void compress(uint8_t out[48], const jac6& p) {
+    if (is_zero(p.z)) {
+        memset(out, 0, 48);
+        out[0] = 0xc0;
+        return;
+    }
+    fp6 zi, zi2, zi3, x, y;
+    inv(zi, p.z);
+    mul(zi2, zi, zi);
+    mul(zi3, zi2, zi);
+    mul(x, p.x, zi2);
+    mul(y, p.y, zi3);
+    fp6 plain1 = {{1, 0, 0, 0, 0, 0}};
+    mul(x, x, plain1);   // out of Montgomery form
+    mul(y, y, plain1);
+    for (int i = 0; i < 48; i++) out[i] = (uint8_t)(x.l[(47 - i) / 8] >> (8 * ((47 - i) % 8)));
+    // y > (p-1)/2  <=>  2y > p - 1  <=>  2y >= p (p odd)
+    uint64_t d[7], c = 0;
+    for (int j = 0; j < 6; j++) {
+        d[j] = (y.l[j] << 1) | c;
+        c = y.l[j] >> 63;
+    }
+    d[6] = c;
+    uint64_t br = 0;
+    for (int j = 0; j < 6; j++) {
+        u128 s = (u128)d[j] - kP[j] - br;
+        br = (uint64_t)(s >> 64) & 1;
+    }
+    const bool big = d[6] >= br;
+    out[0] |= (uint8_t)(0x80 | (big ? 0x20 : 0));
 }
 
 // sum_j 2^(c*j) w[j], Horner from the top window

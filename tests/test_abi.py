@@ -1,0 +1,116 @@
+"""The C ABI boundary without a GPU: the library loads, exports every symbol
+include/blst_hip.h declares, and its host-only entry points (scratch-size
+queries, compression, workload generators) agree with the oracle."""
+import os
+import random
+import re
+
+import numpy as np
+
+import pyref
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "blst_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(blst_\w+)\s*\(", text)))
+
+
+def test_header_declares_drop_in_surface():
+    syms = declared_symbols()
+    for s in ("blst_p1s_mult_pippenger", "blst_p1s_mult_pippenger_scratch_sizeof",
+              "blst_p1s_mult_pippenger_window", "blst_p1s_mult_pippenger_window_scratch_sizeof",
+              "blst_p1s_tile_pippenger"):
+        assert s in syms
+
+
+def test_library_exports_every_declared_symbol():
+    import ctypes
+    import blst_hip
+    L = ctypes.CDLL(blst_hip.LIB_PATH)
+    missing = [s for s in declared_symbols() if not hasattr(L, s)]
+    assert not missing, missing
+    # and the Python binding binds exactly that surface
+    assert sorted(blst_hip.SIGNATURES) == declared_symbols()
+
+
+def test_scratch_sizes_match_reference(orc):
+    import blst_hip
+    for n in [0, 1, 2, 3, 5, 16, 17, 100, 255, 256, 4095, 4096, 4097, 1 << 16, 1 << 18, 1 << 20, (1 << 24) + 3]:
+        assert blst_hip.scratch_sizeof(n) == orc.scratch_sizeof(n), n
+    for w in range(0, 30):
+        assert blst_hip.window_scratch_sizeof(w) == orc.window_scratch_sizeof(w), w
+
+
+def test_compress_matches_oracle(orc):
+    import blst_hip
+    rnd = random.Random(5)
+    g = orc.from_affine(orc.generator_affine())
+    for _ in range(20):
+        k = rnd.getrandbits(255)
+        p = orc.p1_mult(g, k.to_bytes(32, "little"), 255)
+        assert blst_hip.compress(p) == orc.compress(p) == pyref.compress(pyref.mul(k, pyref.G))
+    assert blst_hip.compress(np.zeros(18, np.uint64)) == bytes([0xC0]) + bytes(47)
+
+
+def test_workload_generators_match_oracle(orc):
+    import blst_hip
+    assert np.array_equal(blst_hip.synth_scalars(300, 77), orc.gen_scalars(300, 77))
+    assert np.array_equal(blst_hip.synth_scalars(300, 78, 10), orc.gen_scalars(300, 78, 10))
+    s = blst_hip.synth_srs_scalars(5, 9)
+    assert int.from_bytes(s[0].tobytes(), "little") == 1
+    tau = int.from_bytes(s[1].tobytes(), "little")
+    for i in range(5):
+        assert int.from_bytes(s[i].tobytes(), "little") == pow(tau, i, pyref.R_ORDER)
+    pts = orc.gen_points(500, 1, threads=2)
+    sc = orc.gen_scalars(500, 2)
+    p2, s2 = pts.copy(), sc.copy()
+    orc.apply_edge_cases(pts, sc, 3)
+    blst_hip.synth_edge_cases(p2, s2, 3)
+    assert np.array_equal(pts, p2) and np.array_equal(sc, s2)
+
+
+def test_fail_closed_without_gpu_work():
+    """Argument checks that return before any device call (c_api.cpp:1390-1408)."""
+    import blst_hip
+    L = blst_hip.load()
+    ret = np.full(18, 7, np.uint64)
+    # npoints == 0 and nbits == 0 write infinity without reading inputs
+    L.blst_p1s_mult_pippenger(blst_hip._ptr(ret), None, 0, None, 255, None)
+    assert not ret.any()
+    ret[:] = 7
+    pts = np.zeros((2, 12), np.uint64)
+    sc = np.zeros((2, 32), np.uint8)
+    d = blst_hip.Descriptors(pts, sc)
+    L.blst_p1s_mult_pippenger(blst_hip._ptr(ret), blst_hip._ptr(d.pp), 2, blst_hip._ptr(d.ss), 0, None)
+    assert not ret.any()
+    for nbits, scratch in ((257, True), (255, False)):
+        ret[:] = 7
+        scr = np.zeros(4096, np.uint64)
+        L.blst_p1s_mult_pippenger(blst_hip._ptr(ret), blst_hip._ptr(d.pp), 2, blst_hip._ptr(d.ss), nbits,
+                                  blst_hip._ptr(scr) if scratch else None)
+        assert not ret.any()
+    # null heads
+    ret[:] = 7
+    nullheads = np.zeros(2, np.uint64)
+    scr = np.zeros(4096, np.uint64)
+    L.blst_p1s_mult_pippenger(blst_hip._ptr(ret), blst_hip._ptr(nullheads), 2, blst_hip._ptr(d.ss), 255,
+                              blst_hip._ptr(scr))
+    assert not ret.any()
+    # invalid windows for the window and tile variants
+    for w in (0, 25):
+        ret[:] = 7
+        L.blst_p1s_mult_pippenger_window(blst_hip._ptr(ret), blst_hip._ptr(d.pp), 2, blst_hip._ptr(d.ss), 255,
+                                         blst_hip._ptr(scr), w)
+        assert not ret.any()
+        ret[:] = 7
+        L.blst_p1s_tile_pippenger(blst_hip._ptr(ret), blst_hip._ptr(d.pp), 2, blst_hip._ptr(d.ss), 255,
+                                  blst_hip._ptr(scr), 0, w)
+        assert not ret.any()
+    # tile with bit0 >= nbits
+    ret[:] = 7
+    L.blst_p1s_tile_pippenger(blst_hip._ptr(ret), blst_hip._ptr(d.pp), 2, blst_hip._ptr(d.ss), 255,
+                              blst_hip._ptr(scr), 255, 8)
+    assert not ret.any()

@@ -1,0 +1,235 @@
+"""GPU parity: the gfx950 MSM through the C ABI against the oracle and the
+golden fixtures.  Integer work, so every comparison is byte-exact on the
+48-byte compressed encoding (the blst_p1 Jacobian representative may differ,
+as documented for the boundary)."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import pyref
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+R = pyref.R_ORDER
+
+
+def _scal(vals, nb=32):
+    return np.array([list(int(v).to_bytes(nb, "little")) for v in vals], dtype=np.uint8).reshape(len(vals), nb)
+
+
+@pytest.fixture(scope="module")
+def base_points(orc):
+    return orc.gen_points(4096, 0xB45E, threads=8)
+
+
+def test_booth_digits_match_oracle(hip, orc):
+    rnd = random.Random(21)
+    for nbits in (1, 8, 13, 64, 255, 256):
+        nb = (nbits + 7) // 8
+        ks = [rnd.getrandbits(nbits) for _ in range(300)] + [0, (1 << nbits) - 1]
+        S = _scal(ks, nb)
+        for c in (1, 2, 5, 8, 11, 16, 17, 24):
+            for bit0 in sorted({0, c, 3 * c, nbits - 1, nbits}):
+                if bit0 > nbits:
+                    continue
+                got = hip.booth_digits(S, nbits, bit0, c)
+                e = min(nbits - bit0 + 1, c)
+                for k, g in zip(ks, got):
+                    raw = orc.booth_digit(k.to_bytes(nb, "little"), nbits, bit0, c)
+                    mag, sign = raw & ((1 << e) - 1), (raw >> e) & 1
+                    assert g == (-mag if sign else mag), (nbits, c, bit0, k)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 33, 100, 513, 4096])
+def test_msm_random(hip, orc, base_points, n):
+    pts = np.ascontiguousarray(base_points[:n])
+    sc = orc.gen_scalars(n, 1000 + n)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
+
+
+@pytest.mark.parametrize("nbits", [1, 2, 8, 16, 63, 64, 65, 128, 160, 254, 255, 256])
+def test_msm_nbits(hip, orc, base_points, nbits):
+    n = 300
+    rnd = random.Random(nbits)
+    nb = (nbits + 7) // 8
+    S = _scal([rnd.getrandbits(8 * nb) for _ in range(n)], nb)   # bits above nbits are ignored
+    pts = np.ascontiguousarray(base_points[:n])
+    want = orc.compress(orc.msm_pippenger(pts, S, n, nbits))
+    assert orc.compress(hip.p1s_mult_pippenger(pts, S, nbits)) == want
+
+
+@pytest.mark.parametrize("window", [1, 2, 3, 4, 7, 8, 12, 16, 20, 24])
+def test_msm_explicit_window(hip, orc, base_points, window):
+    n = 200
+    pts = np.ascontiguousarray(base_points[:n])
+    sc = orc.gen_scalars(n, 7 * window)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255, window=window)) == want
+
+
+@pytest.mark.parametrize("mode", ["contiguous", "exact", "mixed"])
+def test_descriptor_conventions(hip, orc, base_points, mode):
+    n = 65
+    pts = np.ascontiguousarray(base_points[:n])
+    sc = orc.gen_scalars(n, 5)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255, mode=mode, k=17)) == want
+
+
+def test_scattered_descriptors(hip, orc, base_points):
+    """Exact pointer arrays that are not contiguous (reversed order)."""
+    n = 40
+    pts = np.ascontiguousarray(base_points[:n])
+    sc = orc.gen_scalars(n, 6)
+    d = hip.Descriptors(pts, sc, "exact")
+    d.pp = d.pp[::-1].copy()
+    d.ss = d.ss[::-1].copy()
+    ret = np.zeros(18, np.uint64)
+    scr = np.zeros(hip.scratch_sizeof(n) // 8, np.uint64)
+    hip.load().blst_p1s_mult_pippenger(hip._ptr(ret), hip._ptr(d.pp), n, hip._ptr(d.ss), 255, hip._ptr(scr))
+    want = orc.msm_pippenger(np.ascontiguousarray(pts[::-1]), np.ascontiguousarray(sc[::-1]), n, 255)
+    assert orc.compress(ret) == orc.compress(want)
+
+
+def test_single_point_without_scratch(hip, orc, base_points):
+    pts = np.ascontiguousarray(base_points[:1])
+    for k in (0, 1, 2, R - 1, (1 << 255) - 19):
+        sc = _scal([k])
+        want = orc.compress(orc.msm_pippenger(pts, sc, 1, 255))
+        assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255, scratch=False)) == want
+
+
+def test_edge_scalars_and_points(hip, orc, base_points):
+    n = 64
+    pts = np.array(base_points[:n])
+    vals = [int.from_bytes(s.tobytes(), "little") for s in orc.gen_scalars(n, 8)]
+    vals[0] = 0
+    vals[1] = 1
+    vals[2] = R - 1
+    vals[3] = R
+    vals[4] = (1 << 255) - 1
+    pts[10] = pts[11]              # repeated point, same scalar -> doubling in a bucket
+    vals[10] = vals[11]
+    pts[12] = pts[13]              # repeated point, negated scalar -> cancels in a bucket
+    vals[12] = R - vals[13]
+    pts[20] = 0                    # all-zero affine = infinity: contributes nothing
+    neg = pts[21].copy()           # P and -P with equal scalars
+    yv = pyref.limbs_value(neg[6:12])
+    neg[6:12] = pyref.fp_limbs((pyref.P - yv) % pyref.P)
+    pts[22] = neg
+    vals[22] = vals[21]
+    sc = _scal(vals)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
+    # naive term-by-term sum agrees too
+    assert orc.compress(orc.msm_naive(pts, sc, n, 255)) == want
+
+
+def test_all_scalars_equal_single_heavy_bucket(hip, orc, base_points):
+    n = 4096
+    pts = np.ascontiguousarray(base_points[:n])
+    sc = _scal([123456789] * n)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
+    sc = _scal([0] * n)
+    assert hip.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == bytes([0xC0]) + bytes(47)
+
+
+def test_scalars_beyond_r_256_bits(hip, orc):
+    A = np.array([pyref.affine_to_limbs(pyref.G)] * 2, dtype=np.uint64)
+    S = np.full((2, 32), 0xFF, np.uint8)
+    want = pyref.compress(pyref.mul(2 * ((1 << 256) - 1) % R, pyref.G))
+    assert orc.compress(hip.p1s_mult_pippenger(A, S, 256)) == want
+
+
+def test_fuzz_corpus(hip):
+    cases = json.load(open(os.path.join(GOLD, "fuzz_msm.json")))
+    for c in cases:
+        n = c["n"]
+        A = np.array([pyref.affine_to_limbs(pyref.mul(m, pyref.G)) for m in c["point_multipliers"]],
+                     dtype=np.uint64)
+        S = np.array(c["scalars"], dtype=np.uint8).reshape(n, 1)
+        assert hip.compress(hip.p1s_mult_pippenger(A, S, 8)).hex() == c["expected"]
+        assert hip.compress(hip.p1s_mult_pippenger(A, S, 8, window=8)).hex() == c["expected"]
+        T = np.array(c["tile_scalars"], dtype=np.uint8).reshape(n, 1)
+        assert hip.compress(hip.p1s_tile_pippenger(A, T, 8, 0, 8)).hex() == c["tile_expected"]
+
+
+def test_tile_matches_oracle(hip, orc, base_points):
+    n = 500
+    pts = np.ascontiguousarray(base_points[:n])
+    sc = orc.gen_scalars(n, 31)
+    for bit0, w in ((0, 8), (3, 5), (100, 13), (250, 10), (254, 4)):
+        want = orc.compress(orc.msm_tile(pts, sc, n, 255, bit0, w))
+        assert orc.compress(hip.p1s_tile_pippenger(pts, sc, 255, bit0, w)) == want
+
+
+def test_hash_to_curve_points(hip):
+    pts = json.load(open(os.path.join(GOLD, "h2c_g1.json")))
+    aff = [(int(v["x"], 16), int(v["y"], 16)) for v in pts]
+    A = np.array([pyref.affine_to_limbs(p) for p in aff], dtype=np.uint64)
+    ks = [3 ** i + 7 for i in range(len(aff))]
+    want = pyref.compress(pyref.msm(aff, ks))
+    assert hip.compress(hip.p1s_mult_pippenger(A, _scal(ks), 255)) == want
+
+
+def test_generator_multiples_kernel(hip, orc):
+    keys = orc.gen_scalars(3000, 4242)
+    keys[5] = 0
+    keys[6] = 0xFF
+    got = hip.generator_multiples(keys)
+    want = orc.gen_points(3000, 4242, threads=8)
+    want[5] = 0
+    g = orc.from_affine(orc.generator_affine())
+    want[6] = orc.to_affine(orc.p1_mult(g, bytes(keys[6]), 255))   # bit 255 is outside nbits
+    assert np.array_equal(got, want)
+
+
+def test_config0_n4096(hip, orc):
+    """The reference's CPU-runnable case: n=4096 uniform, nbits 255."""
+    n = 4096
+    pts = hip.synth_points(n, 0x9E3779B97F4A7C15 ^ 0x10)
+    sc = hip.synth_scalars(n, 0x9E3779B97F4A7C15 ^ 0x11)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    assert hip.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
+
+
+def test_config1_n2pow20_with_edge_cases(hip, orc):
+    """The plain 2^20 workload with its edge cases, bit-exact against the oracle."""
+    n = 1 << 20
+    seed = 0x9E3779B97F4A7C15
+    pts = hip.synth_points(n, seed ^ 1)
+    sc = hip.synth_scalars(n, seed ^ 2)
+    hip.synth_edge_cases(pts, sc, seed ^ 3)
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        got = hip.msm_device(dp.ptr, ds.ptr, n, 255)
+    finally:
+        dp.close()
+        ds.close()
+    want = orc.msm_pippenger(pts, sc, n, 255, threads=16)
+    assert hip.compress(got) == orc.compress(want)
+
+
+def test_linearity_at_2pow18(hip, orc):
+    """MSM(k) + MSM(k') == MSM(k + k' mod r): a size-independent property."""
+    n = 1 << 18
+    pts = hip.synth_points(n, 99)
+    a = hip.synth_scalars(n, 100)
+    b = hip.synth_scalars(n, 101)
+    ai = np.frombuffer(a.tobytes(), dtype="<u8").reshape(n, 4)
+    bi = np.frombuffer(b.tobytes(), dtype="<u8").reshape(n, 4)
+    s = np.zeros((n, 32), np.uint8)
+    for i in range(0, n, 4096):   # big-int sum mod r in blocks
+        for j in range(i, min(n, i + 4096)):
+            v = (int.from_bytes(a[j].tobytes(), "little") + int.from_bytes(b[j].tobytes(), "little")) % R
+            s[j] = np.frombuffer(v.to_bytes(32, "little"), np.uint8)
+    del ai, bi
+    ra = hip.p1s_mult_pippenger(pts, a, 255)
+    rb = hip.p1s_mult_pippenger(pts, b, 255)
+    rs = hip.p1s_mult_pippenger(pts, s, 255)
+    assert orc.compress(orc.p1_add(ra, rb)) == orc.compress(rs)

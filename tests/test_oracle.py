@@ -1,0 +1,252 @@
+"""Pins the oracle (oracle/blst_oracle.c) against checks that do not use it:
+Python big-integer field arithmetic, the textbook affine group law
+(tests/pyref.py), the IETF generator encoding, the hash-to-curve G1 vectors
+and the reference's MSM fuzz corpus (tests/golden/).  CPU only."""
+import json
+import os
+import random
+
+import numpy as np
+import pytest
+
+import pyref
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+P = pyref.P
+
+
+def mont(x):
+    return np.array(pyref.fp_limbs(pyref.to_mont(x)), dtype=np.uint64)
+
+
+def unmont(a):
+    return pyref.from_mont(pyref.limbs_value(a))
+
+
+def aff_limbs(pt):
+    return np.array(pyref.affine_to_limbs(pt), dtype=np.uint64)
+
+
+EDGE = [0, 1, 2, P - 1, P - 2, (P - 1) // 2, (P + 1) // 2, (1 << 381) % P, (1 << 383) % P]
+
+
+def test_fp_ops_against_bigint(orc):
+    rnd = random.Random(11)
+    vals = EDGE + [rnd.randrange(P) for _ in range(200)]
+    for i, a in enumerate(vals):
+        b = vals[(7 * i + 3) % len(vals)]
+        am, bm = mont(a), mont(b)
+        assert unmont(orc.fp_mul(am, bm)) == a * b % P
+        assert unmont(orc.fp_binop("or_fp_add", am, bm)) == (a + b) % P
+        assert unmont(orc.fp_binop("or_fp_sub", am, bm)) == (a - b) % P
+        assert unmont(orc.fp_unop("or_fp_neg", am)) == (-a) % P
+        # raw Montgomery product of the limb images: a*b*2^-384
+        raw = orc.fp_mul(orc.limbs(a), orc.limbs(b))
+        assert orc.value(raw) == a * b * pow(1 << 384, -1, P) % P
+        if a:
+            assert unmont(orc.fp_unop("or_fp_inv", am)) == pow(a, -1, P)
+    assert unmont(orc.fp_unop("or_fp_inv", mont(0))) == 0
+
+
+def test_montgomery_conversions(orc):
+    rnd = random.Random(12)
+    for a in EDGE + [rnd.randrange(P) for _ in range(50)]:
+        m = orc.fp_unop("or_fp_to_mont", orc.limbs(a))
+        assert orc.value(m) == a * (1 << 384) % P
+        assert orc.value(orc.fp_unop("or_fp_from_mont", m)) == a
+
+
+def test_generator_and_infinity_encoding(orc):
+    kat = json.load(open(os.path.join(GOLD, "kat.json")))
+    g = orc.generator_affine()
+    assert orc.on_curve(g)
+    assert pyref.limbs_to_affine(list(g)) == pyref.G
+    assert orc.compress(orc.from_affine(g)).hex() == kat["g1_generator_compressed"]
+    assert pyref.compress(pyref.G).hex() == kat["g1_generator_compressed"]
+    assert orc.compress(np.zeros(18, np.uint64)).hex() == kat["g1_infinity_compressed"]
+
+
+def test_hash_to_curve_points(orc):
+    pts = json.load(open(os.path.join(GOLD, "h2c_g1.json")))
+    assert len(pts) == 10
+    for v in pts:
+        pt = (int(v["x"], 16), int(v["y"], 16))
+        a = aff_limbs(pt)
+        assert orc.on_curve(a)
+        assert orc.compress(orc.from_affine(a)).hex() == v["compressed"]
+        # a Jacobian representative with Z != 1 compresses the same way
+        j = orc.from_affine(a)
+        j2 = orc.p1_add(j, np.zeros(18, np.uint64))
+        assert orc.compress(orc.p1_double(j2)) == pyref.compress(pyref.add(pt, pt))
+        assert pyref.decompress(bytes.fromhex(v["compressed"])) == pt
+
+
+def test_group_law_against_textbook(orc):
+    rnd = random.Random(13)
+    g = orc.from_affine(orc.generator_affine())
+    for _ in range(12):
+        a, b = rnd.randrange(1, 1 << 64), rnd.randrange(1, 1 << 64)
+        pa = orc.p1_mult(g, a.to_bytes(8, "little"), 64)
+        pb = orc.p1_mult(g, b.to_bytes(8, "little"), 64)
+        assert orc.compress(pa) == pyref.compress(pyref.mul(a, pyref.G))
+        assert orc.compress(orc.p1_add(pa, pb)) == pyref.compress(pyref.mul(a + b, pyref.G))
+        assert orc.compress(orc.p1_double(pa)) == pyref.compress(pyref.mul(2 * a, pyref.G))
+        assert orc.compress(orc.p1_add(pa, pa)) == pyref.compress(pyref.mul(2 * a, pyref.G))
+    # P + (-P) = infinity
+    pa = orc.p1_mult(g, (12345).to_bytes(2, "little"), 16)
+    neg = pa.copy()
+    neg[6:12] = orc.fp_unop("or_fp_neg", pa[6:12])
+    assert orc.compress(orc.p1_add(pa, neg)) == pyref.compress(None)
+
+
+def _pts_and_scalars(n, nbits, seed):
+    rnd = random.Random(seed)
+    ks = [rnd.randrange(1, pyref.R_ORDER) for _ in range(n)]
+    pts = [pyref.mul(k, pyref.G) for k in ks]
+    nb = (nbits + 7) // 8
+    ss = [rnd.getrandbits(nbits) for _ in range(n)]
+    S = np.array([list(s.to_bytes(nb, "little")) for s in ss], dtype=np.uint8).reshape(n, nb)
+    A = np.array([pyref.affine_to_limbs(p) for p in pts], dtype=np.uint64)
+    return pts, ss, A, S
+
+
+@pytest.mark.parametrize("n,nbits", [(2, 8), (3, 8), (5, 1), (7, 17), (9, 64), (12, 255), (6, 256)])
+def test_msm_against_textbook(orc, n, nbits):
+    pts, ss, A, S = _pts_and_scalars(n, nbits, 100 * n + nbits)
+    want = pyref.compress(pyref.msm(pts, ss))
+    assert orc.compress(orc.msm_naive(A, S, n, nbits)) == want
+    assert orc.compress(orc.msm_pippenger(A, S, n, nbits)) == want
+    for w in (1, 2, 3, 5, 8):
+        assert orc.compress(orc.msm_pippenger(A, S, n, nbits, window=w)) == want
+    assert orc.compress(orc.msm_pippenger(A, S, n, nbits, threads=3)) == want
+
+
+def test_msm_closed_form_1G_plus_2_2G(orc):
+    # 1*G + 2*(2G) = 5G (C_api_tests.cpp:1488-1515 pattern)
+    pts = [pyref.G, pyref.mul(2, pyref.G)]
+    A = np.array([pyref.affine_to_limbs(p) for p in pts], dtype=np.uint64)
+    S = np.array([[1], [2]], dtype=np.uint8)
+    assert orc.compress(orc.msm_pippenger(A, S, 2, 8)) == pyref.compress(pyref.mul(5, pyref.G))
+
+
+def test_scalars_beyond_r_act_mod_r(orc):
+    # Group_scalar_mult_tests.cpp:757-808: two copies of G with 2^256-1
+    A = np.array([pyref.affine_to_limbs(pyref.G)] * 2, dtype=np.uint64)
+    S = np.full((2, 32), 0xFF, np.uint8)
+    want = pyref.compress(pyref.mul(2 * ((1 << 256) - 1) % pyref.R_ORDER, pyref.G))
+    assert orc.compress(orc.msm_pippenger(A, S, 2, 256)) == want
+
+
+def test_msm_edge_inputs(orc):
+    r = pyref.R_ORDER
+    pts = [pyref.mul(k, pyref.G) for k in (3, 5, 7, 11, 13)]
+    cases = [
+        ([0, 0, 0, 0, 0], None),
+        ([1, 0, 0, 0, 0], pts[0]),
+        ([r - 1, 0, 0, 0, 0], pyref.neg(pts[0])),
+    ]
+    for ss, want in cases:
+        A = np.array([pyref.affine_to_limbs(p) for p in pts], dtype=np.uint64)
+        S = np.array([list(s.to_bytes(32, "little")) for s in ss], dtype=np.uint8)
+        assert orc.compress(orc.msm_pippenger(A, S, 5, 255)) == pyref.compress(want)
+    # P and -P with equal scalars cancel; repeated points double; infinity skipped
+    plist = [pts[0], pyref.neg(pts[0]), pts[1], pts[1], None]
+    ss = [77, 77, 5, 5, 123]
+    A = np.array([pyref.affine_to_limbs(p) for p in plist], dtype=np.uint64)
+    S = np.array([list(s.to_bytes(32, "little")) for s in ss], dtype=np.uint8)
+    want = pyref.mul(10, pts[1])
+    assert orc.compress(orc.msm_pippenger(A, S, 5, 255)) == pyref.compress(want)
+
+
+def test_fuzz_corpus_replays(orc):
+    cases = json.load(open(os.path.join(GOLD, "fuzz_msm.json")))
+    assert len(cases) == 18
+    for c in cases:
+        n = c["n"]
+        A = np.array([pyref.affine_to_limbs(pyref.mul(m, pyref.G)) for m in c["point_multipliers"]],
+                     dtype=np.uint64)
+        S = np.array(c["scalars"], dtype=np.uint8).reshape(n, 1)
+        assert orc.compress(orc.msm_pippenger(A, S, n, 8)).hex() == c["expected"]
+        assert orc.compress(orc.msm_pippenger(A, S, n, 8, window=8)).hex() == c["expected"]
+        T = np.array(c["tile_scalars"], dtype=np.uint8).reshape(n, 1)
+        assert orc.compress(orc.msm_tile(A, T, n, 8, 0, 8)).hex() == c["tile_expected"]
+
+
+def test_booth_digits_reconstruct_scalar(orc):
+    rnd = random.Random(14)
+    for nbits in (1, 2, 7, 8, 9, 31, 64, 100, 255, 256):
+        nb = (nbits + 7) // 8
+        for _ in range(6):
+            k = rnd.getrandbits(nbits) | (rnd.getrandbits(1) << (nbits - 1))
+            kb = k.to_bytes(nb, "little")
+            for c in (1, 2, 3, 4, 5, 8, 13, 16, 17, 21, 24):
+                nwin = 1 + nbits // c
+                total = 0
+                for w in range(nwin):
+                    e = min(nbits - w * c + 1, c)
+                    raw = orc.booth_digit(kb, nbits, w * c, c)
+                    mag = raw & ((1 << e) - 1)
+                    sign = (raw >> e) & 1
+                    d = -mag if sign else mag
+                    assert abs(d) <= 1 << (e - 1)
+                    total += d << (c * w)
+                assert total == k, (nbits, c, k)
+
+
+def test_booth_digits_exhaustive_small(orc):
+    for nbits in (1, 2, 3, 4, 5, 6):
+        for k in range(1 << nbits):
+            for c in (1, 2, 3, 4):
+                total = 0
+                for w in range(1 + nbits // c):
+                    e = min(nbits - w * c + 1, c)
+                    raw = orc.booth_digit(bytes([k]), nbits, w * c, c)
+                    mag, sign = raw & ((1 << e) - 1), (raw >> e) & 1
+                    total += (-mag if sign else mag) << (c * w)
+                assert total == k
+
+
+def test_window_and_scratch_sizes(orc):
+    # reference values (multi_scalar.h:329-348, 460-479)
+    assert orc.scratch_sizeof(0) == 0
+    assert orc.scratch_sizeof(1 << 6) == 3072
+    assert orc.scratch_sizeof(4096) == 98304
+    assert orc.scratch_sizeof(1 << 16) == 786432
+    assert orc.scratch_sizeof(1 << 18) == 3145728
+    assert orc.scratch_sizeof(1 << 20) == 12582912
+    assert orc.window_size(4096) == 10 and orc.window_size(1 << 20) == 17
+    assert orc.window_scratch_sizeof(0) == 0 and orc.window_scratch_sizeof(25) == 0
+    assert orc.window_scratch_sizeof(24) == 192 << 23
+    for n in (1, 2, 3, 4, 15, 16, 17, 255, 256, 511, 512, 8191, 8192):
+        lg = n.bit_length() - 1
+        w = lg - 3 if lg > 12 else lg - 2 if lg > 8 else lg - 1 if lg > 4 else (2 if lg else 1)
+        assert orc.window_size(n) == w
+
+
+def test_generated_points_are_multiples_of_g(orc):
+    keys = orc.gen_scalars(6, 42)
+    pts = orc.gen_points(6, 42, threads=2)
+    for k, p in zip(keys, pts):
+        assert pyref.limbs_to_affine(list(p)) == pyref.mul(int.from_bytes(k.tobytes(), "little"), pyref.G)
+    srs = orc.gen_srs(4, 9, threads=2)
+    g = pyref.limbs_to_affine(list(srs[1]))
+    assert pyref.limbs_to_affine(list(srs[0])) == pyref.G
+    # [tau^2]G from tau: check via discrete structure e.g. srs[2] = tau*srs[1] needs tau;
+    # instead check srs[i] are on the curve and distinct
+    assert all(pyref.on_curve(pyref.limbs_to_affine(list(s))) for s in srs) and g != pyref.G
+
+
+def test_edge_case_generator(orc):
+    n = 1000
+    pts = orc.gen_points(n, 1, threads=2)
+    sc = orc.gen_scalars(n, 2)
+    p0, s0 = pts.copy(), sc.copy()
+    orc.apply_edge_cases(pts, sc, 3)
+    r = pyref.R_ORDER
+    vals = [int.from_bytes(s.tobytes(), "little") for s in sc]
+    assert sum(v == 0 for v in vals) >= 10
+    assert sum(v == 1 for v in vals) >= 10
+    assert sum(v == r - 1 for v in vals) >= 10
+    changed = sum(not np.array_equal(a, b) for a, b in zip(pts, p0))
+    assert 1 <= changed <= 5
+    assert sum(not np.array_equal(a, b) for a, b in zip(sc, s0)) >= 25
