@@ -87,7 +87,36 @@ int blst_hip_msm_device(blst_p1 *ret, const void *device_points, const void *dev
  * resident SRS bases): copy `bytes` from host memory into a new device
  * buffer on the current device.  Returns NULL on failure. */
 void *blst_hip_device_upload(const void *host, size_t bytes);
+int blst_hip_device_write(void *device_dst, const void *host_src, size_t bytes);
 void blst_hip_device_free(void *device_ptr);
+
+/* One MSM of a batch; points and scalars are device pointers (points
+ * 16-byte aligned).  window 0 selects the tuned window. */
+typedef struct {
+    const void *points;
+    const void *scalars;
+    size_t npoints;
+    size_t nbits;
+    size_t window;
+} blst_hip_msm_job;
+
+/* A batch of MSMs of any sizes, sharing or not sharing bases, computed by one
+ * launch sequence: one digit pass, one sort, one accumulation and one
+ * reduction over all jobs, and a device-side window fold per job.
+ * rets[k] receives job k (infinity for npoints == 0 or nbits == 0).
+ * Returns 0 on success. */
+int blst_hip_msm_batch_device(blst_p1 *rets, const blst_hip_msm_job *jobs, size_t njobs);
+
+/* A batch captured once into a HIP graph and replayed: the job shapes and the
+ * device input pointers are fixed at creation (the caller rewrites the
+ * contents of its scalar / point buffers between runs, e.g. new polynomials
+ * over a resident KZG setup).  run() replays the whole launch sequence and
+ * writes rets[njobs].  Plans are independent of each other and of the
+ * calls above; one plan must not be run from two threads at once. */
+typedef struct blst_hip_batch_plan blst_hip_batch_plan;
+blst_hip_batch_plan *blst_hip_batch_plan_create(const blst_hip_msm_job *jobs, size_t njobs);
+int blst_hip_batch_plan_run(blst_hip_batch_plan *plan, blst_p1 *rets);
+void blst_hip_batch_plan_destroy(blst_hip_batch_plan *plan);
 
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),

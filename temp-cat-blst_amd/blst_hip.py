@@ -28,11 +28,16 @@ SIGNATURES = {
     "blst_hip_last_phase_ms": ([_vp], None),
     "blst_hip_device_upload": ([_vp, _sz], _vp),
     "blst_hip_device_free": ([_vp], None),
+    "blst_hip_device_write": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
     "blst_hip_synth_scalars": ([_vp, _sz, ctypes.c_uint64, ctypes.c_int], None),
     "blst_hip_synth_srs_scalars": ([_vp, _sz, ctypes.c_uint64], None),
     "blst_hip_synth_edge_cases": ([_vp, _vp, _sz, ctypes.c_uint64], None),
     "blst_hip_p1_compress": ([_vp, _vp], None),
+    "blst_hip_msm_batch_device": ([_vp, _vp, _sz], ctypes.c_int),
+    "blst_hip_batch_plan_create": ([_vp, _sz], _vp),
+    "blst_hip_batch_plan_run": ([_vp, _vp], ctypes.c_int),
+    "blst_hip_batch_plan_destroy": ([_vp], None),
 }
 
 _lib = None
@@ -195,6 +200,57 @@ def synth_points(n, seed):
     return generator_multiples(synth_scalars(n, seed))
 
 
+class MsmJob(ctypes.Structure):
+    """blst_hip_msm_job"""
+    _fields_ = [("points", ctypes.c_void_p), ("scalars", ctypes.c_void_p), ("npoints", ctypes.c_size_t),
+                ("nbits", ctypes.c_size_t), ("window", ctypes.c_size_t)]
+
+
+def msm_batch_device(jobs):
+    """jobs: sequence of (device_points, device_scalars, n, nbits[, window]).
+    Returns an (njobs, 18) uint64 array of blst_p1 results."""
+    arr = (MsmJob * len(jobs))()
+    for k, j in enumerate(jobs):
+        arr[k].points, arr[k].scalars, arr[k].npoints, arr[k].nbits = j[0], j[1], j[2], j[3]
+        arr[k].window = j[4] if len(j) > 4 else 0
+    out = np.zeros((len(jobs), 18), np.uint64)
+    rc = load().blst_hip_msm_batch_device(_ptr(out), ctypes.cast(arr, ctypes.c_void_p), len(jobs))
+    if rc:
+        raise RuntimeError("blst_hip_msm_batch_device: " + last_error())
+    return out
+
+
+def _job_array(jobs):
+    arr = (MsmJob * max(len(jobs), 1))()
+    for k, j in enumerate(jobs):
+        arr[k].points, arr[k].scalars, arr[k].npoints, arr[k].nbits = j[0], j[1], j[2], j[3]
+        arr[k].window = j[4] if len(j) > 4 else 0
+    return arr
+
+
+class BatchPlan:
+    """A batch captured into a HIP graph (blst_hip_batch_plan_*)."""
+
+    def __init__(self, jobs):
+        self.njobs = len(jobs)
+        self._arr = _job_array(jobs)
+        self.ptr = load().blst_hip_batch_plan_create(ctypes.cast(self._arr, ctypes.c_void_p), self.njobs)
+        if not self.ptr:
+            raise RuntimeError("blst_hip_batch_plan_create: " + last_error())
+
+    def run(self):
+        out = np.zeros((self.njobs, 18), np.uint64)
+        rc = load().blst_hip_batch_plan_run(ctypes.c_void_p(self.ptr), _ptr(out))
+        if rc:
+            raise RuntimeError("blst_hip_batch_plan_run: " + last_error())
+        return out
+
+    def close(self):
+        if self.ptr:
+            load().blst_hip_batch_plan_destroy(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
 class DeviceBuffer:
     """A device copy of a host numpy array (freed on close)."""
 
@@ -204,6 +260,13 @@ class DeviceBuffer:
         self.ptr = load().blst_hip_device_upload(_ptr(host), host.nbytes)
         if not self.ptr:
             raise RuntimeError("blst_hip_device_upload: " + last_error())
+
+    def write(self, host):
+        """Overwrite the buffer's contents (same byte size) from host memory."""
+        host = np.ascontiguousarray(host)
+        assert host.nbytes == self.nbytes
+        if load().blst_hip_device_write(ctypes.c_void_p(self.ptr), _ptr(host), host.nbytes):
+            raise RuntimeError("blst_hip_device_write: " + last_error())
 
     def close(self):
         if self.ptr:

@@ -6,7 +6,7 @@
 // (blst/c_api.cpp:1382-1418, 1546-1611): they are the only part of the
 // reference's control flow that callers can observe.  Everything past the
 // checks is this backend's own pipeline.
-#include "msm_kernels.hip.h"
+#include "launch.h"
 #include "host_ec.h"
 #include "../../include/blst_hip.h"
 
@@ -53,14 +53,35 @@ struct DevBuf {
     ~DevBuf() { if (p) (void)hipFree(p); }
 };
 
+// Device memory and stream of one in-flight batch.  The per-thread context,
+// each captured batch plan and each serving slot own one.
+struct Workspace {
+    hipStream_t stream = nullptr;
+    DevBuf ent, rank, hist, off, skey, sval, buckets, slot_key, slots, H, scan_tmp, W, res, tables, maxcnt;
+    std::vector<uint8_t> tables_host;
+    uint32_t* host_word = nullptr;       // pinned readback word
+    hipEvent_t ev[8] = {};
+    void init() {
+        HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto& e : ev) HIPCK(hipEventCreate(&e));
+        HIPCK(hipHostMalloc((void**)&host_word, 64, hipHostMallocDefault));
+        maxcnt.ensure(64);
+    }
+    ~Workspace() {
+        if (host_word) (void)hipHostFree(host_word);
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
 struct Ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;        // == ws.stream
     std::mutex mu;
-    DevBuf pts, scl, ent, rank, hist, off, skey, sval, buckets, slot_key, slots, H, scan_tmp, dbg;
+    Workspace ws;
+    DevBuf pts, scl, dbg;
     DevBuf gtable, gen_tmp, gen_out;     // comb table of G, fixed-base scratch
     bool gtable_ready = false;
-    hipEvent_t ev[8] = {};
 };
 
 std::mutex g_ctx_mu;
@@ -74,8 +95,8 @@ Ctx& context() {
     if (it != g_ctx.end()) return *it->second;
     auto c = std::make_unique<Ctx>();
     c->device = dev;
-    HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
-    for (auto& e : c->ev) HIPCK(hipEventCreate(&e));
+    c->ws.init();
+    c->stream = c->ws.stream;
     Ctx& ref = *c;
     g_ctx.emplace(dev, std::move(c));
     return ref;
@@ -111,24 +132,122 @@ uint32_t device_window(size_t n, size_t nbits) {
     return bc;
 }
 
-struct Plan {
-    uint32_t n, nbits, nbytes, c, nwin, B, bit_base;
-    bool fold;        // combine windows (full MSM) or return the single window (tile)
+// One MSM of a batch as the host sees it.
+struct JobSpec {
+    const void* points;    // device, n affine points
+    const void* scalars;   // device, n * ceil(nbits/8) bytes
+    uint32_t n, nbits, c;
+    uint32_t bit_base;     // tile mode: first bit of the single window
+    bool tile;             // one window at bit_base, no window fold
 };
+
+constexpr uint32_t kChunk = 32;     // sorted entries per accumulate lane
+constexpr uint32_t kSegment = 8;    // buckets per reduction lane
+
+// Key, entry, point, group and segment ranges of every job of a batch.
+struct Layout {
+    std::vector<job_t> jobs;
+    std::vector<uint32_t> pbase, ebase, kbase;
+    std::vector<group_t> groups;
+    std::vector<uint32_t> sbase;
+    uint32_t total_points = 0, total_entries = 0, nkeys = 0, total_segments = 0, max_n = 0;
+};
+
+Layout make_layout(const std::vector<JobSpec>& specs) {
+    Layout L;
+    uint64_t pts = 0, ent = 0, keys = 0, segs = 0;
+    for (const JobSpec& s : specs) {
+        job_t J;
+        J.scalars = static_cast<const uint8_t*>(s.scalars);
+        J.points = static_cast<const uint4*>(s.points);
+        J.n = s.n;
+        J.nbits = s.nbits;
+        J.nbytes = (s.nbits + 7) / 8;
+        J.c = s.c;
+        J.nwin = s.tile ? 1u : 1u + s.nbits / s.c;
+        J.B = 1u << (s.c - 1);
+        J.bit_base = s.bit_base;
+        J.vec = (J.nbytes == 32 && ((uintptr_t)s.scalars % 16) == 0) ? 1u : 0u;
+        J.kbase = (uint32_t)keys;
+        J.ebase = (uint32_t)ent;
+        J.pbase = (uint32_t)pts;
+        J.gbase = (uint32_t)L.groups.size();
+        for (uint32_t w = 0; w < J.nwin; w++) {
+            group_t G = {};
+            G.kb = J.kbase + w * J.B;
+            G.B = J.B;
+            G.L = std::min<uint32_t>(kSegment, J.B);
+            G.P = J.B / G.L;
+            G.sb = (uint32_t)segs;
+            segs += G.P;
+            L.groups.push_back(G);
+            L.sbase.push_back(G.sb);
+        }
+        pts += J.n;
+        ent += (uint64_t)J.nwin * J.n;
+        keys += (uint64_t)J.nwin * J.B;
+        L.max_n = std::max(L.max_n, J.n);
+        L.jobs.push_back(J);
+        L.pbase.push_back(J.pbase);
+        L.ebase.push_back(J.ebase);
+        L.kbase.push_back(J.kbase);
+    }
+    if (ent >= 0x7fffffffULL || keys >= 0x7fffffffULL || pts >= 0x7fffffffULL || segs >= 0x7fffffffULL)
+        throw HipFail{"batch too large for one launch"};
+    L.total_points = (uint32_t)pts;
+    L.total_entries = (uint32_t)ent;
+    L.nkeys = (uint32_t)keys;
+    L.total_segments = (uint32_t)segs;
+    return L;
+}
+
+// Device copies of the layout tables (one upload per batch).
+struct DevLayout {
+    const job_t* jobs;
+    const uint32_t *pbase, *ebase, *kbase, *sbase;
+    const group_t* groups;
+};
+
+DevLayout upload_layout(Workspace& cx, const Layout& L) {
+    const size_t nj = L.jobs.size(), ng = L.groups.size();
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    const size_t o_jobs = 0, o_p = al(o_jobs + nj * sizeof(job_t)), o_e = al(o_p + nj * 4),
+                 o_k = al(o_e + nj * 4), o_g = al(o_k + nj * 4), o_s = al(o_g + ng * sizeof(group_t)),
+                 total = al(o_s + ng * 4);
+    cx.tables.ensure(total);
+    cx.tables_host.resize(total);
+    uint8_t* h = cx.tables_host.data();
+    std::memcpy(h + o_jobs, L.jobs.data(), nj * sizeof(job_t));
+    std::memcpy(h + o_p, L.pbase.data(), nj * 4);
+    std::memcpy(h + o_e, L.ebase.data(), nj * 4);
+    std::memcpy(h + o_k, L.kbase.data(), nj * 4);
+    std::memcpy(h + o_g, L.groups.data(), ng * sizeof(group_t));
+    std::memcpy(h + o_s, L.sbase.data(), ng * 4);
+    HIPCK(hipMemcpyAsync(cx.tables.p, h, total, hipMemcpyHostToDevice, cx.stream));
+    uint8_t* d = cx.tables.as<uint8_t>();
+    DevLayout D;
+    D.jobs = reinterpret_cast<const job_t*>(d + o_jobs);
+    D.pbase = reinterpret_cast<const uint32_t*>(d + o_p);
+    D.ebase = reinterpret_cast<const uint32_t*>(d + o_e);
+    D.kbase = reinterpret_cast<const uint32_t*>(d + o_k);
+    D.groups = reinterpret_cast<const group_t*>(d + o_g);
+    D.sbase = reinterpret_cast<const uint32_t*>(d + o_s);
+    return D;
+}
 
 // ---------------------------------------------------------------- device
 
-void scan_exclusive(Ctx& cx, const uint32_t* in, uint32_t* out, uint32_t N, uint32_t* tmp) {
+void scan_exclusive(Workspace& cx, const uint32_t* in, uint32_t* out, uint32_t N, uint32_t* tmp) {
     const uint32_t tiles = div_up(N, kScanTile);
     if (tiles == 1) {
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(kScanThreads), 0, cx.stream, in, out, nullptr, N);
+        launch_k_scan_tiles(dim3(1), dim3(kScanThreads), cx.stream, in, out, nullptr, N);
         return;
     }
     uint32_t* sums = tmp;
     uint32_t* sums_scanned = tmp + tiles;
-    hipLaunchKernelGGL(k_scan_tiles, dim3(tiles), dim3(kScanThreads), 0, cx.stream, in, out, sums, N);
+    launch_k_scan_tiles(dim3(tiles), dim3(kScanThreads), cx.stream, in, out, sums, N);
     scan_exclusive(cx, sums, sums_scanned, tiles, tmp + 2 * (size_t)tiles);
-    hipLaunchKernelGGL(k_scan_add, dim3(div_up(N, 256)), dim3(256), 0, cx.stream, out, sums_scanned, N);
+    launch_k_scan_add(dim3(div_up(N, 256)), dim3(256), cx.stream, out, sums_scanned, N);
 }
 
 size_t scan_tmp_words(uint32_t N) {
@@ -142,94 +261,129 @@ size_t scan_tmp_words(uint32_t N) {
     return w + 16;
 }
 
-constexpr uint32_t kChunk = 32;     // sorted entries per accumulate lane
-constexpr uint32_t kSegment = 8;    // buckets per reduction lane
+void fill_words(Workspace& cx, void* dst, size_t words, uint32_t value) {
+    const uint32_t blocks = (uint32_t)std::min<size_t>(div_up(words, 256), 8192);
+    launch_k_fill(dim3(std::max<uint32_t>(blocks, 1)), dim3(256), cx.stream,
+                       static_cast<uint32_t*>(dst), words, value);
+}
 
-// Runs the whole device pipeline for points/scalars already on the device
-// and writes the window sums to host memory (xyzz6 per window).
-void run_device(Ctx& cx, const Plan& pl, const void* d_points, const void* d_scalars,
-                std::vector<hostec::xyzz6>& wins) {
-    const uint32_t nkeys = pl.nwin * pl.B;
-    const uint64_t nent = (uint64_t)pl.nwin * pl.n;
-    if (nent >= 0x7fffffffULL || (uint64_t)nkeys >= 0x7fffffffULL)
-        throw HipFail{"MSM too large for one launch"};
-    const uint32_t nchunks = div_up(nent, kChunk);
-    const uint32_t nslots = 2 * nchunks;
-    const uint32_t L = std::min<uint32_t>(kSegment, pl.B);
-    const uint32_t P = pl.B / L;
+// Partial-tree rounds that cover any bucket of jobs of at most max_n points:
+// a bucket holds at most n entries of one window, so it spans at most
+// ceil(n / T) + 1 chunks.
+uint32_t bound_rounds(uint32_t max_n) {
+    const uint32_t span = div_up(max_n, kChunk) + 1;
+    uint32_t r = 0;
+    while ((1u << r) < span) r++;
+    return r;
+}
 
-    cx.ent.ensure(nent * 4);
-    cx.rank.ensure(nent * 4);
-    cx.skey.ensure(nent * 4);
-    cx.sval.ensure(nent * 4);
+void ensure_workspace(Workspace& cx, const Layout& L) {
+    const uint32_t nkeys = L.nkeys, nent = L.total_entries;
+    const uint32_t nslots = 2 * std::max<uint32_t>(div_up(nent, kChunk), 1);
+    cx.ent.ensure((size_t)nent * 4 + 16);
+    cx.rank.ensure((size_t)nent * 4 + 16);
+    cx.skey.ensure((size_t)nent * 4 + 16);
+    cx.sval.ensure((size_t)nent * 4 + 16);
     cx.hist.ensure(((size_t)nkeys + 1) * 4);
     cx.off.ensure(((size_t)nkeys + 1) * 4);
     cx.scan_tmp.ensure(scan_tmp_words(nkeys + 1) * 4);
     cx.buckets.ensure((size_t)nkeys * sizeof(xyzz));
     cx.slot_key.ensure((size_t)nslots * 4);
     cx.slots.ensure((size_t)nslots * sizeof(xyzz));
-    cx.H.ensure((size_t)pl.nwin * P * sizeof(xyzz));
+    cx.H.ensure((size_t)L.total_segments * sizeof(xyzz));
+    cx.W.ensure((size_t)L.groups.size() * sizeof(xyzz));
+    cx.res.ensure((size_t)L.jobs.size() * sizeof(xyzz));
+}
 
-    win_params wp;
-    wp.n = pl.n;
-    wp.nbytes = pl.nbytes;
-    wp.nbits = pl.nbits;
-    wp.c = pl.c;
-    wp.nwin = pl.nwin;
-    wp.bit_base = pl.bit_base;
-    wp.B = pl.B;
-    wp.vec_scalars = (pl.nbytes == 32 && ((uintptr_t)d_scalars % 16) == 0) ? 1u : 0u;
+// Device bytes a batch of this layout needs beyond its inputs.
+size_t workspace_bytes(const Layout& L) {
+    const size_t nslots = 2 * (size_t)std::max<uint32_t>(div_up(L.total_entries, kChunk), 1);
+    return (size_t)L.total_entries * 16 + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * sizeof(xyzz) +
+           nslots * (4 + sizeof(xyzz)) + (size_t)L.total_segments * sizeof(xyzz) +
+           (L.groups.size() + L.jobs.size()) * sizeof(xyzz) + (1 << 20);
+}
 
+// Enqueues the device pipeline of a batch on cx.stream.  rounds < 0: measure
+// the biggest bucket (one synchronising 4-byte readback) and run just the
+// partial-tree rounds it needs; rounds >= 0: run that many (capturable).
+// With host_fold the window sums W[g] are copied to host_out (one xyzz per
+// group); otherwise k_fold combines the windows of every job and one xyzz per
+// job is copied.  host_out should be pinned for the copy to be asynchronous.
+void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int rounds, bool host_fold, void* host_out,
+                   bool timing = true) {
+    const uint32_t nkeys = L.nkeys;
+    const uint32_t nent = L.total_entries;
+    const uint32_t nchunks = std::max<uint32_t>(div_up(nent, kChunk), 1);
+    const uint32_t nslots = 2 * nchunks;
+    const uint32_t ngroups = (uint32_t)L.groups.size();
+    const uint32_t njobs = (uint32_t)L.jobs.size();
     hipStream_t st = cx.stream;
     uint32_t* hist = cx.hist.as<uint32_t>();
     uint32_t* off = cx.off.as<uint32_t>();
 
-    HIPCK(hipEventRecord(cx.ev[0], st));
-    HIPCK(hipMemsetAsync(hist, 0, ((size_t)nkeys + 1) * 4, st));
-    hipLaunchKernelGGL(k_digits, dim3(div_up(pl.n, 256)), dim3(256), 0, st, wp,
-                       (const uint8_t*)d_scalars, (const uint4*)d_points, cx.ent.as<uint32_t>(),
-                       cx.rank.as<uint32_t>(), hist);
+    if (timing) HIPCK(hipEventRecord(cx.ev[0], st));
+    fill_words(cx, hist, (size_t)nkeys + 1, 0);
+    launch_k_digits(dim3(div_up(L.total_points, 256)), dim3(256), st, D.jobs, D.pbase, njobs, L.total_points,
+                    cx.ent.as<uint32_t>(), cx.rank.as<uint32_t>(), hist);
     HIPCK(hipGetLastError());
-    HIPCK(hipEventRecord(cx.ev[1], st));
+    if (timing) HIPCK(hipEventRecord(cx.ev[1], st));
 
     scan_exclusive(cx, hist, off, nkeys + 1, cx.scan_tmp.as<uint32_t>());
-    hipLaunchKernelGGL(k_scatter, dim3(div_up(nent, 256)), dim3(256), 0, st, wp,
-                       cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off,
-                       cx.skey.as<uint32_t>(), cx.sval.as<uint32_t>());
+    launch_k_scatter(dim3(div_up(nent, 256)), dim3(256), st, D.jobs, D.ebase, njobs, nent,
+                     cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off, cx.skey.as<uint32_t>(),
+                     cx.sval.as<uint32_t>());
     HIPCK(hipGetLastError());
-    HIPCK(hipMemsetAsync(cx.buckets.p, 0, (size_t)nkeys * sizeof(xyzz), st));
-    HIPCK(hipEventRecord(cx.ev[2], st));
-    hipLaunchKernelGGL(k_accumulate, dim3(div_up(nchunks, 256)), dim3(256), 0, st,
-                       cx.skey.as<const uint32_t>(), cx.sval.as<const uint32_t>(), off + nkeys,
-                       (const uint4*)d_points, cx.buckets.as<xyzz>(), cx.slot_key.as<uint32_t>(),
-                       cx.slots.as<xyzz>(), kChunk, nchunks);
+    fill_words(cx, cx.buckets.p, (size_t)nkeys * (sizeof(xyzz) / 4), 0);
+    if (timing) HIPCK(hipEventRecord(cx.ev[2], st));
+    launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.skey.as<const uint32_t>(),
+                        cx.sval.as<const uint32_t>(), off + nkeys, D.jobs, D.kbase, njobs, cx.buckets.as<xyzz>(),
+                        cx.slot_key.as<uint32_t>(), cx.slots.as<xyzz>(), kChunk, nchunks);
     HIPCK(hipGetLastError());
-    HIPCK(hipEventRecord(cx.ev[3], st));
+    if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
 
-    // a bucket holds at most n entries of a window, so it spans at most
-    // ceil(n / T) + 1 chunks
-    const uint32_t maxspan = div_up(pl.n, kChunk) + 1;
-    for (uint32_t d = 1; d < maxspan; d <<= 1)
-        hipLaunchKernelGGL(k_partial_round, dim3(div_up(nslots, 256)), dim3(256), 0, st,
-                           cx.slot_key.as<const uint32_t>(), cx.slots.as<xyzz>(), off, hist,
-                           kChunk, nslots, d);
-    hipLaunchKernelGGL(k_partial_final, dim3(div_up(nslots, 256)), dim3(256), 0, st,
-                       cx.slot_key.as<const uint32_t>(), cx.slots.as<const xyzz>(),
-                       cx.buckets.as<xyzz>(), nslots);
+    uint32_t nrounds;
+    if (rounds < 0) {
+        fill_words(cx, cx.maxcnt.p, 1, 0);
+        launch_k_max_count(dim3(std::min<uint32_t>(div_up(nkeys, 256), 1024)), dim3(256), st, hist, nkeys,
+                           cx.maxcnt.as<uint32_t>());
+        HIPCK(hipMemcpyAsync(cx.host_word, cx.maxcnt.p, 4, hipMemcpyDeviceToHost, st));
+        HIPCK(hipStreamSynchronize(st));
+        const uint32_t maxspan = div_up(*cx.host_word, kChunk) + 1;
+        nrounds = 0;
+        while ((1u << nrounds) < maxspan) nrounds++;
+    } else {
+        nrounds = (uint32_t)rounds;
+    }
+    for (uint32_t r = 0; r < nrounds; r++)
+        launch_k_partial_round(dim3(div_up(nslots, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
+                               cx.slots.as<xyzz>(), off, hist, kChunk, nslots, 1u << r);
+    launch_k_partial_final(dim3(div_up(nslots, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
+                           cx.slots.as<const xyzz>(), cx.buckets.as<xyzz>(), nslots);
     HIPCK(hipGetLastError());
-    HIPCK(hipEventRecord(cx.ev[4], st));
+    if (timing) HIPCK(hipEventRecord(cx.ev[4], st));
 
-    hipLaunchKernelGGL(k_reduce_segments, dim3(div_up((uint64_t)pl.nwin * P, 256)), dim3(256), 0, st,
-                       cx.buckets.as<const xyzz>(), cx.H.as<xyzz>(), pl.B, L, pl.nwin);
-    for (uint32_t step = 1; step < P; step <<= 1)
-        hipLaunchKernelGGL(k_pair_sum, dim3(div_up((uint64_t)pl.nwin * (P / (2 * step)), 256)), dim3(256),
-                           0, st, cx.H.as<xyzz>(), P, pl.nwin, step);
+    launch_k_reduce_segments(dim3(div_up(L.total_segments, 256)), dim3(256), st, D.groups, D.sbase, ngroups,
+                             L.total_segments, cx.buckets.as<const xyzz>(), cx.H.as<xyzz>());
+    uint32_t maxP = 1;
+    for (const group_t& G : L.groups) maxP = std::max(maxP, G.P);
+    for (uint32_t step = 1; step < maxP; step <<= 1)
+        launch_k_pair_sum(dim3(div_up(L.total_segments, 256)), dim3(256), st, D.groups, D.sbase, ngroups,
+                          L.total_segments, cx.H.as<xyzz>(), step);
+    launch_k_gather_windows(dim3(div_up(ngroups, 256)), dim3(256), st, D.groups, ngroups, cx.H.as<const xyzz>(),
+                            cx.W.as<xyzz>());
     HIPCK(hipGetLastError());
-    wins.resize(pl.nwin);
-    HIPCK(hipMemcpy2DAsync(wins.data(), sizeof(xyzz), cx.H.p, (size_t)P * sizeof(xyzz), sizeof(xyzz),
-                           pl.nwin, hipMemcpyDeviceToHost, st));
-    HIPCK(hipEventRecord(cx.ev[5], st));
-    HIPCK(hipStreamSynchronize(st));
+    if (host_fold) {
+        HIPCK(hipMemcpyAsync(host_out, cx.W.p, (size_t)ngroups * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+    } else {
+        launch_k_fold(dim3(div_up(njobs, 64)), dim3(64), st, D.jobs, njobs, cx.W.as<const xyzz>(),
+                      cx.res.as<xyzz>());
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(host_out, cx.res.p, (size_t)njobs * sizeof(xyzz), hipMemcpyDeviceToHost, st));
+    }
+    if (timing) HIPCK(hipEventRecord(cx.ev[5], st));
+}
+
+void record_phases(Workspace& cx) {
     float ms;
     for (int k = 0; k < 5; k++) {
         HIPCK(hipEventElapsedTime(&ms, cx.ev[k], cx.ev[k + 1]));
@@ -237,17 +391,36 @@ void run_device(Ctx& cx, const Plan& pl, const void* d_points, const void* d_sca
     }
 }
 
-void finish(const Plan& pl, const std::vector<hostec::xyzz6>& wins, blst_p1* ret) {
+// Synchronous batch: layout upload, pipeline, results in out.
+void run_batch(Workspace& cx, const Layout& L, bool host_fold, std::vector<hostec::xyzz6>& out) {
+    ensure_workspace(cx, L);
+    const DevLayout D = upload_layout(cx, L);
+    out.resize(host_fold ? L.groups.size() : L.jobs.size());
+    enqueue_batch(cx, L, D, -1, host_fold, out.data());
+    HIPCK(hipStreamSynchronize(cx.stream));
+    record_phases(cx);
+}
+
+// window sums of one job -> blst_p1
+void finish_single(const JobSpec& js, const std::vector<hostec::xyzz6>& wins, blst_p1* ret) {
     auto t0 = std::chrono::steady_clock::now();
     hostec::xyzz6 r;
-    if (pl.fold) hostec::fold_windows(r, wins.data(), (int)pl.nwin, (int)pl.c);
+    if (!js.tile) hostec::fold_windows(r, wins.data(), (int)wins.size(), (int)js.c);
     else r = wins[0];
     hostec::jac6 j;
     hostec::to_jac(j, r);
     static_assert(sizeof(hostec::jac6) == sizeof(blst_p1), "blst_p1 layout");
     std::memcpy(ret, &j, sizeof(j));
-    auto t1 = std::chrono::steady_clock::now();
-    g_phase[5] = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// one MSM (or tile) over device-resident inputs
+void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
+    std::vector<JobSpec> v(1, js);
+    Layout L = make_layout(v);
+    std::vector<hostec::xyzz6> wins;
+    run_batch(cx.ws, L, true, wins);
+    finish_single(js, wins, ret);
 }
 
 // Resolves the reference's descriptor convention (a NULL entry after the head
@@ -295,18 +468,15 @@ int host_msm(blst_p1* ret, const blst_p1_affine* const points[], size_t npoints,
         cx.scl.ensure(npoints * nbytes + 16);
         HIPCK(hipMemcpyAsync(cx.pts.p, hp, npoints * sizeof(blst_p1_affine), hipMemcpyHostToDevice, cx.stream));
         HIPCK(hipMemcpyAsync(cx.scl.p, hs, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
-        Plan pl;
-        pl.n = (uint32_t)npoints;
-        pl.nbits = (uint32_t)nbits;
-        pl.nbytes = (uint32_t)nbytes;
-        pl.c = (uint32_t)(window ? window : device_window(npoints, nbits));
-        pl.nwin = fold ? (uint32_t)(1 + nbits / pl.c) : 1u;
-        pl.B = 1u << (pl.c - 1);
-        pl.bit_base = (uint32_t)bit0;
-        pl.fold = fold;
-        std::vector<hostec::xyzz6> wins;
-        run_device(cx, pl, cx.pts.p, cx.scl.p, wins);
-        finish(pl, wins, ret);
+        JobSpec js;
+        js.points = cx.pts.p;
+        js.scalars = cx.scl.p;
+        js.n = (uint32_t)npoints;
+        js.nbits = (uint32_t)nbits;
+        js.c = (uint32_t)(window ? window : device_window(npoints, nbits));
+        js.bit_base = (uint32_t)bit0;
+        js.tile = !fold;
+        device_msm(cx, js, ret);
     } catch (const HipFail& f) {
         g_err = f.msg;
         std::memset(ret, 0, sizeof(*ret));
@@ -355,7 +525,54 @@ bool heads_ok(const void* const* xs, const void* const* ys, size_t n) {
     return n == 0 || (xs != nullptr && ys != nullptr && xs[0] != nullptr && ys[0] != nullptr);
 }
 
+// A batch captured once as a HIP graph: fixed job shapes and fixed device
+// input pointers; each run replays the whole launch sequence.
+struct PlanImpl {
+    int device = 0;
+    Workspace ws;
+    Layout L;
+    DevLayout D;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hostec::xyzz6* host_res = nullptr;   // pinned, one per nonempty job
+    std::vector<size_t> idx;             // batch position of each nonempty job
+    size_t njobs = 0;
+    ~PlanImpl() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (host_res) (void)hipHostFree(host_res);
+    }
+};
+
+bool specs_from_jobs(const blst_hip_msm_job* jobs, size_t njobs, std::vector<JobSpec>& specs,
+                     std::vector<size_t>& idx) {
+    for (size_t k = 0; k < njobs; k++) {
+        const blst_hip_msm_job& j = jobs[k];
+        if (j.npoints == 0 || j.nbits == 0) continue;   // infinity, as the reference
+        if (j.nbits > 256 || j.window > 24 || !j.points || !j.scalars || ((uintptr_t)j.points % 16) != 0 ||
+            j.npoints >= 0x7fffffffULL) {
+            g_err = "invalid job " + std::to_string(k);
+            return false;
+        }
+        JobSpec js;
+        js.points = j.points;
+        js.scalars = j.scalars;
+        js.n = (uint32_t)j.npoints;
+        js.nbits = (uint32_t)j.nbits;
+        js.c = (uint32_t)(j.window ? j.window : device_window(j.npoints, j.nbits));
+        js.bit_base = 0;
+        js.tile = false;
+        specs.push_back(js);
+        idx.push_back(k);
+    }
+    return true;
+}
+
 }  // namespace
+
+struct blst_hip_batch_plan {
+    PlanImpl impl;
+};
 
 extern "C" {
 
@@ -433,25 +650,22 @@ int blst_hip_msm_device(blst_p1* ret, const void* device_points, const void* dev
     std::memset(ret, 0, sizeof(*ret));
     if (npoints == 0 || nbits == 0) { g_err.clear(); return 0; }
     if (nbits > 256 || window > 24 || !device_points || !device_scalars ||
-        ((uintptr_t)device_points % 16) != 0) {
+        ((uintptr_t)device_points % 16) != 0 || npoints >= 0x7fffffffULL) {
         g_err = "invalid arguments";
         return -1;
     }
     try {
         Ctx& cx = context();
         std::lock_guard<std::mutex> lk(cx.mu);
-        Plan pl;
-        pl.n = (uint32_t)npoints;
-        pl.nbits = (uint32_t)nbits;
-        pl.nbytes = (uint32_t)((nbits + 7) / 8);
-        pl.c = (uint32_t)(window ? window : device_window(npoints, nbits));
-        pl.nwin = (uint32_t)(1 + nbits / pl.c);
-        pl.B = 1u << (pl.c - 1);
-        pl.bit_base = 0;
-        pl.fold = true;
-        std::vector<hostec::xyzz6> wins;
-        run_device(cx, pl, device_points, device_scalars, wins);
-        finish(pl, wins, ret);
+        JobSpec js;
+        js.points = device_points;
+        js.scalars = device_scalars;
+        js.n = (uint32_t)npoints;
+        js.nbits = (uint32_t)nbits;
+        js.c = (uint32_t)(window ? window : device_window(npoints, nbits));
+        js.bit_base = 0;
+        js.tile = false;
+        device_msm(cx, js, ret);
     } catch (const HipFail& f) {
         g_err = f.msg;
         std::memset(ret, 0, sizeof(*ret));
@@ -461,6 +675,92 @@ int blst_hip_msm_device(blst_p1* ret, const void* device_points, const void* dev
     g_err.clear();
     return 0;
 }
+
+int blst_hip_msm_batch_device(blst_p1* rets, const blst_hip_msm_job* jobs, size_t njobs) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (njobs == 0) { g_err.clear(); return 0; }
+    if (!rets || !jobs) { g_err = "invalid arguments"; return -1; }
+    std::memset(rets, 0, njobs * sizeof(blst_p1));
+    std::vector<JobSpec> specs;
+    std::vector<size_t> idx;
+    if (!specs_from_jobs(jobs, njobs, specs, idx)) return -1;
+    if (specs.empty()) { g_err.clear(); return 0; }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        Layout L = make_layout(specs);
+        std::vector<hostec::xyzz6> res;
+        run_batch(cx.ws, L, false, res);
+        auto t1 = std::chrono::steady_clock::now();
+        for (size_t k = 0; k < specs.size(); k++) {
+            hostec::jac6 jj;
+            hostec::to_jac(jj, res[k]);
+            std::memcpy(&rets[idx[k]], &jj, sizeof(jj));
+        }
+        g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        std::memset(rets, 0, njobs * sizeof(blst_p1));
+        return -1;
+    }
+    g_phase[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    g_err.clear();
+    return 0;
+}
+
+blst_hip_batch_plan* blst_hip_batch_plan_create(const blst_hip_msm_job* jobs, size_t njobs) {
+    if (!jobs && njobs) { g_err = "invalid arguments"; return nullptr; }
+    auto plan = std::make_unique<blst_hip_batch_plan>();
+    PlanImpl& P = plan->impl;
+    P.njobs = njobs;
+    std::vector<JobSpec> specs;
+    if (!specs_from_jobs(jobs, njobs, specs, P.idx)) return nullptr;
+    try {
+        HIPCK(hipGetDevice(&P.device));
+        P.ws.init();
+        if (!specs.empty()) {
+            P.L = make_layout(specs);
+            ensure_workspace(P.ws, P.L);
+            P.D = upload_layout(P.ws, P.L);
+            HIPCK(hipHostMalloc((void**)&P.host_res, specs.size() * sizeof(hostec::xyzz6), hipHostMallocDefault));
+            HIPCK(hipStreamSynchronize(P.ws.stream));
+            HIPCK(hipStreamBeginCapture(P.ws.stream, hipStreamCaptureModeThreadLocal));
+            enqueue_batch(P.ws, P.L, P.D, (int)bound_rounds(P.L.max_n), false, P.host_res, false);
+            HIPCK(hipStreamEndCapture(P.ws.stream, &P.graph));
+            HIPCK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
+        }
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return nullptr;
+    }
+    g_err.clear();
+    return plan.release();
+}
+
+int blst_hip_batch_plan_run(blst_hip_batch_plan* plan, blst_p1* rets) {
+    auto t0 = std::chrono::steady_clock::now();
+    if (!plan || !rets) { g_err = "invalid arguments"; return -1; }
+    PlanImpl& P = plan->impl;
+    std::memset(rets, 0, P.njobs * sizeof(blst_p1));
+    if (P.idx.empty()) { g_err.clear(); return 0; }
+    try {
+        HIPCK(hipGraphLaunch(P.exec, P.ws.stream));
+        HIPCK(hipStreamSynchronize(P.ws.stream));
+        for (size_t k = 0; k < P.idx.size(); k++) {
+            hostec::jac6 jj;
+            hostec::to_jac(jj, P.host_res[k]);
+            std::memcpy(&rets[P.idx[k]], &jj, sizeof(jj));
+        }
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_phase[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    g_err.clear();
+    return 0;
+}
+
+void blst_hip_batch_plan_destroy(blst_hip_batch_plan* plan) { delete plan; }
 
 int blst_hip_booth_digits(int32_t* out, const blst_byte* scalars, size_t npoints, size_t nbits,
                           size_t bit0, size_t window) {
@@ -476,7 +776,7 @@ int blst_hip_booth_digits(int32_t* out, const blst_byte* scalars, size_t npoints
         cx.scl.ensure(npoints * nbytes + 16);
         cx.dbg.ensure(npoints * 4);
         HIPCK(hipMemcpyAsync(cx.scl.p, scalars, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
-        hipLaunchKernelGGL(k_digits_debug, dim3(div_up(npoints, 256)), dim3(256), 0, cx.stream,
+        launch_k_digits_debug(dim3(div_up(npoints, 256)), dim3(256), cx.stream,
                            cx.scl.as<const uint8_t>(), (uint32_t)npoints, (uint32_t)nbytes, (uint32_t)nbits,
                            (uint32_t)bit0, (uint32_t)window, cx.dbg.as<int32_t>());
         HIPCK(hipGetLastError());
@@ -503,6 +803,16 @@ void* blst_hip_device_upload(const void* host, size_t bytes) {
     return d;
 }
 
+int blst_hip_device_write(void* device_dst, const void* host_src, size_t bytes) {
+    if (!device_dst || !host_src) { g_err = "invalid arguments"; return -1; }
+    if (hipMemcpy(device_dst, host_src, bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        g_err = "hipMemcpy failed";
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
 void blst_hip_device_free(void* device_ptr) {
     if (device_ptr) (void)hipFree(device_ptr);
 }
@@ -524,10 +834,10 @@ int blst_hip_p1s_mult_generator(blst_p1_affine* out, const blst_byte* scalars, s
         cx.gen_tmp.ensure(npoints * sizeof(xyzz));
         cx.gen_out.ensure(npoints * sizeof(aff));
         HIPCK(hipMemcpyAsync(cx.scl.p, scalars, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
-        hipLaunchKernelGGL(k_fixed_base, dim3(div_up(npoints, 256)), dim3(256), 0, cx.stream,
+        launch_k_fixed_base(dim3(div_up(npoints, 256)), dim3(256), cx.stream,
                            cx.scl.as<const uint8_t>(), (uint32_t)npoints, (uint32_t)nbytes, (uint32_t)nbits,
                            cx.gtable.as<const uint4>(), cx.gen_tmp.as<xyzz>());
-        hipLaunchKernelGGL(k_to_affine, dim3(div_up(div_up(npoints, Q), 256)), dim3(256), 0, cx.stream,
+        launch_k_to_affine(dim3(div_up(div_up(npoints, Q), 256)), dim3(256), cx.stream,
                            cx.gen_tmp.as<const xyzz>(), cx.gen_out.as<aff>(), (uint32_t)npoints, Q);
         HIPCK(hipGetLastError());
         HIPCK(hipMemcpyAsync(out, cx.gen_out.p, npoints * sizeof(aff), hipMemcpyDeviceToHost, cx.stream));

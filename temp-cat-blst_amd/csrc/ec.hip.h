@@ -11,8 +11,8 @@
 // Formula sources are the EFD entries the reference names
 // (blst/scalar/curve/point.cpp:1933-1936, 2021-2025): madd-2008-s and
 // mdbl-2008-s-1 for XYZZ+affine, add-2008-s and dbl-2008-s-1 for XYZZ+XYZZ.
-// The exceptional inputs (equal points, opposite points, infinity) take a
-// branch; they are rare in an MSM and keep the common path straight-line.
+// The exceptional inputs (equal points, opposite points, infinity) are
+// resolved inside the adders without a second multiplication sequence.
 #pragma once
 #include "fp.hip.h"
 
@@ -34,27 +34,6 @@ __device__ __forceinline__ bool aff_is_inf(const aff& a) {
 
 __device__ __forceinline__ void xyzz_from_aff(xyzz& p, const aff& a) {
     p.x = a.x; p.y = a.y; fp_one(p.zz); fp_one(p.zzz);
-}
-
-// mdbl-2008-s-1: 2*(x, y) into XYZZ
-__device__ __noinline__ void xyzz_mdbl(xyzz& r, const aff& a) {
-    fp U, V, W, S, M, t;
-    fp_dbl(U, a.y);          // U = 2y
-    fp_sqr(V, U);            // V = U^2
-    fp_mul(W, U, V);         // W = U*V
-    fp_mul(S, a.x, V);       // S = x*V
-    fp_sqr(M, a.x);
-    fp_dbl(t, M);
-    fp_add(M, M, t);         // M = 3x^2
-    fp_sqr(r.x, M);
-    fp_dbl(t, S);
-    fp_sub(r.x, r.x, t);     // X3 = M^2 - 2S
-    fp_sub(t, S, r.x);
-    fp_mul(t, t, M);
-    fp_mul(r.y, W, a.y);
-    fp_sub(r.y, t, r.y);     // Y3 = M(S - X3) - W*y
-    r.zz = V;
-    r.zzz = W;
 }
 
 // dbl-2008-s-1: 2*P in XYZZ
@@ -81,10 +60,14 @@ __device__ __forceinline__ void xyzz_dbl_inl(xyzz& r, const xyzz& p) {
     r = o;
 }
 
-// out-of-line doubling for the rare equal-points branch of the adders
-__device__ __noinline__ void xyzz_dbl(xyzz& r, const xyzz& p) { xyzz_dbl_inl(r, p); }
 
-// acc += a (a affine, not infinity): madd-2008-s, 8M + 2S
+// Equal inputs reuse the addition's own multiplication sequence: with
+// P = R = 0 (the same point), substituting P' = 2*S1 and R' = 3*U1^2 turns
+// add-2008-s into dbl-2008-s-1 of the first input in the addition's scaled
+// frame (U1, S1, ZZ1*ZZ2, ZZZ1*ZZZ2), except that X3 = R'^2 - 2Q' carries no
+// -PPP' term, which is added back.  P = 0 with R != 0 is P + (-P) = infinity.
+
+// acc += a (a affine, not infinity): madd-2008-s, 8M + 2S; doubling when a == acc
 __device__ __forceinline__ void xyzz_madd(xyzz& acc, const aff& a) {
     if (xyzz_is_inf(acc)) { xyzz_from_aff(acc, a); return; }
     fp P, R;
@@ -92,17 +75,22 @@ __device__ __forceinline__ void xyzz_madd(xyzz& acc, const aff& a) {
     fp_mul(R, a.y, acc.zzz);
     fp_sub(P, P, acc.x);     // P = U2 - X1
     fp_sub(R, R, acc.y);     // R = S2 - Y1
-    if (fp_is_zero(P)) {
-        if (fp_is_zero(R)) xyzz_mdbl(acc, a);
-        else xyzz_set_inf(acc);
-        return;
+    const bool pz = fp_is_zero(P);
+    const bool dbl = pz && fp_is_zero(R);
+    if (pz && !dbl) { xyzz_set_inf(acc); return; }
+    if (dbl) {
+        fp t;
+        fp_dbl(P, acc.y);        // P' = 2*Y1
+        fp_sqr(t, acc.x);
+        fp_dbl(R, t);
+        fp_add(R, R, t);         // R' = 3*X1^2
     }
     fp PP, PPP, Q, t;
     fp_sqr(PP, P);
     fp_mul(PPP, PP, P);
     fp_mul(Q, acc.x, PP);
     fp_sqr(t, R);
-    fp_sub(t, t, PPP);
+    if (!dbl) fp_sub(t, t, PPP);
     fp_sub(t, t, Q);
     fp_sub(acc.x, t, Q);     // X3 = R^2 - PPP - 2Q
     fp_sub(Q, Q, acc.x);
@@ -113,7 +101,7 @@ __device__ __forceinline__ void xyzz_madd(xyzz& acc, const aff& a) {
     fp_mul(acc.zzz, acc.zzz, PPP);
 }
 
-// r = p + q: add-2008-s, 12M + 2S
+// r = p + q: add-2008-s, 12M + 2S; doubling when p == q
 __device__ __forceinline__ void xyzz_add(xyzz& r, const xyzz& p, const xyzz& q) {
     if (xyzz_is_inf(q)) { r = p; return; }
     if (xyzz_is_inf(p)) { r = q; return; }
@@ -124,10 +112,15 @@ __device__ __forceinline__ void xyzz_add(xyzz& r, const xyzz& p, const xyzz& q) 
     fp_mul(R, q.y, p.zzz);
     fp_sub(P, P, U1);
     fp_sub(R, R, S1);
-    if (fp_is_zero(P)) {
-        if (fp_is_zero(R)) xyzz_dbl(r, p);
-        else xyzz_set_inf(r);
-        return;
+    const bool pz = fp_is_zero(P);
+    const bool dbl = pz && fp_is_zero(R);
+    if (pz && !dbl) { xyzz_set_inf(r); return; }
+    if (dbl) {
+        fp t;
+        fp_dbl(P, S1);           // P' = 2*S1
+        fp_sqr(t, U1);
+        fp_dbl(R, t);
+        fp_add(R, R, t);         // R' = 3*U1^2
     }
     fp PP, PPP, Q, t;
     xyzz o;
@@ -135,7 +128,7 @@ __device__ __forceinline__ void xyzz_add(xyzz& r, const xyzz& p, const xyzz& q) 
     fp_mul(PPP, PP, P);
     fp_mul(Q, U1, PP);
     fp_sqr(t, R);
-    fp_sub(t, t, PPP);
+    if (!dbl) fp_sub(t, t, PPP);
     fp_sub(t, t, Q);
     fp_sub(o.x, t, Q);
     fp_sub(Q, Q, o.x);

@@ -175,7 +175,7 @@ __device__ __forceinline__ void fp_sqr(fp& r, const fp& a) { fp_mul(r, a, a); }
 
 // a^(p-2) = a^-1 (Fermat), left-to-right over the fixed exponent bits; the
 // branch is uniform across the wave.  0 maps to 0.
-__device__ __noinline__ void fp_inv(fp& r, const fp& a) {
+__device__ __forceinline__ void fp_inv(fp& r, const fp& a) {
     constexpr uint32_t E[12] = {BLS_P0 - 2, BLS_P1, BLS_P2, BLS_P3, BLS_P4, BLS_P5,
                                 BLS_P6, BLS_P7, BLS_P8, BLS_P9, BLS_P10, BLS_P11};
     fp acc;

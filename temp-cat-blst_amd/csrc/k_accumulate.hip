@@ -1,0 +1,74 @@
+// Bucket accumulation: chunks of the sorted list per lane, XYZZ mixed
+// additions of gathered affine points.
+#include "msm_common.hip.h"
+#include "launch.h"
+
+namespace bls {
+
+// One lane per chunk of T sorted entries.  A bucket segment that starts and
+// ends inside the chunk is final and goes to buckets[key].  Otherwise, with
+// k0 the chunk holding the bucket's first entry: the segment in chunk k0 goes
+// to slot 2*k0+1, the segment in any later chunk k to slot 2*k.
+__global__ void __launch_bounds__(256)
+k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+             const uint32_t* __restrict__ total, const job_t* __restrict__ jobs,
+             const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets,
+             uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks) {
+    const uint32_t chunk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (chunk >= nchunks) return;
+    slot_key[2 * chunk] = kInvalid;
+    slot_key[2 * chunk + 1] = kInvalid;
+    const uint32_t M = *total;
+    const uint32_t start = chunk * T;
+    if (start >= M) return;
+    const uint32_t end = min(start + T, M);
+    uint32_t cur = skey[start];
+    uint32_t j = find_owner(kbase, njobs, cur);
+    uint32_t next_kb = (j + 1 < njobs) ? kbase[j + 1] : 0xffffffffu;
+    const uint4* pts = jobs[j].points;
+    const bool before = start > 0 && skey[start - 1] == cur;
+    bool first = true;
+    xyzz acc;
+    xyzz_set_inf(acc);
+    #pragma unroll 1
+    for (uint32_t e = start; e < end; e++) {
+        const uint32_t key = skey[e];
+        if (key != cur) {
+            if (first && before) {
+                slot_key[2 * chunk] = cur;
+                store_xyzz(&slots[2 * chunk], acc);
+            } else {
+                store_xyzz(&buckets[cur], acc);
+            }
+            first = false;
+            cur = key;
+            xyzz_set_inf(acc);
+            if (key >= next_kb) {
+                while (j + 1 < njobs && kbase[j + 1] <= key) j++;
+                next_kb = (j + 1 < njobs) ? kbase[j + 1] : 0xffffffffu;
+                pts = jobs[j].points;
+            }
+        }
+        const uint32_t v = sval[e];
+        aff a;
+        load_aff(a, pts, v & 0x7fffffffu);
+        if (v >> 31) fp_neg(a.y, a.y);
+        xyzz_madd(acc, a);
+    }
+    const bool after = end < M && skey[end] == cur;
+    if (first && before) {
+        slot_key[2 * chunk] = cur;
+        store_xyzz(&slots[2 * chunk], acc);
+    } else if (after) {
+        slot_key[2 * chunk + 1] = cur;
+        store_xyzz(&slots[2 * chunk + 1], acc);
+    } else {
+        store_xyzz(&buckets[cur], acc);
+    }
+}
+
+void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks) {
+    hipLaunchKernelGGL(k_accumulate, grid, block, 0, stream, skey, sval, total, jobs, kbase, njobs, buckets, slot_key, slots, T, nchunks);
+}
+
+}  // namespace bls

@@ -1,0 +1,151 @@
+// Digit, histogram, scan and scatter stages: the device counting sort of
+// (bucket key, point | sign) pairs.
+#include "msm_common.hip.h"
+#include "launch.h"
+
+namespace bls {
+
+__global__ void __launch_bounds__(256)
+k_digits(const job_t* __restrict__ jobs, const uint32_t* __restrict__ pbase, uint32_t njobs,
+         uint32_t total_points, uint32_t* __restrict__ ent, uint32_t* __restrict__ rank,
+         uint32_t* __restrict__ hist) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total_points) return;
+    const uint32_t j = find_owner(pbase, njobs, t);
+    const job_t J = jobs[j];
+    const uint32_t i = t - J.pbase;
+    uint32_t k[9];
+    load_scalar(k, J.scalars + (size_t)i * J.nbytes, J.nbytes, J.nbits, J.vec);
+    // an all-zero affine point is infinity and contributes nothing
+    // (Pointxyzz::dadd with an all-zero affine input, point.cpp:2036-2039)
+    const uint4* pp = J.points + (size_t)i * 6;
+    uint32_t nz = 0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        uint4 v = pp[q];
+        nz |= v.x | v.y | v.z | v.w;
+    }
+    for (uint32_t w = 0; w < J.nwin; w++) {
+        const uint32_t bit0 = J.bit_base + w * J.c;
+        // bit0 == nbits is the top window of an nbits that c divides: it holds
+        // only the carry of the window below (WindowExtract asserts
+        // window_idx <= nbits / window, multi_scalar.h:365)
+        const int32_t d = (nz && bit0 <= J.nbits) ? booth_digit(k, bit0, J.c, J.nbits) : 0;
+        const size_t slot = (size_t)J.ebase + (size_t)w * J.n + i;
+        if (d != 0) {
+            const uint32_t neg = d < 0;
+            const uint32_t b = (uint32_t)(neg ? -d : d) - 1u;
+            const uint32_t key = J.kbase + w * J.B + b;
+            rank[slot] = atomicAdd(&hist[key], 1u);
+            ent[slot] = key | (neg << 31);
+        } else {
+            ent[slot] = kInvalid;
+        }
+    }
+}
+
+__global__ void k_digits_debug(const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes,
+                               uint32_t nbits, uint32_t bit0, uint32_t c, int32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    uint32_t k[9];
+    load_scalar(k, scalars + (size_t)i * nbytes, nbytes, nbits, false);
+    out[i] = booth_digit(k, bit0, c, nbits);
+}
+
+// largest histogram entry (the biggest bucket) -> *out (pre-zeroed)
+__global__ void __launch_bounds__(256)
+k_max_count(const uint32_t* __restrict__ hist, uint32_t N, uint32_t* __restrict__ out) {
+    uint32_t m = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
+        m = max(m, hist[i]);
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off));
+    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
+}
+
+// dst[0..words) = value; a kernel rather than a memset node so the sequence
+// can be captured in a HIP graph
+__global__ void __launch_bounds__(256)
+k_fill(uint32_t* __restrict__ dst, size_t words, uint32_t value) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = value;
+}
+
+__global__ void __launch_bounds__(kScanThreads)
+k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+             uint32_t* __restrict__ tile_sums, uint32_t N) {
+    __shared__ uint32_t sh[kScanThreads];
+    const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+    uint32_t v[kScanItems], run = 0;
+#pragma unroll
+    for (int j = 0; j < kScanItems; j++) {
+        v[j] = (base + j < N) ? in[base + j] : 0u;
+        run += v[j];
+    }
+    sh[threadIdx.x] = run;
+    __syncthreads();
+    for (int off = 1; off < kScanThreads; off <<= 1) {
+        uint32_t t = (threadIdx.x >= (uint32_t)off) ? sh[threadIdx.x - off] : 0u;
+        __syncthreads();
+        sh[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t excl = sh[threadIdx.x] - run;
+#pragma unroll
+    for (int j = 0; j < kScanItems; j++) {
+        if (base + j < N) out[base + j] = excl;
+        excl += v[j];
+    }
+    if (threadIdx.x == kScanThreads - 1 && tile_sums) tile_sums[blockIdx.x] = sh[threadIdx.x];
+}
+
+__global__ void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restrict__ tile_offs, uint32_t N) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) out[i] += tile_offs[i / kScanTile];
+}
+
+__global__ void __launch_bounds__(256)
+k_scatter(const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, uint32_t njobs,
+          uint32_t total_entries, const uint32_t* __restrict__ ent, const uint32_t* __restrict__ rank,
+          const uint32_t* __restrict__ off, uint32_t* __restrict__ skey, uint32_t* __restrict__ sval) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= total_entries) return;
+    const uint32_t e = ent[t];
+    if (e == kInvalid) return;
+    const uint32_t j = find_owner(ebase, njobs, t);
+    const uint32_t i = (t - ebase[j]) % jobs[j].n;
+    const uint32_t key = e & 0x7fffffffu;
+    const uint32_t pos = off[key] + rank[t];
+    skey[pos] = key;
+    sval[pos] = i | (e & 0x80000000u);
+}
+
+void launch_k_digits(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ pbase, uint32_t njobs, uint32_t total_points, uint32_t* __restrict__ ent, uint32_t* __restrict__ rank, uint32_t* __restrict__ hist) {
+    hipLaunchKernelGGL(k_digits, grid, block, 0, stream, jobs, pbase, njobs, total_points, ent, rank, hist);
+}
+
+void launch_k_digits_debug(dim3 grid, dim3 block, hipStream_t stream, const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits, uint32_t bit0, uint32_t c, int32_t* __restrict__ out) {
+    hipLaunchKernelGGL(k_digits_debug, grid, block, 0, stream, scalars, n, nbytes, nbits, bit0, c, out);
+}
+
+void launch_k_max_count(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ hist, uint32_t N, uint32_t* __restrict__ out) {
+    hipLaunchKernelGGL(k_max_count, grid, block, 0, stream, hist, N, out);
+}
+
+void launch_k_fill(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restrict__ dst, size_t words, uint32_t value) {
+    hipLaunchKernelGGL(k_fill, grid, block, 0, stream, dst, words, value);
+}
+
+void launch_k_scan_tiles(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t* __restrict__ tile_sums, uint32_t N) {
+    hipLaunchKernelGGL(k_scan_tiles, grid, block, 0, stream, in, out, tile_sums, N);
+}
+
+void launch_k_scan_add(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restrict__ out, const uint32_t* __restrict__ tile_offs, uint32_t N) {
+    hipLaunchKernelGGL(k_scan_add, grid, block, 0, stream, out, tile_offs, N);
+}
+
+void launch_k_scatter(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, uint32_t njobs, uint32_t total_entries, const uint32_t* __restrict__ ent, const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off, uint32_t* __restrict__ skey, uint32_t* __restrict__ sval) {
+    hipLaunchKernelGGL(k_scatter, grid, block, 0, stream, jobs, ebase, njobs, total_entries, ent, rank, off, skey, sval);
+}
+
+}  // namespace bls

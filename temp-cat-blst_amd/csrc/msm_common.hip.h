@@ -1,0 +1,157 @@
+// Shared device definitions of the G1 Pippenger MSM stages on gfx950.
+//
+//   digits      signed Booth digits of every (window, scalar), a global
+//               histogram of (window, bucket) keys with the rank of each
+//               entry inside its key (WindowExtract / booth_encode,
+//               blst/scalar/curve/multi_scalar.h:350-404, ec_mult.h:63-142)
+//   scan        exclusive prefix sum of the histogram -> bucket offsets
+//   scatter     counting-sort placement of (key, point | sign) pairs
+//   accumulate  fixed-length chunks of the sorted list per lane, XYZZ mixed
+//               additions of gathered affine points (the bucket fill of
+//               points_tile_pippenger, multi_scalar.cpp:806-865); buckets that
+//               cross chunk boundaries leave partial sums in two slots/chunk
+//   partials    log-depth pairwise tree over the partial sums of a bucket
+//   reduce      sum_b b * S_b per window (point_integrate_buckets,
+//               multi_scalar.cpp:730-751) as segment running sums plus a
+//               per-segment scalar weight, then a pairwise tree per window
+//
+// Keys are (window * B + bucket) with B = 2^(c-1) buckets per window, so all
+// windows go through one sort and one accumulate launch.
+#pragma once
+#include "ec.hip.h"
+
+namespace bls {
+
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+
+
+// ---------------------------------------------------------------- helpers
+
+__device__ __forceinline__ void load_aff(aff& a, const uint4* __restrict__ pts, uint32_t idx) {
+    const uint4* s = pts + (size_t)idx * 6;
+    uint4 v[6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) v[k] = s[k];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        a.x.l[4 * k + 0] = v[k].x; a.x.l[4 * k + 1] = v[k].y;
+        a.x.l[4 * k + 2] = v[k].z; a.x.l[4 * k + 3] = v[k].w;
+        a.y.l[4 * k + 0] = v[k + 3].x; a.y.l[4 * k + 1] = v[k + 3].y;
+        a.y.l[4 * k + 2] = v[k + 3].z; a.y.l[4 * k + 3] = v[k + 3].w;
+    }
+}
+
+__device__ __forceinline__ void store_xyzz(xyzz* __restrict__ dst, const xyzz& p) {
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    const fp* f = &p.x;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            d[q * 3 + k] = make_uint4(f[q].l[4 * k], f[q].l[4 * k + 1], f[q].l[4 * k + 2], f[q].l[4 * k + 3]);
+}
+
+__device__ __forceinline__ void load_xyzz(xyzz& p, const xyzz* __restrict__ src) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    fp* f = &p.x;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            uint4 v = s[q * 3 + k];
+            f[q].l[4 * k] = v.x; f[q].l[4 * k + 1] = v.y; f[q].l[4 * k + 2] = v.z; f[q].l[4 * k + 3] = v.w;
+        }
+}
+
+// Scalar i as nine 32-bit words (the ninth is zero padding), bits at and
+// above nbits cleared: the reference reads only ceil(nbits/8) bytes and
+// masks each window to nbits (WindowExtract::initialize, window_end).
+__device__ __forceinline__ void load_scalar(uint32_t k[9], const uint8_t* __restrict__ s,
+                                            uint32_t nbytes, uint32_t nbits, bool vec) {
+#pragma unroll
+    for (int w = 0; w < 9; w++) k[w] = 0;
+    if (vec) {
+        const uint4* s4 = reinterpret_cast<const uint4*>(s);
+        uint4 a = s4[0], b = s4[1];
+        k[0] = a.x; k[1] = a.y; k[2] = a.z; k[3] = a.w;
+        k[4] = b.x; k[5] = b.y; k[6] = b.z; k[7] = b.w;
+    } else {
+        for (uint32_t i = 0; i < nbytes; i++) k[i >> 2] |= (uint32_t)s[i] << (8 * (i & 3));
+    }
+#pragma unroll
+    for (int w = 0; w < 8; w++) {
+        const int lo = 32 * w;
+        if ((int)nbits <= lo) k[w] = 0;
+        else if ((int)nbits < lo + 32) k[w] &= (1u << (nbits - lo)) - 1u;
+    }
+}
+
+// len (<= 25) bits of k starting at bit pos (pos + len <= 257)
+__device__ __forceinline__ uint32_t scalar_bits(const uint32_t k[9], uint32_t pos, uint32_t len) {
+    const uint32_t w = pos >> 5, sh = pos & 31;
+    uint32_t lo = 0, hi = 0;
+#pragma unroll
+    for (int j = 0; j < 9; j++) {
+        lo = (j == (int)w) ? k[j] : lo;
+        hi = (j == (int)w + 1) ? k[j] : hi;
+    }
+    const uint64_t v = (((uint64_t)hi << 32) | lo) >> sh;
+    return (uint32_t)v & ((len >= 32) ? 0xffffffffu : ((1u << len) - 1u));
+}
+
+// Signed Booth digit of the window starting at bit0 (< nbits) of width c:
+// bits [bit0-1, min(bit0+c, nbits)) with a zero below bit 0, encoded with
+// e = min(nbits - bit0 + 1, c) as in booth_encode: digit in [-2^(e-1), 2^(e-1)].
+__device__ __forceinline__ int32_t booth_digit(const uint32_t k[9], uint32_t bit0, uint32_t c,
+                                               uint32_t nbits) {
+    const uint32_t e = min(nbits - bit0 + 1, c);
+    const uint32_t end = min(bit0 + c, nbits);
+    uint32_t wval;
+    if (bit0 == 0) wval = scalar_bits(k, 0, end) << 1;
+    else wval = scalar_bits(k, bit0 - 1, end - bit0 + 1);
+    const uint32_t sign = (wval >> e) & 1u;
+    const int32_t val = (int32_t)((wval + 1) >> 1);
+    return sign ? val - (int32_t)(1u << e) : val;
+}
+
+// ---------------------------------------------------------------- stages
+//
+// A launch covers a batch of MSMs ("jobs").  Job j owns the key range
+// [kbase, kbase + nwin*B), the entry range [ebase, ebase + nwin*n) of the
+// digit arrays, the points [pbase, pbase + n) of the flattened point
+// enumeration and the groups (windows) [gbase, gbase + nwin).  A single MSM
+// is a batch of one.
+
+struct job_t {
+    const uint8_t* scalars;   // device, n * nbytes
+    const uint4* points;      // device, n affine points (96 bytes each)
+    uint32_t n, nbytes, nbits, c, nwin, B, bit_base, vec;
+    uint32_t kbase, ebase, pbase, gbase;
+};
+
+struct group_t {
+    uint32_t kb;   // first key (bucket 0 of this window)
+    uint32_t B;    // buckets
+    uint32_t L;    // buckets per reduction segment
+    uint32_t sb;   // first segment index in the H array
+    uint32_t P;    // segments
+    uint32_t pad[3];
+};
+
+// largest j with base[j] <= t (base ascending, base[0] = 0)
+__device__ __forceinline__ uint32_t find_owner(const uint32_t* __restrict__ base, uint32_t cnt, uint32_t t) {
+    uint32_t lo = 0, hi = cnt;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (base[mid] <= t) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// exclusive scan, 1024 threads x 4 elements per block
+constexpr int kScanThreads = 1024;
+constexpr int kScanItems = 4;
+constexpr int kScanTile = kScanThreads * kScanItems;
+
+}  // namespace bls

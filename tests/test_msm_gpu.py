@@ -233,3 +233,62 @@ def test_linearity_at_2pow18(hip, orc):
     rb = hip.p1s_mult_pippenger(pts, b, 255)
     rs = hip.p1s_mult_pippenger(pts, s, 255)
     assert orc.compress(orc.p1_add(ra, rb)) == orc.compress(rs)
+
+
+def test_batch_uneven_distinct_and_shared_bases(hip, orc, base_points):
+    """One launch sequence over MSMs of different sizes, windows and nbits,
+    some sharing their bases; every result equals its own oracle MSM."""
+    rnd = random.Random(77)
+    sizes = [1, 2, 3, 64, 100, 257, 1000, 4096, 4096, 50, 0, 7]
+    nbits_l = [255, 255, 8, 255, 64, 255, 255, 255, 255, 17, 255, 256]
+    windows = [0, 0, 0, 5, 0, 3, 0, 0, 11, 0, 0, 2]
+    shared = hip.DeviceBuffer(base_points)
+    keep, jobs, want = [shared], [], []
+    for k, (n, nb, w) in enumerate(zip(sizes, nbits_l, windows)):
+        nbytes = (nb + 7) // 8
+        sc = np.array([[rnd.getrandbits(8) for _ in range(nbytes)] for _ in range(max(n, 1))], np.uint8)
+        if k in (7, 8):            # two MSMs over the shared, resident bases
+            pts = np.ascontiguousarray(base_points[:n])
+            dp = shared
+        else:
+            pts = orc.gen_points(max(n, 1), 500 + k, threads=4)
+            dp = hip.DeviceBuffer(pts)
+            keep.append(dp)
+        ds = hip.DeviceBuffer(sc)
+        keep.append(ds)
+        jobs.append((dp.ptr, ds.ptr, n, nb, w))
+        want.append(orc.compress(orc.msm_pippenger(pts[:n], sc[:n], n, nb)) if n else bytes([0xC0]) + bytes(47))
+    try:
+        got = hip.msm_batch_device(jobs)
+    finally:
+        for b in keep:
+            b.close()
+    for k in range(len(jobs)):
+        assert hip.compress(got[k]) == want[k], k
+
+
+def test_graph_plan_replays(hip, orc, base_points):
+    """A captured batch replayed several times with new scalars written into
+    the same device buffers between replays, with eager work in between;
+    every replay equals the eager batch and the oracle."""
+    srs = hip.DeviceBuffer(base_points)
+    sizes = [4096, 4096, 1000, 33]
+    scal = [hip.DeviceBuffer(np.zeros((n, 32), np.uint8)) for n in sizes]
+    jobs = [(srs.ptr, scal[k].ptr, n, 255) for k, n in enumerate(sizes)]
+    plan = hip.BatchPlan(jobs)
+    try:
+        for rep in range(3):
+            host_sc = [orc.gen_scalars(n, 900 + 10 * rep + k) for k, n in enumerate(sizes)]
+            for b, h in zip(scal, host_sc):
+                b.write(h)
+            got = plan.run()
+            eager = hip.msm_batch_device(jobs)
+            for k, n in enumerate(sizes):
+                want = orc.compress(orc.msm_pippenger(np.ascontiguousarray(base_points[:n]), host_sc[k], n, 255))
+                assert hip.compress(got[k]) == want, (rep, k)
+                assert hip.compress(eager[k]) == want, (rep, k)
+    finally:
+        plan.close()
+        srs.close()
+        for b in scal:
+            b.close()
