@@ -61,10 +61,19 @@ def load_traffic():
     return None
 
 
-def cpu_baseline(points, scalars, n, gpu_compressed):
-    """The oracle (C port of blst_p1s_mult_pippenger) on the host cores."""
+def cpu_baseline(points, scalars, n, gpu_compressed, samples):
+    """The oracle (C port of blst_p1s_mult_pippenger) on the host cores, and
+    the checker of sampled batched results."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    import blst_hip as bh
+
+    checked = {}
+    for name, items in samples.items():
+        ok = True
+        for pts, sc, res in items:
+            ok = ok and oracle.compress(oracle.msm_pippenger(pts, sc, pts.shape[0], 255, threads=16)) == bh.compress(res)
+        checked[name] = ok
 
     t = time.perf_counter()
     r = oracle.msm_pippenger(points, scalars, n, 255, threads=1)
@@ -85,6 +94,7 @@ def cpu_baseline(points, scalars, n, gpu_compressed):
         "all_cores": {"value": n / dt_mt, "threads": threads, "seconds": dt_mt},
         "bit_exact_vs_gpu": cpu_bytes == gpu_compressed,
         "compressed": cpu_bytes.hex(),
+        "batched_samples_bit_exact": checked,
     }
 
 
@@ -115,6 +125,10 @@ def main():
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--window", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-batched", action="store_true", help="skip the batched workloads")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-rate", type=float, default=600.0, help="offered request rate (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -204,12 +218,31 @@ def main():
     }
     gpu_bytes = bh.compress(r)
     out["compressed"] = gpu_bytes.hex()
-    if not args.no_cpu:
-        cb = cpu_baseline(points, scalars, n, gpu_bytes)
-        out["cpu_baseline"] = cb
-        out["bit_exact"] = cb["bit_exact_vs_gpu"]
     dp.close()
     ds.close()
+
+    import workloads
+
+    samples = {}
+    if not args.no_batched:
+        kz = workloads.kzg()
+        sm = kz.pop("sample")
+        samples["kzg"] = [(sm["srs"], sm["coeffs"][i], sm["results"][i]) for i in range(2)]
+        print("kzg: %.2f ms batched, %.1fx" % (kz["batched_ms"], kz["speedup"]), file=sys.stderr, flush=True)
+        un = workloads.uneven()
+        sm = un.pop("sample")
+        samples["uneven"] = [(sm["points"], sm["scalars"], sm["result"])]
+        print("uneven: %.2f ms batched, %.1fx" % (un["batched_ms"], un["speedup"]), file=sys.stderr, flush=True)
+        out["batched"] = {"kzg": kz, "uneven": un}
+    if not args.no_serving:
+        sv = workloads.serving(args.serve_rate, args.serve_seconds)
+        print("serving: %.3g pts/s, p99 %.1f ms, %.1fx" % (sv["served_points_per_s"], sv["served_latency_ms"]["p99"],
+                                                           sv["speedup"]), file=sys.stderr, flush=True)
+        out["serving"] = sv
+    if not args.no_cpu:
+        cb = cpu_baseline(points, scalars, n, gpu_bytes, samples)
+        out["cpu_baseline"] = cb
+        out["bit_exact"] = cb["bit_exact_vs_gpu"] and all(cb["batched_samples_bit_exact"].values())
     print(json.dumps(out), flush=True)
 
 

@@ -118,6 +118,36 @@ blst_hip_batch_plan *blst_hip_batch_plan_create(const blst_hip_msm_job *jobs, si
 int blst_hip_batch_plan_run(blst_hip_batch_plan *plan, blst_p1 *rets);
 void blst_hip_batch_plan_destroy(blst_hip_batch_plan *plan);
 
+/* Serving loop: continuous batching of MSM requests on the current device.
+ * hbm_budget_bytes bounds the device memory of admitted requests (inputs
+ * plus batch workspace; 0 = 80% of the free memory at creation) and
+ * max_batch_points the points of one batch (0 = 2^24).  Requests are admitted
+ * in arrival order as memory frees up, copied on a copy stream, and launched
+ * in batches on two alternating streams, so requests that arrive while a
+ * batch runs join the next one.  Results are identical to a lone MSM. */
+typedef struct blst_hip_server blst_hip_server;
+blst_hip_server *blst_hip_server_create(size_t hbm_budget_bytes, size_t max_batch_points);
+void blst_hip_server_destroy(blst_hip_server *server);
+/* Copies npoints bases to the device once (e.g. a KZG setup); returns a base
+ * id >= 0 for later requests, or -1. */
+int blst_hip_server_add_bases(blst_hip_server *server, const blst_p1_affine *points, size_t npoints);
+/* Queues one MSM over the first npoints resident bases of base_id, or, for
+ * base_id < 0, over the npoints host points given.  scalars holds npoints *
+ * ceil(nbits/8) host bytes.  The host buffers must stay valid until
+ * blst_hip_server_wait returns for the ticket.  Returns a ticket > 0, or -1. */
+int64_t blst_hip_server_submit(blst_hip_server *server, int base_id, const blst_p1_affine *points,
+                               const blst_byte *scalars, size_t npoints, size_t nbits);
+/* Blocks until the ticket completes; writes the result and the latency from
+ * submission to completion (ms).  Returns 0 on success. */
+int blst_hip_server_wait(blst_hip_server *server, int64_t ticket, blst_p1 *ret, double *latency_ms);
+/* [submitted, completed, batches, max batches in flight, peak admitted
+ * bytes, HBM budget] */
+void blst_hip_server_stats(blst_hip_server *server, uint64_t out[6]);
+
+/* Page-locked host memory (asynchronous copies to the device). */
+void *blst_hip_host_alloc(size_t bytes);
+void blst_hip_host_free(void *host_ptr);
+
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
  * written to out[npoints] (host).  Returns 0 on success. */

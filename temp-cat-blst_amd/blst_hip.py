@@ -38,6 +38,14 @@ SIGNATURES = {
     "blst_hip_batch_plan_create": ([_vp, _sz], _vp),
     "blst_hip_batch_plan_run": ([_vp, _vp], ctypes.c_int),
     "blst_hip_batch_plan_destroy": ([_vp], None),
+    "blst_hip_server_create": ([_sz, _sz], _vp),
+    "blst_hip_server_destroy": ([_vp], None),
+    "blst_hip_server_add_bases": ([_vp, _vp, _sz], ctypes.c_int),
+    "blst_hip_server_submit": ([_vp, ctypes.c_int, _vp, _vp, _sz, _sz], ctypes.c_int64),
+    "blst_hip_server_wait": ([_vp, ctypes.c_int64, _vp, _vp], ctypes.c_int),
+    "blst_hip_server_stats": ([_vp, _vp], None),
+    "blst_hip_host_alloc": ([_sz], _vp),
+    "blst_hip_host_free": ([_vp], None),
 }
 
 _lib = None
@@ -248,6 +256,71 @@ class BatchPlan:
     def close(self):
         if self.ptr:
             load().blst_hip_batch_plan_destroy(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+class PinnedArray:
+    """A numpy view of page-locked host memory (freed on close)."""
+
+    def __init__(self, shape, dtype):
+        self.nbytes = int(np.prod(shape)) * np.dtype(dtype).itemsize
+        self.ptr = load().blst_hip_host_alloc(max(self.nbytes, 1))
+        if not self.ptr:
+            raise RuntimeError("blst_hip_host_alloc: " + last_error())
+        buf = (ctypes.c_uint8 * max(self.nbytes, 1)).from_address(self.ptr)
+        self.array = np.frombuffer(buf, dtype=np.uint8, count=self.nbytes).view(dtype).reshape(shape)
+
+    def close(self):
+        if self.ptr:
+            self.array = None
+            load().blst_hip_host_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
+
+
+class Server:
+    """The continuous-batching serving loop (blst_hip_server_*)."""
+
+    def __init__(self, hbm_budget=0, max_batch_points=0):
+        self.ptr = load().blst_hip_server_create(hbm_budget, max_batch_points)
+        if not self.ptr:
+            raise RuntimeError("blst_hip_server_create: " + last_error())
+        self._keep = {}
+
+    def add_bases(self, points):
+        points = np.ascontiguousarray(points, dtype=np.uint64)
+        bid = load().blst_hip_server_add_bases(ctypes.c_void_p(self.ptr), _ptr(points), points.shape[0])
+        if bid < 0:
+            raise RuntimeError("blst_hip_server_add_bases: " + last_error())
+        return bid
+
+    def submit(self, scalars, nbits, base_id=-1, points=None, n=None):
+        """scalars/points must stay alive until wait(); kept referenced here."""
+        n = scalars.shape[0] if n is None else n
+        pp = _ptr(points) if points is not None else None
+        t = load().blst_hip_server_submit(ctypes.c_void_p(self.ptr), base_id, pp, _ptr(scalars), n, nbits)
+        if t < 0:
+            raise RuntimeError("blst_hip_server_submit: " + last_error())
+        self._keep[t] = (scalars, points)
+        return t
+
+    def wait(self, ticket):
+        ret = np.zeros(18, np.uint64)
+        lat = ctypes.c_double(0)
+        rc = load().blst_hip_server_wait(ctypes.c_void_p(self.ptr), ticket, _ptr(ret), ctypes.byref(lat))
+        self._keep.pop(ticket, None)
+        if rc:
+            raise RuntimeError("blst_hip_server_wait: " + last_error())
+        return ret, lat.value
+
+    def stats(self):
+        out = np.zeros(6, np.uint64)
+        load().blst_hip_server_stats(ctypes.c_void_p(self.ptr), _ptr(out))
+        return dict(zip(["submitted", "completed", "batches", "max_inflight", "peak_bytes", "budget"],
+                        [int(x) for x in out]))
+
+    def close(self):
+        if self.ptr:
+            load().blst_hip_server_destroy(ctypes.c_void_p(self.ptr))
             self.ptr = None
 
 

@@ -6,8 +6,7 @@
 // (blst/c_api.cpp:1382-1418, 1546-1611): they are the only part of the
 // reference's control flow that callers can observe.  Everything past the
 // checks is this backend's own pipeline.
-#include "launch.h"
-#include "host_ec.h"
+#include "pipeline.h"
 #include "../../include/blst_hip.h"
 
 #include <algorithm>
@@ -21,58 +20,11 @@
 
 using namespace bls;
 
-namespace {
+namespace bls {
+namespace host {
 
 thread_local std::string g_err;
 thread_local double g_phase[7];
-
-struct HipFail {
-    std::string msg;
-};
-
-#define HIPCK(expr)                                                                   \
-    do {                                                                              \
-        hipError_t e_ = (expr);                                                       \
-        if (e_ != hipSuccess)                                                         \
-            throw HipFail{std::string(#expr) + ": " + hipGetErrorString(e_)};         \
-    } while (0)
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t cap = 0;
-    template <class T> T* as() const { return static_cast<T*>(p); }
-    void ensure(size_t bytes) {
-        if (bytes <= cap && p) return;
-        if (p) HIPCK(hipFree(p));
-        p = nullptr;
-        cap = 0;
-        const size_t want = std::max<size_t>(bytes + bytes / 8, 4096);
-        HIPCK(hipMalloc(&p, want));
-        cap = want;
-    }
-    ~DevBuf() { if (p) (void)hipFree(p); }
-};
-
-// Device memory and stream of one in-flight batch.  The per-thread context,
-// each captured batch plan and each serving slot own one.
-struct Workspace {
-    hipStream_t stream = nullptr;
-    DevBuf ent, rank, hist, off, skey, sval, buckets, slot_key, slots, H, scan_tmp, W, res, tables, maxcnt;
-    std::vector<uint8_t> tables_host;
-    uint32_t* host_word = nullptr;       // pinned readback word
-    hipEvent_t ev[8] = {};
-    void init() {
-        HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        for (auto& e : ev) HIPCK(hipEventCreate(&e));
-        HIPCK(hipHostMalloc((void**)&host_word, 64, hipHostMallocDefault));
-        maxcnt.ensure(64);
-    }
-    ~Workspace() {
-        if (host_word) (void)hipHostFree(host_word);
-        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
 
 struct Ctx {
     int device = 0;
@@ -132,27 +84,6 @@ uint32_t device_window(size_t n, size_t nbits) {
     return bc;
 }
 
-// One MSM of a batch as the host sees it.
-struct JobSpec {
-    const void* points;    // device, n affine points
-    const void* scalars;   // device, n * ceil(nbits/8) bytes
-    uint32_t n, nbits, c;
-    uint32_t bit_base;     // tile mode: first bit of the single window
-    bool tile;             // one window at bit_base, no window fold
-};
-
-constexpr uint32_t kChunk = 32;     // sorted entries per accumulate lane
-constexpr uint32_t kSegment = 8;    // buckets per reduction lane
-
-// Key, entry, point, group and segment ranges of every job of a batch.
-struct Layout {
-    std::vector<job_t> jobs;
-    std::vector<uint32_t> pbase, ebase, kbase;
-    std::vector<group_t> groups;
-    std::vector<uint32_t> sbase;
-    uint32_t total_points = 0, total_entries = 0, nkeys = 0, total_segments = 0, max_n = 0;
-};
-
 Layout make_layout(const std::vector<JobSpec>& specs) {
     Layout L;
     uint64_t pts = 0, ent = 0, keys = 0, segs = 0;
@@ -200,13 +131,6 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
     L.total_segments = (uint32_t)segs;
     return L;
 }
-
-// Device copies of the layout tables (one upload per batch).
-struct DevLayout {
-    const job_t* jobs;
-    const uint32_t *pbase, *ebase, *kbase, *sbase;
-    const group_t* groups;
-};
 
 DevLayout upload_layout(Workspace& cx, const Layout& L) {
     const size_t nj = L.jobs.size(), ng = L.groups.size();
@@ -310,7 +234,7 @@ size_t workspace_bytes(const Layout& L) {
 // group); otherwise k_fold combines the windows of every job and one xyzz per
 // job is copied.  host_out should be pinned for the copy to be asynchronous.
 void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int rounds, bool host_fold, void* host_out,
-                   bool timing = true) {
+                   bool timing) {
     const uint32_t nkeys = L.nkeys;
     const uint32_t nent = L.total_entries;
     const uint32_t nchunks = std::max<uint32_t>(div_up(nent, kChunk), 1);
@@ -568,7 +492,10 @@ bool specs_from_jobs(const blst_hip_msm_job* jobs, size_t njobs, std::vector<Job
     return true;
 }
 
-}  // namespace
+}  // namespace host
+}  // namespace bls
+
+using namespace bls::host;
 
 struct blst_hip_batch_plan {
     PlanImpl impl;
@@ -811,6 +738,20 @@ int blst_hip_device_write(void* device_dst, const void* host_src, size_t bytes) 
     }
     g_err.clear();
     return 0;
+}
+
+void* blst_hip_host_alloc(size_t bytes) {
+    void* p = nullptr;
+    if (bytes == 0 || hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess) {
+        g_err = "hipHostMalloc failed";
+        return nullptr;
+    }
+    g_err.clear();
+    return p;
+}
+
+void blst_hip_host_free(void* host_ptr) {
+    if (host_ptr) (void)hipHostFree(host_ptr);
 }
 
 void blst_hip_device_free(void* device_ptr) {

@@ -1,0 +1,104 @@
+// Host-side pipeline of the gfx950 MSM backend, shared by the C ABI entry
+// points, the captured batch plans and the serving loop: device buffers,
+// batch layouts and the enqueue of the device stages.
+#pragma once
+#include "launch.h"
+#include "host_ec.h"
+
+#include <string>
+#include <vector>
+
+namespace bls {
+namespace host {
+
+extern thread_local std::string g_err;
+extern thread_local double g_phase[7];
+
+struct HipFail {
+    std::string msg;
+};
+
+#define HIPCK(expr)                                                                   \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess)                                                         \
+            throw HipFail{std::string(#expr) + ": " + hipGetErrorString(e_)};         \
+    } while (0)
+
+struct DevBuf {
+    void* p = nullptr;
+    size_t cap = 0;
+    template <class T> T* as() const { return static_cast<T*>(p); }
+    void ensure(size_t bytes) {
+        if (bytes <= cap && p) return;
+        if (p) HIPCK(hipFree(p));
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes + bytes / 8, 4096);
+        HIPCK(hipMalloc(&p, want));
+        cap = want;
+    }
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+// Device memory and stream of one in-flight batch.  The per-thread context,
+// each captured batch plan and each serving slot own one.
+struct Workspace {
+    hipStream_t stream = nullptr;
+    DevBuf ent, rank, hist, off, skey, sval, buckets, slot_key, slots, H, scan_tmp, W, res, tables, maxcnt;
+    std::vector<uint8_t> tables_host;
+    uint32_t* host_word = nullptr;       // pinned readback word
+    hipEvent_t ev[8] = {};
+    void init() {
+        HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto& e : ev) HIPCK(hipEventCreate(&e));
+        HIPCK(hipHostMalloc((void**)&host_word, 64, hipHostMallocDefault));
+        maxcnt.ensure(64);
+    }
+    ~Workspace() {
+        if (host_word) (void)hipHostFree(host_word);
+        for (auto& e : ev) if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+// One MSM of a batch as the host sees it.
+struct JobSpec {
+    const void* points;    // device, n affine points
+    const void* scalars;   // device, n * ceil(nbits/8) bytes
+    uint32_t n, nbits, c;
+    uint32_t bit_base;     // tile mode: first bit of the single window
+    bool tile;             // one window at bit_base, no window fold
+};
+
+constexpr uint32_t kChunk = 32;     // sorted entries per accumulate lane
+constexpr uint32_t kSegment = 8;    // buckets per reduction lane
+
+// Key, entry, point, group and segment ranges of every job of a batch.
+struct Layout {
+    std::vector<job_t> jobs;
+    std::vector<uint32_t> pbase, ebase, kbase;
+    std::vector<group_t> groups;
+    std::vector<uint32_t> sbase;
+    uint32_t total_points = 0, total_entries = 0, nkeys = 0, total_segments = 0, max_n = 0;
+};
+
+// Device copies of the layout tables (one upload per batch).
+struct DevLayout {
+    const job_t* jobs;
+    const uint32_t *pbase, *ebase, *kbase, *sbase;
+    const group_t* groups;
+};
+
+uint32_t device_window(size_t n, size_t nbits);
+Layout make_layout(const std::vector<JobSpec>& specs);
+DevLayout upload_layout(Workspace& cx, const Layout& L);
+void ensure_workspace(Workspace& cx, const Layout& L);
+size_t workspace_bytes(const Layout& L);
+uint32_t bound_rounds(uint32_t max_n);
+void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int rounds, bool host_fold, void* host_out,
+                   bool timing = true);
+void record_phases(Workspace& cx);
+
+}  // namespace host
+}  // namespace bls

@@ -1,0 +1,468 @@
+// Serving loop: continuous batching of MSM requests on one GPU.
+//
+// Clients submit requests (over resident bases such as a KZG setup, or over
+// fresh bases that travel with the request) and wait for tickets.  A
+// scheduler thread
+//   * admits pending requests in arrival order while their device footprint
+//     (inputs plus the batch workspace the request adds) fits the HBM budget,
+//   * copies the inputs of admitted requests on a copy stream,
+//   * whenever one of two batch slots is free, launches every admitted
+//     request whose copies have landed as one batch (one digit pass, one
+//     sort, one accumulation, one reduction and a device window fold over all
+//     of them), on that slot's own stream, so a batch can start while the
+//     other is still on the device,
+//   * completes the tickets of finished batches and returns their memory.
+// Requests that arrive while batches run join the next batch as soon as
+// memory and a slot allow.
+#include "pipeline.h"
+#include "../../include/blst_hip.h"
+
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+
+namespace bls {
+namespace host {
+namespace {
+
+using clock_t_ = std::chrono::steady_clock;
+
+// Device buffers recycled by capacity: hipFree synchronises the device, so
+// the loop never frees while serving.
+class BufferPool {
+public:
+    ~BufferPool() {
+        for (auto& kv : free_) (void)hipFree(kv.second);
+    }
+    void* get(size_t bytes, size_t& cap) {
+        cap = round_up(bytes);
+        auto it = free_.find(cap);
+        if (it != free_.end()) {
+            void* p = it->second;
+            free_.erase(it);
+            return p;
+        }
+        void* p = nullptr;
+        HIPCK(hipMalloc(&p, cap));
+        allocated_ += cap;
+        return p;
+    }
+    void put(void* p, size_t cap) { free_.emplace(cap, p); }
+    size_t allocated() const { return allocated_; }
+
+private:
+    static size_t round_up(size_t b) {
+        size_t c = 1 << 16;
+        while (c < b) c <<= 1;
+        return c;
+    }
+    std::multimap<size_t, void*> free_;
+    size_t allocated_ = 0;
+};
+
+struct Request {
+    int64_t ticket = 0;
+    int base_id = -1;
+    const void* host_points = nullptr;
+    const void* host_scalars = nullptr;
+    size_t n = 0, nbits = 0;
+    size_t need = 0;                  // admitted footprint
+    void* d_points = nullptr;         // fresh bases only
+    size_t d_points_cap = 0;
+    void* d_scalars = nullptr;
+    size_t d_scalars_cap = 0;
+    hipEvent_t copied = nullptr;
+    bool copy_issued = false;         // set by the copier thread
+    clock_t_::time_point t_submit;
+    bool done = false;
+    int status = 0;
+    blst_p1 result = {};
+    double latency_ms = 0;
+};
+
+struct Slot {
+    Workspace ws;
+    bool busy = false;
+    Layout L;
+    std::vector<std::shared_ptr<Request>> reqs;
+    hostec::xyzz6* host_res = nullptr;
+    size_t host_res_cap = 0;
+    hipEvent_t done = nullptr;
+};
+
+}  // namespace
+
+class Server {
+public:
+    Server(size_t budget, size_t max_batch_points) : max_batch_points_(max_batch_points) {
+        HIPCK(hipGetDevice(&device_));
+        size_t free_b = 0, total_b = 0;
+        HIPCK(hipMemGetInfo(&free_b, &total_b));
+        budget_ = budget ? budget : (size_t)(0.8 * (double)free_b);
+        HIPCK(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
+        for (auto& s : slots_) {
+            s = std::make_unique<Slot>();
+            s->ws.init();
+            HIPCK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
+        }
+        worker_ = std::thread([this] { loop(); });
+        copier_ = std::thread([this] { copy_loop(); });
+    }
+
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        copy_cv_.notify_all();
+        worker_.join();
+        copier_.join();
+        for (auto& s : slots_) {
+            if (s->host_res) (void)hipHostFree(s->host_res);
+            if (s->done) (void)hipEventDestroy(s->done);
+        }
+        for (auto& b : bases_) (void)hipFree(b.ptr);
+        (void)hipStreamDestroy(copy_stream_);
+    }
+
+    int add_bases(const blst_p1_affine* pts, size_t n) {
+        std::lock_guard<std::mutex> lk(mu_);
+        Base b;
+        b.n = n;
+        HIPCK(hipSetDevice(device_));
+        HIPCK(hipMalloc(&b.ptr, n * sizeof(blst_p1_affine)));
+        HIPCK(hipMemcpy(b.ptr, pts, n * sizeof(blst_p1_affine), hipMemcpyHostToDevice));
+        used_ += n * sizeof(blst_p1_affine);
+        bases_.push_back(b);
+        return (int)bases_.size() - 1;
+    }
+
+    int64_t submit(int base_id, const blst_p1_affine* pts, const blst_byte* scalars, size_t n, size_t nbits) {
+        auto r = std::make_shared<Request>();
+        r->base_id = base_id;
+        r->host_points = pts;
+        r->host_scalars = scalars;
+        r->n = n;
+        r->nbits = nbits;
+        r->t_submit = clock_t_::now();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            if (base_id >= 0 && ((size_t)base_id >= bases_.size() || n > bases_[base_id].n)) return -1;
+            r->ticket = ++next_ticket_;
+            r->need = footprint(*r);
+            if (r->need > budget_) return -1;
+            tickets_[r->ticket] = r;
+            pending_.push_back(r);
+            submitted_++;
+        }
+        cv_.notify_all();
+        return r->ticket;
+    }
+
+    int wait(int64_t ticket, blst_p1* ret, double* latency_ms) {
+        std::unique_lock<std::mutex> lk(mu_);
+        auto it = tickets_.find(ticket);
+        if (it == tickets_.end()) return -1;
+        std::shared_ptr<Request> r = it->second;
+        done_cv_.wait(lk, [&] { return r->done; });
+        *ret = r->result;
+        if (latency_ms) *latency_ms = r->latency_ms;
+        tickets_.erase(it);
+        if (r->status) last_error_ = "request failed: " + fail_msg_;
+        return r->status;
+    }
+
+    void stats(uint64_t out[6]) {
+        std::lock_guard<std::mutex> lk(mu_);
+        out[0] = submitted_;
+        out[1] = completed_;
+        out[2] = batches_;
+        out[3] = max_inflight_;
+        out[4] = peak_used_;
+        out[5] = budget_;
+    }
+
+    std::string last_error_;
+
+private:
+    struct Base {
+        void* ptr = nullptr;
+        size_t n = 0;
+    };
+
+    // device bytes of a request: its inputs plus the share of a batch
+    // workspace a job of its size adds
+    size_t footprint(const Request& r) const {
+        JobSpec js;
+        js.points = nullptr;
+        js.scalars = nullptr;
+        js.n = (uint32_t)r.n;
+        js.nbits = (uint32_t)r.nbits;
+        js.c = device_window(r.n, r.nbits);
+        js.bit_base = 0;
+        js.tile = false;
+        Layout L = make_layout(std::vector<JobSpec>(1, js));
+        size_t in = r.n * ((r.nbits + 7) / 8) + 64;
+        if (r.base_id < 0) in += r.n * sizeof(blst_p1_affine);
+        return in + workspace_bytes(L);
+    }
+
+    // under mu_: admit in arrival order while the budget allows; the copies
+    // are issued by the copier thread (a copy from pageable host memory can
+    // block the issuing thread for its whole duration)
+    void admit() {
+        while (!pending_.empty()) {
+            auto r = pending_.front();
+            if (used_ + r->need > budget_ && (inflight_reqs_ > 0 || !admitted_.empty())) break;
+            pending_.pop_front();
+            used_ += r->need;
+            peak_used_ = std::max(peak_used_, used_);
+            const size_t sbytes = r->n * ((r->nbits + 7) / 8);
+            r->d_scalars = pool_.get(sbytes + 16, r->d_scalars_cap);
+            if (r->base_id < 0) r->d_points = pool_.get(r->n * sizeof(blst_p1_affine), r->d_points_cap);
+            HIPCK(hipEventCreateWithFlags(&r->copied, hipEventDisableTiming));
+            admitted_.push_back(r);
+            to_copy_.push_back(r);
+            copy_cv_.notify_one();
+        }
+    }
+
+    void copy_loop() {
+        (void)hipSetDevice(device_);
+        std::unique_lock<std::mutex> lk(mu_);
+        while (true) {
+            copy_cv_.wait(lk, [&] { return stop_ || !to_copy_.empty(); });
+            if (to_copy_.empty()) {
+                if (stop_) return;
+                continue;
+            }
+            auto r = to_copy_.front();
+            to_copy_.pop_front();
+            lk.unlock();
+            bool ok = true;
+            const size_t sbytes = r->n * ((r->nbits + 7) / 8);
+            ok = ok && hipMemcpyAsync(r->d_scalars, r->host_scalars, sbytes, hipMemcpyHostToDevice,
+                                      copy_stream_) == hipSuccess;
+            if (r->base_id < 0)
+                ok = ok && hipMemcpyAsync(r->d_points, r->host_points, r->n * sizeof(blst_p1_affine),
+                                          hipMemcpyHostToDevice, copy_stream_) == hipSuccess;
+            ok = ok && hipEventRecord(r->copied, copy_stream_) == hipSuccess;
+            lk.lock();
+            if (!ok) copy_failed_ = true;
+            r->copy_issued = true;
+            cv_.notify_all();
+        }
+    }
+
+    // under mu_: one batch of copied requests into a free slot
+    void launch(Slot& s) {
+        std::vector<std::shared_ptr<Request>> batch;
+        std::vector<JobSpec> specs;
+        size_t pts = 0;
+        for (auto it = admitted_.begin(); it != admitted_.end();) {
+            auto& r = *it;
+            if (!r->copy_issued || hipEventQuery(r->copied) != hipSuccess) break;   // keep arrival order
+            if (!batch.empty() && pts + r->n > max_batch_points_) break;
+            JobSpec js;
+            js.points = r->base_id >= 0 ? bases_[r->base_id].ptr : r->d_points;
+            js.scalars = r->d_scalars;
+            js.n = (uint32_t)r->n;
+            js.nbits = (uint32_t)r->nbits;
+            js.c = device_window(r->n, r->nbits);
+            js.bit_base = 0;
+            js.tile = false;
+            specs.push_back(js);
+            batch.push_back(r);
+            pts += r->n;
+            it = admitted_.erase(it);
+        }
+        if (batch.empty()) return;
+        s.L = make_layout(specs);
+        ensure_workspace(s.ws, s.L);
+        const DevLayout D = upload_layout(s.ws, s.L);
+        if (s.host_res_cap < batch.size()) {
+            if (s.host_res) HIPCK(hipHostFree(s.host_res));
+            s.host_res_cap = std::max<size_t>(batch.size(), 64);
+            HIPCK(hipHostMalloc((void**)&s.host_res, s.host_res_cap * sizeof(hostec::xyzz6), hipHostMallocDefault));
+        }
+        enqueue_batch(s.ws, s.L, D, (int)bound_rounds(s.L.max_n), false, s.host_res, false);
+        HIPCK(hipEventRecord(s.done, s.ws.stream));
+        s.reqs = std::move(batch);
+        s.busy = true;
+        inflight_reqs_ += s.reqs.size();
+        batches_++;
+        int inflight = 0;
+        for (auto& o : slots_) inflight += o->busy ? 1 : 0;
+        max_inflight_ = std::max<uint64_t>(max_inflight_, inflight);
+    }
+
+    // under mu_: completion of a finished slot
+    void complete(Slot& s) {
+        const auto now = clock_t_::now();
+        for (size_t k = 0; k < s.reqs.size(); k++) {
+            auto& r = s.reqs[k];
+            hostec::jac6 j;
+            hostec::to_jac(j, s.host_res[k]);
+            std::memcpy(&r->result, &j, sizeof(j));
+            r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+            release(*r);
+            r->done = true;
+            completed_++;
+        }
+        inflight_reqs_ -= s.reqs.size();
+        s.reqs.clear();
+        s.busy = false;
+        done_cv_.notify_all();
+    }
+
+    void release(Request& r) {
+        if (r.d_scalars) pool_.put(r.d_scalars, r.d_scalars_cap);
+        if (r.d_points) pool_.put(r.d_points, r.d_points_cap);
+        r.d_scalars = r.d_points = nullptr;
+        if (r.copied) (void)hipEventDestroy(r.copied);
+        r.copied = nullptr;
+        used_ -= r.need;
+    }
+
+    void fail_all(const std::string& msg) {
+        fail_msg_ = msg;
+        auto fail = [&](std::shared_ptr<Request>& r) {
+            release(*r);
+            r->status = -1;
+            r->done = true;
+        };
+        for (auto& r : pending_) { r->status = -1; r->done = true; }
+        pending_.clear();
+        for (auto& r : admitted_) fail(r);
+        admitted_.clear();
+        to_copy_.clear();
+        copy_failed_ = false;
+        for (auto& s : slots_) {
+            for (auto& r : s->reqs) fail(r);
+            s->reqs.clear();
+            s->busy = false;
+        }
+        inflight_reqs_ = 0;
+        done_cv_.notify_all();
+    }
+
+    void loop() {
+        (void)hipSetDevice(device_);
+        std::unique_lock<std::mutex> lk(mu_);
+        while (true) {
+            try {
+                if (copy_failed_) throw HipFail{"host to device copy failed"};
+                bool progress = false;
+                for (auto& s : slots_) {
+                    if (s->busy && hipEventQuery(s->done) == hipSuccess) {
+                        complete(*s);
+                        progress = true;
+                    }
+                }
+                admit();
+                for (auto& s : slots_) {
+                    if (!s->busy && !admitted_.empty()) {
+                        launch(*s);
+                        progress = progress || s->busy;
+                    }
+                }
+                const bool busy = inflight_reqs_ > 0 || !admitted_.empty();
+                if (stop_ && !busy && pending_.empty()) break;
+                if (!progress) {
+                    if (busy) cv_.wait_for(lk, std::chrono::microseconds(50));
+                    else cv_.wait(lk, [&] { return stop_ || !pending_.empty(); });
+                }
+            } catch (const HipFail& f) {
+                fail_all(f.msg);
+            }
+        }
+    }
+
+    int device_ = 0;
+    size_t budget_ = 0, used_ = 0, peak_used_ = 0, max_batch_points_;
+    hipStream_t copy_stream_ = nullptr;
+    std::unique_ptr<Slot> slots_[2];
+    std::vector<Base> bases_;
+    BufferPool pool_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::deque<std::shared_ptr<Request>> pending_;
+    std::deque<std::shared_ptr<Request>> admitted_;
+    std::deque<std::shared_ptr<Request>> to_copy_;
+    std::condition_variable copy_cv_;
+    bool copy_failed_ = false;
+    std::thread copier_;
+    std::map<int64_t, std::shared_ptr<Request>> tickets_;
+    int64_t next_ticket_ = 0;
+    size_t inflight_reqs_ = 0;
+    uint64_t submitted_ = 0, completed_ = 0, batches_ = 0, max_inflight_ = 0;
+    std::string fail_msg_;
+    bool stop_ = false;
+    std::thread worker_;
+};
+
+}  // namespace host
+}  // namespace bls
+
+struct blst_hip_server {
+    std::unique_ptr<bls::host::Server> impl;
+};
+
+extern "C" {
+
+blst_hip_server* blst_hip_server_create(size_t hbm_budget_bytes, size_t max_batch_points) {
+    try {
+        auto s = std::make_unique<blst_hip_server>();
+        s->impl = std::make_unique<bls::host::Server>(hbm_budget_bytes,
+                                                      max_batch_points ? max_batch_points : ((size_t)1 << 24));
+        bls::host::g_err.clear();
+        return s.release();
+    } catch (const bls::host::HipFail& f) {
+        bls::host::g_err = f.msg;
+        return nullptr;
+    }
+}
+
+void blst_hip_server_destroy(blst_hip_server* s) { delete s; }
+
+int blst_hip_server_add_bases(blst_hip_server* s, const blst_p1_affine* points, size_t npoints) {
+    if (!s || !points || npoints == 0) { bls::host::g_err = "invalid arguments"; return -1; }
+    try {
+        return s->impl->add_bases(points, npoints);
+    } catch (const bls::host::HipFail& f) {
+        bls::host::g_err = f.msg;
+        return -1;
+    }
+}
+
+int64_t blst_hip_server_submit(blst_hip_server* s, int base_id, const blst_p1_affine* points,
+                               const blst_byte* scalars, size_t npoints, size_t nbits) {
+    if (!s || !scalars || npoints == 0 || npoints >= 0x7fffffffULL || nbits == 0 || nbits > 256 ||
+        (base_id < 0 && (!points || ((uintptr_t)points % 8) != 0))) {
+        bls::host::g_err = "invalid arguments";
+        return -1;
+    }
+    const int64_t t = s->impl->submit(base_id, points, scalars, npoints, nbits);
+    if (t < 0) bls::host::g_err = "request rejected (unknown bases or larger than the HBM budget)";
+    return t;
+}
+
+int blst_hip_server_wait(blst_hip_server* s, int64_t ticket, blst_p1* ret, double* latency_ms) {
+    if (!s || !ret) { bls::host::g_err = "invalid arguments"; return -1; }
+    const int rc = s->impl->wait(ticket, ret, latency_ms);
+    if (rc) bls::host::g_err = s->impl->last_error_.empty() ? "unknown ticket" : s->impl->last_error_;
+    return rc;
+}
+
+void blst_hip_server_stats(blst_hip_server* s, uint64_t out[6]) {
+    if (s) s->impl->stats(out);
+}
+
+}  // extern "C"

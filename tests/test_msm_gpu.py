@@ -292,3 +292,39 @@ def test_graph_plan_replays(hip, orc, base_points):
         srs.close()
         for b in scal:
             b.close()
+
+
+def test_serving_loop_results_match_alone(hip, orc, base_points):
+    """Requests of mixed sizes over resident and fresh bases, submitted while
+    earlier ones run; every result equals the same MSM computed alone and
+    the oracle.  A small HBM budget forces admission to wait for memory."""
+    srv = hip.Server(hbm_budget=96 << 20, max_batch_points=1 << 16)
+    try:
+        sid = srv.add_bases(base_points)
+        rnd = random.Random(5)
+        reqs = []
+        for k in range(40):
+            kind = rnd.random()
+            if kind < 0.6:
+                n = rnd.choice([1, 100, 4096])
+                sc = orc.gen_scalars(n, 3000 + k)
+                reqs.append((n, sc, None, srv.submit(sc, 255, base_id=sid, n=n)))
+            else:
+                n = rnd.choice([2, 777, 20000])
+                pts = orc.gen_points(n, 4000 + k, threads=8)
+                sc = orc.gen_scalars(n, 5000 + k)
+                reqs.append((n, sc, pts, srv.submit(sc, 255, points=pts)))
+        lat = []
+        for n, sc, pts, t in reqs:
+            got, ms = srv.wait(t)
+            lat.append(ms)
+            P = np.ascontiguousarray(base_points[:n]) if pts is None else pts
+            alone = hip.p1s_mult_pippenger(P, sc, 255)
+            want = orc.compress(orc.msm_pippenger(P, sc, n, 255))
+            assert hip.compress(got) == hip.compress(alone) == want, (n, t)
+        st = srv.stats()
+        assert st["completed"] == 40 and st["submitted"] == 40
+        assert st["peak_bytes"] <= st["budget"] + (4096 * 96)
+        assert st["batches"] < 40                      # requests were batched together
+    finally:
+        srv.close()
