@@ -148,6 +148,19 @@ void blst_hip_server_stats(blst_hip_server *server, uint64_t out[6]);
 void *blst_hip_host_alloc(size_t bytes);
 void blst_hip_host_free(void *host_ptr);
 
+/* Explicit window and tile shape (chunk = sorted bucket entries per
+ * accumulation lane, segment = buckets per reduction lane; 0 = tuned), the
+ * hook the autotuner measures with. */
+int blst_hip_msm_device_tuned(blst_p1 *ret, const void *device_points, const void *device_scalars,
+                              size_t npoints, size_t nbits, size_t window, size_t chunk, size_t segment);
+
+/* Tuning table: (window, chunk, segment) for MSMs of 2^log2n..2^(log2n+1)-1
+ * points, for a lone MSM (batch = 0) or a job inside a batch (batch = 1).
+ * The library loads the autotuner's table from $BLST_HIP_TUNING or from
+ * tuning.txt next to the library; these calls read and override it. */
+int blst_hip_tuning_set(int batch, size_t log2n, size_t window, size_t chunk, size_t segment);
+void blst_hip_tuning_get(int batch, size_t npoints, size_t nbits, size_t out[3]);
+
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
  * written to out[npoints] (host).  Returns 0 on success. */

@@ -46,6 +46,9 @@ SIGNATURES = {
     "blst_hip_server_stats": ([_vp, _vp], None),
     "blst_hip_host_alloc": ([_sz], _vp),
     "blst_hip_host_free": ([_vp], None),
+    "blst_hip_msm_device_tuned": ([_vp, _vp, _vp, _sz, _sz, _sz, _sz, _sz], ctypes.c_int),
+    "blst_hip_tuning_set": ([ctypes.c_int, _sz, _sz, _sz, _sz], ctypes.c_int),
+    "blst_hip_tuning_get": ([ctypes.c_int, _sz, _sz, _vp], None),
 }
 
 _lib = None
@@ -157,12 +160,12 @@ def booth_digits(scalars, nbits, bit0, window):
     return out
 
 
-def msm_device(d_points_ptr, d_scalars_ptr, n, nbits, window=0):
+def msm_device(d_points_ptr, d_scalars_ptr, n, nbits, window=0, chunk=0, segment=0):
     """MSM over device-resident inputs given as raw device addresses."""
     L = load()
     ret = np.zeros(18, np.uint64)
-    rc = L.blst_hip_msm_device(_ptr(ret), ctypes.c_void_p(d_points_ptr), ctypes.c_void_p(d_scalars_ptr),
-                               n, nbits, window)
+    rc = L.blst_hip_msm_device_tuned(_ptr(ret), ctypes.c_void_p(d_points_ptr), ctypes.c_void_p(d_scalars_ptr),
+                                     n, nbits, window, chunk, segment)
     if rc:
         raise RuntimeError("blst_hip_msm_device: " + last_error())
     return ret
@@ -345,6 +348,17 @@ class DeviceBuffer:
         if self.ptr:
             load().blst_hip_device_free(ctypes.c_void_p(self.ptr))
             self.ptr = None
+
+
+def tuning_set(batch, log2n, window, chunk, segment):
+    if load().blst_hip_tuning_set(int(batch), log2n, window, chunk, segment):
+        raise RuntimeError("blst_hip_tuning_set: " + last_error())
+
+
+def tuning_get(batch, n, nbits=255):
+    out = np.zeros(3, np.uint64)
+    load().blst_hip_tuning_get(int(batch), n, nbits, _ptr(out))
+    return tuple(int(x) for x in out)
 
 
 def last_phase_ms():

@@ -9,7 +9,11 @@
 #include "pipeline.h"
 #include "../../include/blst_hip.h"
 
+#include <dlfcn.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <chrono>
 #include <cstring>
 #include <map>
@@ -84,9 +88,95 @@ uint32_t device_window(size_t n, size_t nbits) {
     return bc;
 }
 
+// ------------------------------------------------------------------ tuning
+//
+// Two tables (a lone MSM, and a job inside a batch) indexed by floor(log2 n),
+// filled from the autotuner's file when present: BLST_HIP_TUNING, else
+// tuning.txt next to this library.  Lines: "single|batch log2n c T L".
+// Entries the file does not set fall back to the cost model.
+
+struct TuneTable {
+    Tune t[32];
+    bool set[32];
+};
+
+std::mutex g_tune_mu;
+TuneTable g_tune[2];
+bool g_tune_loaded = false;
+
+void load_tuning_locked() {
+    if (g_tune_loaded) return;
+    g_tune_loaded = true;
+    for (auto& tb : g_tune)
+        for (bool& b : tb.set) b = false;
+    std::string path;
+    if (const char* env = std::getenv("BLST_HIP_TUNING")) {
+        path = env;
+    } else {
+        Dl_info info;
+        if (dladdr((void*)&load_tuning_locked, &info) && info.dli_fname) {
+            path = info.dli_fname;
+            const size_t slash = path.rfind('/');
+            path = (slash == std::string::npos ? std::string(".") : path.substr(0, slash)) + "/tuning.txt";
+        }
+    }
+    FILE* f = path.empty() ? nullptr : std::fopen(path.c_str(), "r");
+    if (!f) return;
+    char mode[16];
+    unsigned lg, c, T, L;
+    char line[256];
+    while (std::fgets(line, sizeof line, f)) {
+        if (line[0] == '#') continue;
+        if (std::sscanf(line, "%15s %u %u %u %u", mode, &lg, &c, &T, &L) != 5) continue;
+        const int m = std::strcmp(mode, "batch") == 0 ? 1 : (std::strcmp(mode, "single") == 0 ? 0 : -1);
+        if (m < 0 || lg > 31 || c < 1 || c > 24 || T < 1 || T > 4096 || L < 1 || L > 4096) continue;
+        g_tune[m].t[lg] = Tune{c, T, L};
+        g_tune[m].set[lg] = true;
+    }
+    std::fclose(f);
+}
+
+int floor_log2(size_t n) {
+    int l = 0;
+    while (n >>= 1) l++;
+    return l;
+}
+
+Tune tuned(size_t n, size_t nbits, bool batch) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_tuning_locked();
+    const int lg = std::min(floor_log2(std::max<size_t>(n, 1)), 31);
+    // the tables are measured for full-width scalars
+    if (nbits >= 200 && g_tune[batch ? 1 : 0].set[lg]) return g_tune[batch ? 1 : 0].t[lg];
+    return Tune{device_window(n, nbits), 32, 8};
+}
+
+void set_tuning(bool batch, int lg, Tune t) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_tuning_locked();
+    g_tune[batch ? 1 : 0].t[lg] = t;
+    g_tune[batch ? 1 : 0].set[lg] = true;
+}
+
+// job description with the tuned (or explicit) window and tile shape
+JobSpec make_spec(const void* points, const void* scalars, size_t n, size_t nbits, size_t window, bool batch) {
+    const Tune t = tuned(n, nbits, batch);
+    JobSpec js;
+    js.points = points;
+    js.scalars = scalars;
+    js.n = (uint32_t)n;
+    js.nbits = (uint32_t)nbits;
+    js.c = window ? (uint32_t)window : t.c;
+    js.T = t.T;
+    js.L = t.L;
+    js.bit_base = 0;
+    js.tile = false;
+    return js;
+}
+
 Layout make_layout(const std::vector<JobSpec>& specs) {
     Layout L;
-    uint64_t pts = 0, ent = 0, keys = 0, segs = 0;
+    uint64_t pts = 0, ent = 0, keys = 0, segs = 0, biggest = 0;
     for (const JobSpec& s : specs) {
         job_t J;
         J.scalars = static_cast<const uint8_t*>(s.scalars);
@@ -107,12 +197,16 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
             group_t G = {};
             G.kb = J.kbase + w * J.B;
             G.B = J.B;
-            G.L = std::min<uint32_t>(kSegment, J.B);
+            G.L = std::min<uint32_t>(std::max<uint32_t>(s.L, 1), J.B);
             G.P = J.B / G.L;
             G.sb = (uint32_t)segs;
             segs += G.P;
             L.groups.push_back(G);
             L.sbase.push_back(G.sb);
+        }
+        if ((uint64_t)J.nwin * J.n > biggest) {
+            biggest = (uint64_t)J.nwin * J.n;
+            L.T = std::max<uint32_t>(s.T, 1);
         }
         pts += J.n;
         ent += (uint64_t)J.nwin * J.n;
@@ -194,8 +288,8 @@ void fill_words(Workspace& cx, void* dst, size_t words, uint32_t value) {
 // Partial-tree rounds that cover any bucket of jobs of at most max_n points:
 // a bucket holds at most n entries of one window, so it spans at most
 // ceil(n / T) + 1 chunks.
-uint32_t bound_rounds(uint32_t max_n) {
-    const uint32_t span = div_up(max_n, kChunk) + 1;
+uint32_t bound_rounds(uint32_t max_n, uint32_t T) {
+    const uint32_t span = div_up(max_n, T) + 1;
     uint32_t r = 0;
     while ((1u << r) < span) r++;
     return r;
@@ -203,11 +297,10 @@ uint32_t bound_rounds(uint32_t max_n) {
 
 void ensure_workspace(Workspace& cx, const Layout& L) {
     const uint32_t nkeys = L.nkeys, nent = L.total_entries;
-    const uint32_t nslots = 2 * std::max<uint32_t>(div_up(nent, kChunk), 1);
+    const uint32_t nslots = 2 * std::max<uint32_t>(div_up(nent, L.T), 1);
     cx.ent.ensure((size_t)nent * 4 + 16);
     cx.rank.ensure((size_t)nent * 4 + 16);
-    cx.skey.ensure((size_t)nent * 4 + 16);
-    cx.sval.ensure((size_t)nent * 4 + 16);
+    cx.sorted.ensure((size_t)nent * 8 + 16);
     cx.hist.ensure(((size_t)nkeys + 1) * 4);
     cx.off.ensure(((size_t)nkeys + 1) * 4);
     cx.scan_tmp.ensure(scan_tmp_words(nkeys + 1) * 4);
@@ -221,7 +314,7 @@ void ensure_workspace(Workspace& cx, const Layout& L) {
 
 // Device bytes a batch of this layout needs beyond its inputs.
 size_t workspace_bytes(const Layout& L) {
-    const size_t nslots = 2 * (size_t)std::max<uint32_t>(div_up(L.total_entries, kChunk), 1);
+    const size_t nslots = 2 * (size_t)std::max<uint32_t>(div_up(L.total_entries, L.T), 1);
     return (size_t)L.total_entries * 16 + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * sizeof(xyzz) +
            nslots * (4 + sizeof(xyzz)) + (size_t)L.total_segments * sizeof(xyzz) +
            (L.groups.size() + L.jobs.size()) * sizeof(xyzz) + (1 << 20);
@@ -237,7 +330,8 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int round
                    bool timing) {
     const uint32_t nkeys = L.nkeys;
     const uint32_t nent = L.total_entries;
-    const uint32_t nchunks = std::max<uint32_t>(div_up(nent, kChunk), 1);
+    const uint32_t T = L.T;
+    const uint32_t nchunks = std::max<uint32_t>(div_up(nent, T), 1);
     const uint32_t nslots = 2 * nchunks;
     const uint32_t ngroups = (uint32_t)L.groups.size();
     const uint32_t njobs = (uint32_t)L.jobs.size();
@@ -254,14 +348,13 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int round
 
     scan_exclusive(cx, hist, off, nkeys + 1, cx.scan_tmp.as<uint32_t>());
     launch_k_scatter(dim3(div_up(nent, 256)), dim3(256), st, D.jobs, D.ebase, njobs, nent,
-                     cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off, cx.skey.as<uint32_t>(),
-                     cx.sval.as<uint32_t>());
+                     cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off, cx.sorted.as<uint2>());
     HIPCK(hipGetLastError());
     fill_words(cx, cx.buckets.p, (size_t)nkeys * (sizeof(xyzz) / 4), 0);
     if (timing) HIPCK(hipEventRecord(cx.ev[2], st));
-    launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.skey.as<const uint32_t>(),
-                        cx.sval.as<const uint32_t>(), off + nkeys, D.jobs, D.kbase, njobs, cx.buckets.as<xyzz>(),
-                        cx.slot_key.as<uint32_t>(), cx.slots.as<xyzz>(), kChunk, nchunks);
+    launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.sorted.as<const uint2>(), off + nkeys, D.jobs,
+                        D.kbase, njobs, cx.buckets.as<xyzz>(), cx.slot_key.as<uint32_t>(), cx.slots.as<xyzz>(), T,
+                        nchunks);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
 
@@ -272,7 +365,7 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int round
                            cx.maxcnt.as<uint32_t>());
         HIPCK(hipMemcpyAsync(cx.host_word, cx.maxcnt.p, 4, hipMemcpyDeviceToHost, st));
         HIPCK(hipStreamSynchronize(st));
-        const uint32_t maxspan = div_up(*cx.host_word, kChunk) + 1;
+        const uint32_t maxspan = div_up(*cx.host_word, T) + 1;
         nrounds = 0;
         while ((1u << nrounds) < maxspan) nrounds++;
     } else {
@@ -280,7 +373,7 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int round
     }
     for (uint32_t r = 0; r < nrounds; r++)
         launch_k_partial_round(dim3(div_up(nslots, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
-                               cx.slots.as<xyzz>(), off, hist, kChunk, nslots, 1u << r);
+                               cx.slots.as<xyzz>(), off, hist, T, nslots, 1u << r);
     launch_k_partial_final(dim3(div_up(nslots, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
                            cx.slots.as<const xyzz>(), cx.buckets.as<xyzz>(), nslots);
     HIPCK(hipGetLastError());
@@ -392,12 +485,7 @@ int host_msm(blst_p1* ret, const blst_p1_affine* const points[], size_t npoints,
         cx.scl.ensure(npoints * nbytes + 16);
         HIPCK(hipMemcpyAsync(cx.pts.p, hp, npoints * sizeof(blst_p1_affine), hipMemcpyHostToDevice, cx.stream));
         HIPCK(hipMemcpyAsync(cx.scl.p, hs, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
-        JobSpec js;
-        js.points = cx.pts.p;
-        js.scalars = cx.scl.p;
-        js.n = (uint32_t)npoints;
-        js.nbits = (uint32_t)nbits;
-        js.c = (uint32_t)(window ? window : device_window(npoints, nbits));
+        JobSpec js = make_spec(cx.pts.p, cx.scl.p, npoints, nbits, window, false);
         js.bit_base = (uint32_t)bit0;
         js.tile = !fold;
         device_msm(cx, js, ret);
@@ -478,14 +566,7 @@ bool specs_from_jobs(const blst_hip_msm_job* jobs, size_t njobs, std::vector<Job
             g_err = "invalid job " + std::to_string(k);
             return false;
         }
-        JobSpec js;
-        js.points = j.points;
-        js.scalars = j.scalars;
-        js.n = (uint32_t)j.npoints;
-        js.nbits = (uint32_t)j.nbits;
-        js.c = (uint32_t)(j.window ? j.window : device_window(j.npoints, j.nbits));
-        js.bit_base = 0;
-        js.tile = false;
+        JobSpec js = make_spec(j.points, j.scalars, j.npoints, j.nbits, j.window, true);
         specs.push_back(js);
         idx.push_back(k);
     }
@@ -573,10 +654,15 @@ int blst_hip_available(void) {
 
 int blst_hip_msm_device(blst_p1* ret, const void* device_points, const void* device_scalars,
                         size_t npoints, size_t nbits, size_t window) {
+    return blst_hip_msm_device_tuned(ret, device_points, device_scalars, npoints, nbits, window, 0, 0);
+}
+
+int blst_hip_msm_device_tuned(blst_p1* ret, const void* device_points, const void* device_scalars,
+                              size_t npoints, size_t nbits, size_t window, size_t chunk, size_t segment) {
     auto t0 = std::chrono::steady_clock::now();
     std::memset(ret, 0, sizeof(*ret));
     if (npoints == 0 || nbits == 0) { g_err.clear(); return 0; }
-    if (nbits > 256 || window > 24 || !device_points || !device_scalars ||
+    if (nbits > 256 || window > 24 || chunk > 4096 || segment > 4096 || !device_points || !device_scalars ||
         ((uintptr_t)device_points % 16) != 0 || npoints >= 0x7fffffffULL) {
         g_err = "invalid arguments";
         return -1;
@@ -584,14 +670,9 @@ int blst_hip_msm_device(blst_p1* ret, const void* device_points, const void* dev
     try {
         Ctx& cx = context();
         std::lock_guard<std::mutex> lk(cx.mu);
-        JobSpec js;
-        js.points = device_points;
-        js.scalars = device_scalars;
-        js.n = (uint32_t)npoints;
-        js.nbits = (uint32_t)nbits;
-        js.c = (uint32_t)(window ? window : device_window(npoints, nbits));
-        js.bit_base = 0;
-        js.tile = false;
+        JobSpec js = make_spec(device_points, device_scalars, npoints, nbits, window, false);
+        if (chunk) js.T = (uint32_t)chunk;
+        if (segment) js.L = (uint32_t)segment;
         device_msm(cx, js, ret);
     } catch (const HipFail& f) {
         g_err = f.msg;
@@ -652,7 +733,7 @@ blst_hip_batch_plan* blst_hip_batch_plan_create(const blst_hip_msm_job* jobs, si
             HIPCK(hipHostMalloc((void**)&P.host_res, specs.size() * sizeof(hostec::xyzz6), hipHostMallocDefault));
             HIPCK(hipStreamSynchronize(P.ws.stream));
             HIPCK(hipStreamBeginCapture(P.ws.stream, hipStreamCaptureModeThreadLocal));
-            enqueue_batch(P.ws, P.L, P.D, (int)bound_rounds(P.L.max_n), false, P.host_res, false);
+            enqueue_batch(P.ws, P.L, P.D, (int)bound_rounds(P.L.max_n, P.L.T), false, P.host_res, false);
             HIPCK(hipStreamEndCapture(P.ws.stream, &P.graph));
             HIPCK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
         }
@@ -795,6 +876,23 @@ void blst_hip_p1_compress(blst_byte out[48], const blst_p1* in) {
     hostec::jac6 j;
     std::memcpy(&j, in, sizeof(j));
     hostec::compress(out, j);
+}
+
+int blst_hip_tuning_set(int batch, size_t log2n, size_t window, size_t chunk, size_t segment) {
+    if (log2n > 31 || window < 1 || window > 24 || chunk < 1 || chunk > 4096 || segment < 1 || segment > 4096) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    set_tuning(batch != 0, (int)log2n, Tune{(uint32_t)window, (uint32_t)chunk, (uint32_t)segment});
+    g_err.clear();
+    return 0;
+}
+
+void blst_hip_tuning_get(int batch, size_t npoints, size_t nbits, size_t out[3]) {
+    const Tune t = tuned(npoints, nbits, batch != 0);
+    out[0] = t.c;
+    out[1] = t.T;
+    out[2] = t.L;
 }
 
 void blst_hip_last_phase_ms(double out[7]) {

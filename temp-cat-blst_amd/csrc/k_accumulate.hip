@@ -10,7 +10,7 @@ namespace bls {
 // k0 the chunk holding the bucket's first entry: the segment in chunk k0 goes
 // to slot 2*k0+1, the segment in any later chunk k to slot 2*k.
 __global__ void __launch_bounds__(256)
-k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval,
+k_accumulate(const uint2* __restrict__ sorted,
              const uint32_t* __restrict__ total, const job_t* __restrict__ jobs,
              const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets,
              uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks) {
@@ -22,17 +22,18 @@ k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sva
     const uint32_t start = chunk * T;
     if (start >= M) return;
     const uint32_t end = min(start + T, M);
-    uint32_t cur = skey[start];
+    uint32_t cur = sorted[start].x;
     uint32_t j = find_owner(kbase, njobs, cur);
     uint32_t next_kb = (j + 1 < njobs) ? kbase[j + 1] : 0xffffffffu;
     const uint4* pts = jobs[j].points;
-    const bool before = start > 0 && skey[start - 1] == cur;
+    const bool before = start > 0 && sorted[start - 1].x == cur;
     bool first = true;
     xyzz acc;
     xyzz_set_inf(acc);
     #pragma unroll 1
     for (uint32_t e = start; e < end; e++) {
-        const uint32_t key = skey[e];
+        const uint2 kv = sorted[e];
+        const uint32_t key = kv.x;
         if (key != cur) {
             if (first && before) {
                 slot_key[2 * chunk] = cur;
@@ -49,13 +50,13 @@ k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sva
                 pts = jobs[j].points;
             }
         }
-        const uint32_t v = sval[e];
+        const uint32_t v = kv.y;
         aff a;
         load_aff(a, pts, v & 0x7fffffffu);
         if (v >> 31) fp_neg(a.y, a.y);
         xyzz_madd(acc, a);
     }
-    const bool after = end < M && skey[end] == cur;
+    const bool after = end < M && sorted[end].x == cur;
     if (first && before) {
         slot_key[2 * chunk] = cur;
         store_xyzz(&slots[2 * chunk], acc);
@@ -67,8 +68,8 @@ k_accumulate(const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sva
     }
 }
 
-void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ skey, const uint32_t* __restrict__ sval, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks) {
-    hipLaunchKernelGGL(k_accumulate, grid, block, 0, stream, skey, sval, total, jobs, kbase, njobs, buckets, slot_key, slots, T, nchunks);
+void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks) {
+    hipLaunchKernelGGL(k_accumulate, grid, block, 0, stream, sorted, total, jobs, kbase, njobs, buckets, slot_key, slots, T, nchunks);
 }
 
 }  // namespace bls

@@ -107,7 +107,7 @@ __global__ void k_scan_add(uint32_t* __restrict__ out, const uint32_t* __restric
 __global__ void __launch_bounds__(256)
 k_scatter(const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, uint32_t njobs,
           uint32_t total_entries, const uint32_t* __restrict__ ent, const uint32_t* __restrict__ rank,
-          const uint32_t* __restrict__ off, uint32_t* __restrict__ skey, uint32_t* __restrict__ sval) {
+          const uint32_t* __restrict__ off, uint2* __restrict__ sorted) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total_entries) return;
     const uint32_t e = ent[t];
@@ -116,8 +116,7 @@ k_scatter(const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, ui
     const uint32_t i = (t - ebase[j]) % jobs[j].n;
     const uint32_t key = e & 0x7fffffffu;
     const uint32_t pos = off[key] + rank[t];
-    skey[pos] = key;
-    sval[pos] = i | (e & 0x80000000u);
+    sorted[pos] = make_uint2(key, i | (e & 0x80000000u));
 }
 
 void launch_k_digits(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ pbase, uint32_t njobs, uint32_t total_points, uint32_t* __restrict__ ent, uint32_t* __restrict__ rank, uint32_t* __restrict__ hist) {
@@ -144,8 +143,8 @@ void launch_k_scan_add(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __re
     hipLaunchKernelGGL(k_scan_add, grid, block, 0, stream, out, tile_offs, N);
 }
 
-void launch_k_scatter(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, uint32_t njobs, uint32_t total_entries, const uint32_t* __restrict__ ent, const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off, uint32_t* __restrict__ skey, uint32_t* __restrict__ sval) {
-    hipLaunchKernelGGL(k_scatter, grid, block, 0, stream, jobs, ebase, njobs, total_entries, ent, rank, off, skey, sval);
+void launch_k_scatter(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, uint32_t njobs, uint32_t total_entries, const uint32_t* __restrict__ ent, const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off, uint2* __restrict__ sorted) {
+    hipLaunchKernelGGL(k_scatter, grid, block, 0, stream, jobs, ebase, njobs, total_entries, ent, rank, off, sorted);
 }
 
 }  // namespace bls

@@ -45,7 +45,7 @@ struct DevBuf {
 // each captured batch plan and each serving slot own one.
 struct Workspace {
     hipStream_t stream = nullptr;
-    DevBuf ent, rank, hist, off, skey, sval, buckets, slot_key, slots, H, scan_tmp, W, res, tables, maxcnt;
+    DevBuf ent, rank, hist, off, sorted, buckets, slot_key, slots, H, scan_tmp, W, res, tables, maxcnt;
     std::vector<uint8_t> tables_host;
     uint32_t* host_word = nullptr;       // pinned readback word
     hipEvent_t ev[8] = {};
@@ -67,12 +67,20 @@ struct JobSpec {
     const void* points;    // device, n affine points
     const void* scalars;   // device, n * ceil(nbits/8) bytes
     uint32_t n, nbits, c;
+    uint32_t T, L;         // tile shape (the batch uses the T of its largest job)
     uint32_t bit_base;     // tile mode: first bit of the single window
     bool tile;             // one window at bit_base, no window fold
 };
 
-constexpr uint32_t kChunk = 32;     // sorted entries per accumulate lane
-constexpr uint32_t kSegment = 8;    // buckets per reduction lane
+// Tile shape and window of one MSM size: c window bits, T sorted entries
+// per accumulate lane, L buckets per reduction lane.
+struct Tune {
+    uint32_t c, T, L;
+};
+
+// Tuned parameters for an MSM of n points (batch: as one job of a batch).
+// Entries come from the autotuner's table when loaded, else from a cost model.
+Tune tuned(size_t n, size_t nbits, bool batch);
 
 // Key, entry, point, group and segment ranges of every job of a batch.
 struct Layout {
@@ -81,6 +89,7 @@ struct Layout {
     std::vector<group_t> groups;
     std::vector<uint32_t> sbase;
     uint32_t total_points = 0, total_entries = 0, nkeys = 0, total_segments = 0, max_n = 0;
+    uint32_t T = 32;       // sorted entries per accumulate lane
 };
 
 // Device copies of the layout tables (one upload per batch).
@@ -91,11 +100,12 @@ struct DevLayout {
 };
 
 uint32_t device_window(size_t n, size_t nbits);
+JobSpec make_spec(const void* points, const void* scalars, size_t n, size_t nbits, size_t window, bool batch);
 Layout make_layout(const std::vector<JobSpec>& specs);
 DevLayout upload_layout(Workspace& cx, const Layout& L);
 void ensure_workspace(Workspace& cx, const Layout& L);
 size_t workspace_bytes(const Layout& L);
-uint32_t bound_rounds(uint32_t max_n);
+uint32_t bound_rounds(uint32_t max_n, uint32_t T);
 void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int rounds, bool host_fold, void* host_out,
                    bool timing = true);
 void record_phases(Workspace& cx);

@@ -200,15 +200,7 @@ private:
     // device bytes of a request: its inputs plus the share of a batch
     // workspace a job of its size adds
     size_t footprint(const Request& r) const {
-        JobSpec js;
-        js.points = nullptr;
-        js.scalars = nullptr;
-        js.n = (uint32_t)r.n;
-        js.nbits = (uint32_t)r.nbits;
-        js.c = device_window(r.n, r.nbits);
-        js.bit_base = 0;
-        js.tile = false;
-        Layout L = make_layout(std::vector<JobSpec>(1, js));
+        Layout L = make_layout(std::vector<JobSpec>(1, make_spec(nullptr, nullptr, r.n, r.nbits, 0, true)));
         size_t in = r.n * ((r.nbits + 7) / 8) + 64;
         if (r.base_id < 0) in += r.n * sizeof(blst_p1_affine);
         return in + workspace_bytes(L);
@@ -270,15 +262,8 @@ private:
             auto& r = *it;
             if (!r->copy_issued || hipEventQuery(r->copied) != hipSuccess) break;   // keep arrival order
             if (!batch.empty() && pts + r->n > max_batch_points_) break;
-            JobSpec js;
-            js.points = r->base_id >= 0 ? bases_[r->base_id].ptr : r->d_points;
-            js.scalars = r->d_scalars;
-            js.n = (uint32_t)r->n;
-            js.nbits = (uint32_t)r->nbits;
-            js.c = device_window(r->n, r->nbits);
-            js.bit_base = 0;
-            js.tile = false;
-            specs.push_back(js);
+            specs.push_back(make_spec(r->base_id >= 0 ? bases_[r->base_id].ptr : r->d_points, r->d_scalars, r->n,
+                                      r->nbits, 0, true));
             batch.push_back(r);
             pts += r->n;
             it = admitted_.erase(it);
@@ -292,7 +277,7 @@ private:
             s.host_res_cap = std::max<size_t>(batch.size(), 64);
             HIPCK(hipHostMalloc((void**)&s.host_res, s.host_res_cap * sizeof(hostec::xyzz6), hipHostMallocDefault));
         }
-        enqueue_batch(s.ws, s.L, D, (int)bound_rounds(s.L.max_n), false, s.host_res, false);
+        enqueue_batch(s.ws, s.L, D, (int)bound_rounds(s.L.max_n, s.L.T), false, s.host_res, false);
         HIPCK(hipEventRecord(s.done, s.ws.stream));
         s.reqs = std::move(batch);
         s.busy = true;

@@ -146,13 +146,13 @@ def _pct(x, q):
     return float(np.percentile(np.asarray(x), q)) if x else None
 
 
-def serving(rate=600.0, duration=2.0, check_every=97):
-    data = _ServingData()
-    trace = serving_trace(rate, duration)
-    pts_total = sum(1 << lg for _, lg, _, _ in trace)
+def _serve(data, trace, burst):
+    """Runs the trace through a server; burst=True submits everything at once
+    (capacity), else at the trace's arrival times.  Returns (seconds, results,
+    latencies, stats)."""
     srv = bh.Server()
-    res_served = [None] * len(trace)
-    lat_served = [None] * len(trace)
+    res = [None] * len(trace)
+    lat = [None] * len(trace)
     try:
         sid = srv.add_bases(data.srs)
 
@@ -167,66 +167,78 @@ def serving(rate=600.0, duration=2.0, check_every=97):
         for t in [submit(k) for k in range(min(8, len(trace)))]:
             srv.wait(t)
         tickets = [None] * len(trace)
-        done_evt = threading.Event()
 
         def waiter():
             for k in range(len(trace)):
                 while tickets[k] is None:
                     time.sleep(0.0001)
-                res_served[k], lat_served[k] = srv.wait(tickets[k])
-            done_evt.set()
+                res[k], lat[k] = srv.wait(tickets[k])
 
         th = threading.Thread(target=waiter)
         th.start()
         t0 = time.perf_counter()
         for k, (at, _, _, _) in enumerate(trace):
-            dt = t0 + at - time.perf_counter()
-            if dt > 0:
-                time.sleep(dt)
+            if not burst:
+                dt = t0 + at - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
             tickets[k] = submit(k)
-        done_evt.wait()
-        served_s = time.perf_counter() - t0
         th.join()
+        secs = time.perf_counter() - t0
         stats = srv.stats()
     finally:
         srv.close()
+    return secs, res, lat, stats
 
-    # plain: one call per request, in arrival order, on the same trace
-    d_srs = bh.DeviceBuffer(data.srs)
-    d_sc = bh.DeviceBuffer(np.zeros((4096, 32), np.uint8))
-    lat_plain, same = [], True
+
+def serving(rate=1500.0, duration=2.0, check_every=97):
+    data = _ServingData()
+    trace = serving_trace(rate, duration)
+    pts_total = sum(1 << lg for _, lg, _, _ in trace)
     try:
-        bh.msm_device(d_srs.ptr, d_sc.ptr, 4096, 255)
-        t0 = time.perf_counter()
-        for k, (at, lg, fi, si) in enumerate(trace):
-            now = time.perf_counter() - t0
-            if now < at:
-                time.sleep(at - now)
-            sc = data.scal[lg][si]
-            if lg == 12:
-                d_sc.write(sc)
-                r = bh.msm_device(d_srs.ptr, d_sc.ptr, 4096, 255)
-            else:
-                r = bh.p1s_mult_pippenger(data.fresh[lg][fi], sc, 255)
-            lat_plain.append((time.perf_counter() - t0 - at) * 1e3)
-            if k % check_every == 0:
-                same = same and bh.compress(r) == bh.compress(res_served[k])
-        plain_s = time.perf_counter() - t0
+        served_s, res_served, lat_served, stats = _serve(data, trace, burst=False)
+        burst_s, res_burst, _, _ = _serve(data, trace, burst=True)
+
+        # plain: one call per request, in arrival order, on the same trace
+        d_srs = bh.DeviceBuffer(data.srs)
+        d_sc = bh.DeviceBuffer(np.zeros((4096, 32), np.uint8))
+        lat_plain, same = [], True
+        try:
+            bh.msm_device(d_srs.ptr, d_sc.ptr, 4096, 255)
+            t0 = time.perf_counter()
+            for k, (at, lg, fi, si) in enumerate(trace):
+                now = time.perf_counter() - t0
+                if now < at:
+                    time.sleep(at - now)
+                sc = data.scal[lg][si]
+                if lg == 12:
+                    d_sc.write(sc)
+                    r = bh.msm_device(d_srs.ptr, d_sc.ptr, 4096, 255)
+                else:
+                    r = bh.p1s_mult_pippenger(data.fresh[lg][fi], sc, 255)
+                lat_plain.append((time.perf_counter() - t0 - at) * 1e3)
+                if k % check_every == 0:
+                    same = same and bh.compress(r) == bh.compress(res_served[k]) == bh.compress(res_burst[k])
+            plain_s = time.perf_counter() - t0
+        finally:
+            d_srs.close()
+            d_sc.close()
     finally:
-        d_srs.close()
-        d_sc.close()
         data.close()
     return {
         "workload": "Poisson arrivals at %.0f req/s for %.1f s (%d requests, %d points), mix 70/25/5 of "
                     "4096 (resident setup) / 2^16 / 2^20 (fresh bases)" % (rate, duration, len(trace), pts_total),
         "offered_rate": rate,
+        "offered_points_per_s": pts_total / duration,
         "requests": len(trace),
         "points": pts_total,
         "served_points_per_s": pts_total / served_s,
         "served_latency_ms": {"p50": _pct(lat_served, 50), "p99": _pct(lat_served, 99), "max": max(lat_served)},
+        "served_capacity_points_per_s": pts_total / burst_s,
         "plain_points_per_s": pts_total / plain_s,
         "plain_latency_ms": {"p50": _pct(lat_plain, 50), "p99": _pct(lat_plain, 99), "max": max(lat_plain)},
         "speedup": plain_s / served_s,
+        "capacity_speedup": plain_s / burst_s,
         "server": stats,
         "served_equals_plain_sampled": same,
     }
