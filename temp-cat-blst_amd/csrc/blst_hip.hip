@@ -176,7 +176,7 @@ JobSpec make_spec(const void* points, const void* scalars, size_t n, size_t nbit
 
 Layout make_layout(const std::vector<JobSpec>& specs) {
     Layout L;
-    uint64_t pts = 0, ent = 0, keys = 0, segs = 0, biggest = 0;
+    uint64_t pts = 0, ent = 0, keys = 0, segs = 0, biggest = 0, tiles = 0;
     for (const JobSpec& s : specs) {
         job_t J;
         J.scalars = static_cast<const uint8_t*>(s.scalars);
@@ -203,7 +203,20 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
             segs += G.P;
             L.groups.push_back(G);
             L.sbase.push_back(G.sb);
+            // sort tiles: chunks of max(2B, 4096) points per window
+            sgroup_t S = {};
+            S.job = (uint32_t)L.jobs.size();
+            S.w = w;
+            S.ch = std::max<uint32_t>(2 * J.B, 4096);
+            S.nch = std::max<uint32_t>(div_up(J.n, S.ch), 1);
+            S.tb = (uint32_t)tiles;
+            S.cb = 0;
+            tiles += S.nch;
+            L.sgroups.push_back(S);
+            L.tbase.push_back(S.tb);
+            L.gkb.push_back(G.kb);
         }
+        L.max_B = std::max(L.max_B, J.B);
         if ((uint64_t)J.nwin * J.n > biggest) {
             biggest = (uint64_t)J.nwin * J.n;
             L.T = std::max<uint32_t>(s.T, 1);
@@ -219,6 +232,15 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
     }
     if (ent >= 0x7fffffffULL || keys >= 0x7fffffffULL || pts >= 0x7fffffffULL || segs >= 0x7fffffffULL)
         throw HipFail{"batch too large for one launch"};
+    if (tiles >= 0x7fffffffULL) throw HipFail{"batch too large for one launch"};
+    size_t cw = 0;
+    for (sgroup_t& S : L.sgroups) {
+        S.cb = (uint32_t)cw;
+        cw += (size_t)S.nch * L.jobs[S.job].B;
+    }
+    if (cw >= 0xffffffffULL) throw HipFail{"batch too large for one launch"};
+    L.count_words = cw;
+    L.ntiles = (uint32_t)tiles;
     L.total_points = (uint32_t)pts;
     L.total_entries = (uint32_t)ent;
     L.nkeys = (uint32_t)keys;
@@ -231,7 +253,8 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_jobs = 0, o_p = al(o_jobs + nj * sizeof(job_t)), o_e = al(o_p + nj * 4),
                  o_k = al(o_e + nj * 4), o_g = al(o_k + nj * 4), o_s = al(o_g + ng * sizeof(group_t)),
-                 total = al(o_s + ng * 4);
+                 o_sg = al(o_s + ng * 4), o_tb = al(o_sg + ng * sizeof(sgroup_t)), o_gk = al(o_tb + ng * 4),
+                 total = al(o_gk + ng * 4);
     cx.tables.ensure(total);
     cx.tables_host.resize(total);
     uint8_t* h = cx.tables_host.data();
@@ -241,6 +264,9 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     std::memcpy(h + o_k, L.kbase.data(), nj * 4);
     std::memcpy(h + o_g, L.groups.data(), ng * sizeof(group_t));
     std::memcpy(h + o_s, L.sbase.data(), ng * 4);
+    std::memcpy(h + o_sg, L.sgroups.data(), ng * sizeof(sgroup_t));
+    std::memcpy(h + o_tb, L.tbase.data(), ng * 4);
+    std::memcpy(h + o_gk, L.gkb.data(), ng * 4);
     HIPCK(hipMemcpyAsync(cx.tables.p, h, total, hipMemcpyHostToDevice, cx.stream));
     uint8_t* d = cx.tables.as<uint8_t>();
     DevLayout D;
@@ -250,6 +276,9 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     D.kbase = reinterpret_cast<const uint32_t*>(d + o_k);
     D.groups = reinterpret_cast<const group_t*>(d + o_g);
     D.sbase = reinterpret_cast<const uint32_t*>(d + o_s);
+    D.sgroups = reinterpret_cast<const sgroup_t*>(d + o_sg);
+    D.tbase = reinterpret_cast<const uint32_t*>(d + o_tb);
+    D.gkb = reinterpret_cast<const uint32_t*>(d + o_gk);
     return D;
 }
 
@@ -288,6 +317,13 @@ void fill_words(Workspace& cx, void* dst, size_t words, uint32_t value) {
 // Partial-tree rounds that cover any bucket of jobs of at most max_n points:
 // a bucket holds at most n entries of one window, so it spans at most
 // ceil(n / T) + 1 chunks.
+// The tiled sort keeps a window's bucket counters in LDS; wider windows take
+// the global-histogram path.
+bool tiled_sort(const Layout& L) {
+    static const size_t limit = sort_lds_limit();
+    return (size_t)L.max_B * 4 <= limit;
+}
+
 uint32_t bound_rounds(uint32_t max_n, uint32_t T) {
     const uint32_t span = div_up(max_n, T) + 1;
     uint32_t r = 0;
@@ -298,8 +334,12 @@ uint32_t bound_rounds(uint32_t max_n, uint32_t T) {
 void ensure_workspace(Workspace& cx, const Layout& L) {
     const uint32_t nkeys = L.nkeys, nent = L.total_entries;
     const uint32_t nslots = 2 * std::max<uint32_t>(div_up(nent, L.T), 1);
-    cx.ent.ensure((size_t)nent * 4 + 16);
-    cx.rank.ensure((size_t)nent * 4 + 16);
+    if (tiled_sort(L)) {
+        cx.counts.ensure(L.count_words * 4 + 16);
+    } else {
+        cx.ent.ensure((size_t)nent * 4 + 16);
+        cx.rank.ensure((size_t)nent * 4 + 16);
+    }
     cx.sorted.ensure((size_t)nent * 8 + 16);
     cx.hist.ensure(((size_t)nkeys + 1) * 4);
     cx.off.ensure(((size_t)nkeys + 1) * 4);
@@ -315,7 +355,8 @@ void ensure_workspace(Workspace& cx, const Layout& L) {
 // Device bytes a batch of this layout needs beyond its inputs.
 size_t workspace_bytes(const Layout& L) {
     const size_t nslots = 2 * (size_t)std::max<uint32_t>(div_up(L.total_entries, L.T), 1);
-    return (size_t)L.total_entries * 16 + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * sizeof(xyzz) +
+    const size_t sort_bytes = tiled_sort(L) ? L.count_words * 4 : (size_t)L.total_entries * 8;
+    return (size_t)L.total_entries * 8 + sort_bytes + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * sizeof(xyzz) +
            nslots * (4 + sizeof(xyzz)) + (size_t)L.total_segments * sizeof(xyzz) +
            (L.groups.size() + L.jobs.size()) * sizeof(xyzz) + (1 << 20);
 }
@@ -340,15 +381,27 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int round
     uint32_t* off = cx.off.as<uint32_t>();
 
     if (timing) HIPCK(hipEventRecord(cx.ev[0], st));
-    fill_words(cx, hist, (size_t)nkeys + 1, 0);
-    launch_k_digits(dim3(div_up(L.total_points, 256)), dim3(256), st, D.jobs, D.pbase, njobs, L.total_points,
-                    cx.ent.as<uint32_t>(), cx.rank.as<uint32_t>(), hist);
-    HIPCK(hipGetLastError());
-    if (timing) HIPCK(hipEventRecord(cx.ev[1], st));
-
-    scan_exclusive(cx, hist, off, nkeys + 1, cx.scan_tmp.as<uint32_t>());
-    launch_k_scatter(dim3(div_up(nent, 256)), dim3(256), st, D.jobs, D.ebase, njobs, nent,
-                     cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off, cx.sorted.as<uint2>());
+    if (tiled_sort(L)) {
+        const size_t lds = (size_t)L.max_B * 4;
+        uint32_t* counts = cx.counts.as<uint32_t>();
+        launch_k_count(dim3(L.ntiles), dim3(kSortThreads), lds, st, D.jobs, D.sgroups, D.tbase, ngroups, counts);
+        launch_k_prefix(dim3(div_up(nkeys + 1, 256)), dim3(256), st, D.groups, D.sgroups, D.gkb, ngroups, nkeys,
+                        counts, hist);
+        HIPCK(hipGetLastError());
+        if (timing) HIPCK(hipEventRecord(cx.ev[1], st));
+        scan_exclusive(cx, hist, off, nkeys + 1, cx.scan_tmp.as<uint32_t>());
+        launch_k_place(dim3(L.ntiles), dim3(kSortThreads), lds, st, D.jobs, D.sgroups, D.tbase, ngroups, counts,
+                       off, cx.sorted.as<uint2>());
+    } else {
+        fill_words(cx, hist, (size_t)nkeys + 1, 0);
+        launch_k_digits(dim3(div_up(L.total_points, 256)), dim3(256), st, D.jobs, D.pbase, njobs, L.total_points,
+                        cx.ent.as<uint32_t>(), cx.rank.as<uint32_t>(), hist);
+        HIPCK(hipGetLastError());
+        if (timing) HIPCK(hipEventRecord(cx.ev[1], st));
+        scan_exclusive(cx, hist, off, nkeys + 1, cx.scan_tmp.as<uint32_t>());
+        launch_k_scatter(dim3(div_up(nent, 256)), dim3(256), st, D.jobs, D.ebase, njobs, nent,
+                         cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off, cx.sorted.as<uint2>());
+    }
     HIPCK(hipGetLastError());
     fill_words(cx, cx.buckets.p, (size_t)nkeys * (sizeof(xyzz) / 4), 0);
     if (timing) HIPCK(hipEventRecord(cx.ev[2], st));

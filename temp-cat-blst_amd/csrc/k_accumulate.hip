@@ -53,8 +53,12 @@ k_accumulate(const uint2* __restrict__ sorted,
         const uint32_t v = kv.y;
         aff a;
         load_aff(a, pts, v & 0x7fffffffu);
-        if (v >> 31) fp_neg(a.y, a.y);
-        xyzz_madd(acc, a);
+        // an all-zero affine point is infinity and contributes nothing
+        // (Pointxyzz::dadd with an all-zero affine input, point.cpp:2036-2039)
+        if (!aff_is_inf(a)) {
+            if (v >> 31) fp_neg(a.y, a.y);
+            xyzz_madd(acc, a);
+        }
     }
     const bool after = end < M && sorted[end].x == cur;
     if (first && before) {

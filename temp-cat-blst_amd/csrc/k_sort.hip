@@ -48,9 +48,93 @@ __global__ void k_digits_debug(const uint8_t* __restrict__ scalars, uint32_t n, 
                                uint32_t nbits, uint32_t bit0, uint32_t c, int32_t* __restrict__ out) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    uint32_t k[9];
-    load_scalar(k, scalars + (size_t)i * nbytes, nbytes, nbits, false);
-    out[i] = booth_digit(k, bit0, c, nbits);
+    // the tiled sort's reader (word reads for aligned 32-byte scalars)
+    const bool vec = nbytes == 32 && ((uintptr_t)scalars & 15) == 0;
+    out[i] = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
+}
+
+// Tile = chunk k of the points of group g.  LDS holds the B bucket counters
+// of the group's window.
+__global__ void __launch_bounds__(kSortThreads)
+k_count(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ tbase,
+        uint32_t ngroups, uint32_t* __restrict__ counts) {
+    extern __shared__ uint32_t bins[];
+    const uint32_t g = find_owner(tbase, ngroups, blockIdx.x);
+    const sgroup_t S = sg[g];
+    const job_t& J = jobs[S.job];
+    const uint32_t B = J.B, c = J.c, nbits = J.nbits, nbytes = J.nbytes;
+    const bool vec = J.vec;
+    const uint8_t* scalars = J.scalars;
+    for (uint32_t b = threadIdx.x; b < B; b += blockDim.x) bins[b] = 0;
+    __syncthreads();
+    const uint32_t k = blockIdx.x - S.tb;
+    const uint32_t i0 = k * S.ch, i1 = min(i0 + S.ch, J.n);
+    const uint32_t bit0 = J.bit_base + S.w * c;
+    if (bit0 <= nbits) {
+        for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+            const int32_t d = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
+            if (d) atomicAdd(&bins[(uint32_t)(d < 0 ? -d : d) - 1u], 1u);
+        }
+    }
+    __syncthreads();
+    uint32_t* out = counts + S.cb + (size_t)k * B;
+    for (uint32_t b = threadIdx.x; b < B; b += blockDim.x) out[b] = bins[b];
+}
+
+// Per key: chunk counts -> exclusive running offsets inside the key, and the
+// key's total into hist (hist[nkeys] = 0 closes the scan).
+__global__ void __launch_bounds__(256)
+k_prefix(const group_t* __restrict__ groups, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb,
+         uint32_t ngroups, uint32_t nkeys, uint32_t* __restrict__ counts, uint32_t* __restrict__ hist) {
+    const uint32_t key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key > nkeys) return;
+    if (key == nkeys) {
+        hist[key] = 0;
+        return;
+    }
+    const uint32_t g = find_owner(gkb, ngroups, key);
+    const uint32_t B = groups[g].B;
+    const sgroup_t S = sg[g];
+    uint32_t* p = counts + S.cb + (key - groups[g].kb);
+    uint32_t run = 0;
+    for (uint32_t k = 0; k < S.nch; k++, p += B) {
+        const uint32_t v = *p;
+        *p = run;
+        run += v;
+    }
+    hist[key] = run;
+}
+
+// Per tile again: LDS cursors start at the bucket offset plus the chunk's
+// offset inside the bucket and hand out positions.
+__global__ void __launch_bounds__(kSortThreads)
+k_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ tbase,
+        uint32_t ngroups, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off,
+        uint2* __restrict__ sorted) {
+    extern __shared__ uint32_t cursor[];
+    const uint32_t g = find_owner(tbase, ngroups, blockIdx.x);
+    const sgroup_t S = sg[g];
+    const job_t& J = jobs[S.job];
+    const uint32_t B = J.B, c = J.c, nbits = J.nbits, nbytes = J.nbytes;
+    const bool vec = J.vec;
+    const uint8_t* scalars = J.scalars;
+    const uint32_t k = blockIdx.x - S.tb;
+    const uint32_t kb = J.kbase + S.w * B;
+    const uint32_t* cnt = counts + S.cb + (size_t)k * B;
+    for (uint32_t b = threadIdx.x; b < B; b += blockDim.x) cursor[b] = off[kb + b] + cnt[b];
+    __syncthreads();
+    const uint32_t i0 = k * S.ch, i1 = min(i0 + S.ch, J.n);
+    const uint32_t bit0 = J.bit_base + S.w * c;
+    if (bit0 > nbits) return;
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const int32_t d = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
+        if (d) {
+            const uint32_t neg = d < 0;
+            const uint32_t b = (uint32_t)(neg ? -d : d) - 1u;
+            const uint32_t pos = atomicAdd(&cursor[b], 1u);
+            sorted[pos] = make_uint2(kb + b, i | (neg << 31));
+        }
+    }
 }
 
 // largest histogram entry (the biggest bucket) -> *out (pre-zeroed)
@@ -145,6 +229,32 @@ void launch_k_scan_add(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __re
 
 void launch_k_scatter(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, uint32_t njobs, uint32_t total_entries, const uint32_t* __restrict__ ent, const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off, uint2* __restrict__ sorted) {
     hipLaunchKernelGGL(k_scatter, grid, block, 0, stream, jobs, ebase, njobs, total_entries, ent, rank, off, sorted);
+}
+
+void launch_k_count(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ tbase, uint32_t ngroups, uint32_t* __restrict__ counts) {
+    hipLaunchKernelGGL(k_count, grid, block, lds, stream, jobs, sg, tbase, ngroups, counts);
+}
+
+void launch_k_prefix(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, uint32_t ngroups, uint32_t nkeys, uint32_t* __restrict__ counts, uint32_t* __restrict__ hist) {
+    hipLaunchKernelGGL(k_prefix, grid, block, 0, stream, groups, sg, gkb, ngroups, nkeys, counts, hist);
+}
+
+void launch_k_place(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ tbase, uint32_t ngroups, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off, uint2* __restrict__ sorted) {
+    hipLaunchKernelGGL(k_place, grid, block, lds, stream, jobs, sg, tbase, ngroups, counts, off, sorted);
+}
+
+// largest dynamic LDS the tiled sort may ask for (0 when it cannot be raised
+// to what the device allows)
+size_t sort_lds_limit() {
+    int dev = 0, v = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) return 0;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(&k_count), hipFuncAttributeMaxDynamicSharedMemorySize, v) != hipSuccess ||
+        hipFuncSetAttribute(reinterpret_cast<const void*>(&k_place), hipFuncAttributeMaxDynamicSharedMemorySize, v) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return (size_t)v;
 }
 
 }  // namespace bls

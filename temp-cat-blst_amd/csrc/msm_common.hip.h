@@ -1,11 +1,15 @@
 // Shared device definitions of the G1 Pippenger MSM stages on gfx950.
 //
-//   digits      signed Booth digits of every (window, scalar), a global
-//               histogram of (window, bucket) keys with the rank of each
-//               entry inside its key (WindowExtract / booth_encode,
+//   count       per tile (one window of one chunk of points): signed Booth
+//               digits (WindowExtract / booth_encode,
 //               blst/scalar/curve/multi_scalar.h:350-404, ec_mult.h:63-142)
-//   scan        exclusive prefix sum of the histogram -> bucket offsets
-//   scatter     counting-sort placement of (key, point | sign) pairs
+//               counted into an LDS histogram of the window's buckets
+//   prefix      per bucket: offsets of its chunks, and the bucket's total
+//   scan        exclusive prefix sum of the totals -> bucket offsets
+//   place       per tile again: LDS cursors put each (key, point | sign)
+//               pair at its sorted position
+//               (windows with more buckets than the LDS holds use a global
+//               histogram with per-entry ranks instead: digits, scatter)
 //   accumulate  fixed-length chunks of the sorted list per lane, XYZZ mixed
 //               additions of gathered affine points (the bucket fill of
 //               points_tile_pippenger, multi_scalar.cpp:806-865); buckets that
@@ -114,6 +118,39 @@ __device__ __forceinline__ int32_t booth_digit(const uint32_t k[9], uint32_t bit
     return sign ? val - (int32_t)(1u << e) : val;
 }
 
+// len (<= 25) bits of a scalar starting at bit pos, read straight from its
+// bytes; pos + len must not pass the scalar's nbits (so no bit at or above
+// nbits is ever read, as with load_scalar's masking).  vec: 32-byte scalars
+// on a 16-byte boundary, read as 32-bit words.
+__device__ __forceinline__ uint32_t scalar_window(const uint8_t* __restrict__ s, bool vec, uint32_t pos,
+                                                  uint32_t len) {
+    if (vec) {
+        const uint32_t* w = reinterpret_cast<const uint32_t*>(s);
+        const uint32_t wi = pos >> 5, sh = pos & 31;
+        uint64_t v = w[wi];
+        if (sh + len > 32) v |= (uint64_t)w[wi + 1] << 32;
+        return (uint32_t)(v >> sh) & ((1u << len) - 1u);
+    }
+    const uint32_t b0 = pos >> 3, b1 = (pos + len - 1) >> 3;
+    uint32_t v = 0;
+    for (uint32_t b = b0; b <= b1; b++) v |= (uint32_t)s[b] << (8 * (b - b0));
+    return (v >> (pos & 7)) & ((1u << len) - 1u);
+}
+
+// booth_digit of the window at bit0 (<= nbits), reading only that window's
+// bits of the scalar.
+__device__ __forceinline__ int32_t booth_digit_at(const uint8_t* __restrict__ s, bool vec, uint32_t bit0,
+                                                  uint32_t c, uint32_t nbits) {
+    const uint32_t e = min(nbits - bit0 + 1, c);
+    const uint32_t end = min(bit0 + c, nbits);
+    uint32_t wval;
+    if (bit0 == 0) wval = scalar_window(s, vec, 0, end) << 1;
+    else wval = scalar_window(s, vec, bit0 - 1, end - bit0 + 1);
+    const uint32_t sign = (wval >> e) & 1u;
+    const int32_t val = (int32_t)((wval + 1) >> 1);
+    return sign ? val - (int32_t)(1u << e) : val;
+}
+
 // ---------------------------------------------------------------- stages
 //
 // A launch covers a batch of MSMs ("jobs").  Job j owns the key range
@@ -137,6 +174,19 @@ struct group_t {
     uint32_t P;    // segments
     uint32_t pad[3];
 };
+
+// Sort tiles of one group (a window of a job): the job's points are cut into
+// nch chunks of ch points and tile tb + k covers chunk k.  Chunk k's bucket
+// counts live at counts[cb + k * B + bucket].
+struct sgroup_t {
+    uint32_t job, w;
+    uint32_t tb;
+    uint32_t cb;
+    uint32_t nch, ch;
+    uint32_t pad[2];
+};
+
+constexpr uint32_t kSortThreads = 512;
 
 // largest j with base[j] <= t (base ascending, base[0] = 0)
 __device__ __forceinline__ uint32_t find_owner(const uint32_t* __restrict__ base, uint32_t cnt, uint32_t t) {

@@ -45,7 +45,7 @@ struct DevBuf {
 // each captured batch plan and each serving slot own one.
 struct Workspace {
     hipStream_t stream = nullptr;
-    DevBuf ent, rank, hist, off, sorted, buckets, slot_key, slots, H, scan_tmp, W, res, tables, maxcnt;
+    DevBuf ent, rank, counts, hist, off, sorted, buckets, slot_key, slots, H, scan_tmp, W, res, tables, maxcnt;
     std::vector<uint8_t> tables_host;
     uint32_t* host_word = nullptr;       // pinned readback word
     hipEvent_t ev[8] = {};
@@ -88,15 +88,20 @@ struct Layout {
     std::vector<uint32_t> pbase, ebase, kbase;
     std::vector<group_t> groups;
     std::vector<uint32_t> sbase;
+    std::vector<sgroup_t> sgroups;     // sort tiles per group
+    std::vector<uint32_t> tbase, gkb;  // first tile, first key of every group
     uint32_t total_points = 0, total_entries = 0, nkeys = 0, total_segments = 0, max_n = 0;
+    uint32_t ntiles = 0, max_B = 0;
+    size_t count_words = 0;
     uint32_t T = 32;       // sorted entries per accumulate lane
 };
 
 // Device copies of the layout tables (one upload per batch).
 struct DevLayout {
     const job_t* jobs;
-    const uint32_t *pbase, *ebase, *kbase, *sbase;
+    const uint32_t *pbase, *ebase, *kbase, *sbase, *tbase, *gkb;
     const group_t* groups;
+    const sgroup_t* sgroups;
 };
 
 uint32_t device_window(size_t n, size_t nbits);
@@ -106,6 +111,7 @@ DevLayout upload_layout(Workspace& cx, const Layout& L);
 void ensure_workspace(Workspace& cx, const Layout& L);
 size_t workspace_bytes(const Layout& L);
 uint32_t bound_rounds(uint32_t max_n, uint32_t T);
+bool tiled_sort(const Layout& L);
 void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int rounds, bool host_fold, void* host_out,
                    bool timing = true);
 void record_phases(Workspace& cx);
