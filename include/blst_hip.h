@@ -167,6 +167,13 @@ void blst_hip_tuning_get(int batch, size_t npoints, size_t nbits, size_t out[3])
 int blst_hip_booth_digits(int32_t *out, const blst_byte *scalars, size_t npoints,
                           size_t nbits, size_t bit0, size_t window);
 
+/* Test hook: checks the four-lane point additions and doublings of the
+ * reduction stages against the one-lane formulas on the affine points
+ * (device memory, 96 bytes each, on the curve): for each point P and its
+ * successor Q, 2P + 2Q, 2P + 2P in two frames, 2P - 2P, sums with infinity
+ * and doublings.  Returns the number of mismatching coordinates, -1 on error. */
+int blst_hip_point_ops_check(const void *device_points, size_t npoints);
+
 /* [k_i]G for each scalar (generator multiples, affine, all-zero for k = 0),
  * computed on the device from a comb table of G: the synthetic bases of the
  * benchmark workloads and KZG setup points [tau^i]G.  Returns 0 on success. */

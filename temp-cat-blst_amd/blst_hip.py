@@ -25,6 +25,7 @@ SIGNATURES = {
     "blst_hip_available": ([], ctypes.c_int),
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
+    "blst_hip_point_ops_check": ([_vp, _sz], ctypes.c_int),
     "blst_hip_last_phase_ms": ([_vp], None),
     "blst_hip_device_upload": ([_vp, _sz], _vp),
     "blst_hip_device_free": ([_vp], None),
@@ -158,6 +159,15 @@ def booth_digits(scalars, nbits, bit0, window):
     if rc:
         raise RuntimeError("blst_hip_booth_digits: " + last_error())
     return out
+
+
+def point_ops_check(d_points_ptr, n):
+    """Mismatching coordinates between the four-lane and one-lane point
+    formulas on n device-resident affine points."""
+    rc = load().blst_hip_point_ops_check(d_points_ptr, n)
+    if rc < 0:
+        raise RuntimeError("blst_hip_point_ops_check: " + last_error())
+    return rc
 
 
 def msm_device(d_points_ptr, d_scalars_ptr, n, nbits, window=0, chunk=0, segment=0):

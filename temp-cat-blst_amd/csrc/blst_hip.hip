@@ -197,7 +197,9 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
             group_t G = {};
             G.kb = J.kbase + w * J.B;
             G.B = J.B;
-            G.L = std::min<uint32_t>(std::max<uint32_t>(s.L, 1), J.B);
+            uint32_t seg = 1;
+            while (seg * 2 <= std::max<uint32_t>(s.L, 1)) seg *= 2;   // a power of two divides B
+            G.L = std::min<uint32_t>(seg, J.B);
             G.P = J.B / G.L;
             G.sb = (uint32_t)segs;
             segs += G.P;
@@ -241,6 +243,38 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
     if (cw >= 0xffffffffULL) throw HipFail{"batch too large for one launch"};
     L.count_words = cw;
     L.ntiles = (uint32_t)tiles;
+    // reduction tree passes: each block merges up to kTreeNodes nodes of a
+    // group into one, until one node per group is left
+    {
+        const size_t ng = L.groups.size();
+        std::vector<uint32_t> n(ng), in(ng);
+        for (size_t g = 0; g < ng; g++) {
+            n[g] = L.groups[g].P;
+            in[g] = L.groups[g].sb;
+        }
+        size_t in_off = 0, in_nodes = segs;
+        while (true) {
+            uint32_t blocks = 0, outn = 0;
+            bool more = false;
+            for (size_t g = 0; g < ng; g++) {
+                const uint32_t b = div_up(n[g], kTreeNodes);
+                L.tree_blk.push_back(blocks);
+                L.tree_tn.push_back(make_uint4(in[g], n[g], outn, 0));
+                blocks += b;
+                in[g] = outn;
+                outn += b;
+                n[g] = b;
+                more = more || b > 1;
+            }
+            L.tree_blocks.push_back(blocks);
+            L.tree_in.push_back(in_off);
+            L.tree_out.push_back(in_off + in_nodes);
+            if (!more) break;
+            in_off += in_nodes;
+            in_nodes = outn;
+        }
+        L.tree_nodes = L.tree_out.back();
+    }
     L.total_points = (uint32_t)pts;
     L.total_entries = (uint32_t)ent;
     L.nkeys = (uint32_t)keys;
@@ -254,7 +288,8 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     const size_t o_jobs = 0, o_p = al(o_jobs + nj * sizeof(job_t)), o_e = al(o_p + nj * 4),
                  o_k = al(o_e + nj * 4), o_g = al(o_k + nj * 4), o_s = al(o_g + ng * sizeof(group_t)),
                  o_sg = al(o_s + ng * 4), o_tb = al(o_sg + ng * sizeof(sgroup_t)), o_gk = al(o_tb + ng * 4),
-                 total = al(o_gk + ng * 4);
+                 o_trb = al(o_gk + ng * 4), o_trt = al(o_trb + L.tree_blk.size() * 4),
+                 total = al(o_trt + L.tree_tn.size() * sizeof(uint4));
     cx.tables.ensure(total);
     cx.tables_host.resize(total);
     uint8_t* h = cx.tables_host.data();
@@ -267,6 +302,8 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     std::memcpy(h + o_sg, L.sgroups.data(), ng * sizeof(sgroup_t));
     std::memcpy(h + o_tb, L.tbase.data(), ng * 4);
     std::memcpy(h + o_gk, L.gkb.data(), ng * 4);
+    std::memcpy(h + o_trb, L.tree_blk.data(), L.tree_blk.size() * 4);
+    std::memcpy(h + o_trt, L.tree_tn.data(), L.tree_tn.size() * sizeof(uint4));
     HIPCK(hipMemcpyAsync(cx.tables.p, h, total, hipMemcpyHostToDevice, cx.stream));
     uint8_t* d = cx.tables.as<uint8_t>();
     DevLayout D;
@@ -279,6 +316,8 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     D.sgroups = reinterpret_cast<const sgroup_t*>(d + o_sg);
     D.tbase = reinterpret_cast<const uint32_t*>(d + o_tb);
     D.gkb = reinterpret_cast<const uint32_t*>(d + o_gk);
+    D.tree_blk = reinterpret_cast<const uint32_t*>(d + o_trb);
+    D.tree_tn = reinterpret_cast<const uint4*>(d + o_trt);
     return D;
 }
 
@@ -314,21 +353,11 @@ void fill_words(Workspace& cx, void* dst, size_t words, uint32_t value) {
                        static_cast<uint32_t*>(dst), words, value);
 }
 
-// Partial-tree rounds that cover any bucket of jobs of at most max_n points:
-// a bucket holds at most n entries of one window, so it spans at most
-// ceil(n / T) + 1 chunks.
 // The tiled sort keeps a window's bucket counters in LDS; wider windows take
 // the global-histogram path.
 bool tiled_sort(const Layout& L) {
     static const size_t limit = sort_lds_limit();
     return (size_t)L.max_B * 4 <= limit;
-}
-
-uint32_t bound_rounds(uint32_t max_n, uint32_t T) {
-    const uint32_t span = div_up(max_n, T) + 1;
-    uint32_t r = 0;
-    while ((1u << r) < span) r++;
-    return r;
 }
 
 void ensure_workspace(Workspace& cx, const Layout& L) {
@@ -347,7 +376,8 @@ void ensure_workspace(Workspace& cx, const Layout& L) {
     cx.buckets.ensure((size_t)nkeys * sizeof(xyzz));
     cx.slot_key.ensure((size_t)nslots * 4);
     cx.slots.ensure((size_t)nslots * sizeof(xyzz));
-    cx.H.ensure((size_t)L.total_segments * sizeof(xyzz));
+    cx.heavy.ensure(((size_t)nslots / 2 + 64) * 4);
+    cx.H.ensure(L.tree_nodes * 2 * sizeof(xyzz));
     cx.W.ensure((size_t)L.groups.size() * sizeof(xyzz));
     cx.res.ensure((size_t)L.jobs.size() * sizeof(xyzz));
 }
@@ -357,17 +387,15 @@ size_t workspace_bytes(const Layout& L) {
     const size_t nslots = 2 * (size_t)std::max<uint32_t>(div_up(L.total_entries, L.T), 1);
     const size_t sort_bytes = tiled_sort(L) ? L.count_words * 4 : (size_t)L.total_entries * 8;
     return (size_t)L.total_entries * 8 + sort_bytes + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * sizeof(xyzz) +
-           nslots * (4 + sizeof(xyzz)) + (size_t)L.total_segments * sizeof(xyzz) +
+           nslots * (6 + sizeof(xyzz)) + L.tree_nodes * 2 * sizeof(xyzz) +
            (L.groups.size() + L.jobs.size()) * sizeof(xyzz) + (1 << 20);
 }
 
-// Enqueues the device pipeline of a batch on cx.stream.  rounds < 0: measure
-// the biggest bucket (one synchronising 4-byte readback) and run just the
-// partial-tree rounds it needs; rounds >= 0: run that many (capturable).
-// With host_fold the window sums W[g] are copied to host_out (one xyzz per
+// Enqueues the device pipeline of a batch on cx.stream; nothing in it waits
+// for the host, so the sequence can be captured in a HIP graph.  With host_fold the window sums W[g] are copied to host_out (one xyzz per
 // group); otherwise k_fold combines the windows of every job and one xyzz per
 // job is copied.  host_out should be pinned for the copy to be asynchronous.
-void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int rounds, bool host_fold, void* host_out,
+void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host_fold, void* host_out,
                    bool timing) {
     const uint32_t nkeys = L.nkeys;
     const uint32_t nent = L.total_entries;
@@ -411,36 +439,25 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int round
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
 
-    uint32_t nrounds;
-    if (rounds < 0) {
-        fill_words(cx, cx.maxcnt.p, 1, 0);
-        launch_k_max_count(dim3(std::min<uint32_t>(div_up(nkeys, 256), 1024)), dim3(256), st, hist, nkeys,
-                           cx.maxcnt.as<uint32_t>());
-        HIPCK(hipMemcpyAsync(cx.host_word, cx.maxcnt.p, 4, hipMemcpyDeviceToHost, st));
-        HIPCK(hipStreamSynchronize(st));
-        const uint32_t maxspan = div_up(*cx.host_word, T) + 1;
-        nrounds = 0;
-        while ((1u << nrounds) < maxspan) nrounds++;
-    } else {
-        nrounds = (uint32_t)rounds;
-    }
-    for (uint32_t r = 0; r < nrounds; r++)
-        launch_k_partial_round(dim3(div_up(nslots, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
-                               cx.slots.as<xyzz>(), off, hist, T, nslots, 1u << r);
-    launch_k_partial_final(dim3(div_up(nslots, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
-                           cx.slots.as<const xyzz>(), cx.buckets.as<xyzz>(), nslots);
+    // buckets spanning chunks: two partials in one lane, longer spans listed
+    // for one workgroup each
+    uint32_t* heavy = cx.heavy.as<uint32_t>();
+    fill_words(cx, heavy, 1, 0);
+    launch_k_partial_light(dim3(div_up(nchunks, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
+                           cx.slots.as<const xyzz>(), off, hist, T, nchunks, cx.buckets.as<xyzz>(), heavy, heavy + 64);
+    launch_k_partial_heavy(dim3(std::min<uint32_t>(nchunks, 256)), st, heavy, heavy + 64, cx.slots.as<const xyzz>(),
+                           off, hist, T, cx.buckets.as<xyzz>());
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[4], st));
 
+    xyzz* Hn = cx.H.as<xyzz>();
     launch_k_reduce_segments(dim3(div_up(L.total_segments, 256)), dim3(256), st, D.groups, D.sbase, ngroups,
-                             L.total_segments, cx.buckets.as<const xyzz>(), cx.H.as<xyzz>());
-    uint32_t maxP = 1;
-    for (const group_t& G : L.groups) maxP = std::max(maxP, G.P);
-    for (uint32_t step = 1; step < maxP; step <<= 1)
-        launch_k_pair_sum(dim3(div_up(L.total_segments, 256)), dim3(256), st, D.groups, D.sbase, ngroups,
-                          L.total_segments, cx.H.as<xyzz>(), step);
-    launch_k_gather_windows(dim3(div_up(ngroups, 256)), dim3(256), st, D.groups, ngroups, cx.H.as<const xyzz>(),
-                            cx.W.as<xyzz>());
+                             L.total_segments, cx.buckets.as<const xyzz>(), Hn);
+    const size_t npass = L.tree_blocks.size();
+    for (size_t p = 0; p < npass; p++)
+        launch_k_reduce_tree(dim3(L.tree_blocks[p]), st, D.tree_blk + p * ngroups, D.tree_tn + p * ngroups, ngroups,
+                             Hn + 2 * L.tree_in[p], Hn + 2 * L.tree_out[p], cx.W.as<xyzz>(),
+                             p + 1 == npass ? 1u : 0u);
     HIPCK(hipGetLastError());
     if (host_fold) {
         HIPCK(hipMemcpyAsync(host_out, cx.W.p, (size_t)ngroups * sizeof(xyzz), hipMemcpyDeviceToHost, st));
@@ -466,7 +483,7 @@ void run_batch(Workspace& cx, const Layout& L, bool host_fold, std::vector<hoste
     ensure_workspace(cx, L);
     const DevLayout D = upload_layout(cx, L);
     out.resize(host_fold ? L.groups.size() : L.jobs.size());
-    enqueue_batch(cx, L, D, -1, host_fold, out.data());
+    enqueue_batch(cx, L, D, host_fold, out.data());
     HIPCK(hipStreamSynchronize(cx.stream));
     record_phases(cx);
 }
@@ -786,7 +803,7 @@ blst_hip_batch_plan* blst_hip_batch_plan_create(const blst_hip_msm_job* jobs, si
             HIPCK(hipHostMalloc((void**)&P.host_res, specs.size() * sizeof(hostec::xyzz6), hipHostMallocDefault));
             HIPCK(hipStreamSynchronize(P.ws.stream));
             HIPCK(hipStreamBeginCapture(P.ws.stream, hipStreamCaptureModeThreadLocal));
-            enqueue_batch(P.ws, P.L, P.D, (int)bound_rounds(P.L.max_n, P.L.T), false, P.host_res, false);
+            enqueue_batch(P.ws, P.L, P.D, false, P.host_res, false);
             HIPCK(hipStreamEndCapture(P.ws.stream, &P.graph));
             HIPCK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
         }
@@ -822,6 +839,31 @@ int blst_hip_batch_plan_run(blst_hip_batch_plan* plan, blst_p1* rets) {
 }
 
 void blst_hip_batch_plan_destroy(blst_hip_batch_plan* plan) { delete plan; }
+
+int blst_hip_point_ops_check(const void* device_points, size_t npoints) {
+    if (npoints == 0) return 0;
+    if (!device_points || npoints > (1u << 24) || ((uintptr_t)device_points & 15)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        cx.dbg.ensure(64);
+        fill_words(cx.ws, cx.dbg.p, 1, 0);
+        launch_k_quad_check(dim3(div_up(4 * npoints, 256)), dim3(256), cx.stream,
+                            static_cast<const uint4*>(device_points), (uint32_t)npoints, cx.dbg.as<uint32_t>());
+        HIPCK(hipGetLastError());
+        uint32_t bad = 0;
+        HIPCK(hipMemcpyAsync(&bad, cx.dbg.p, 4, hipMemcpyDeviceToHost, cx.stream));
+        HIPCK(hipStreamSynchronize(cx.stream));
+        g_err.clear();
+        return (int)bad;
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+}
 
 int blst_hip_booth_digits(int32_t* out, const blst_byte* scalars, size_t npoints, size_t nbits,
                           size_t bit0, size_t window) {

@@ -1,109 +1,118 @@
-// Bucket reduction sum_b b*S_b per window and the window sums.
+// Bucket reduction sum_b (b+1) S_b per window (point_integrate_buckets,
+// blst/scalar/curve/multi_scalar.cpp:730-751).
+//
+// The B buckets of a window are cut into P = B/L segments of L buckets (L a
+// power of two).  Segment s yields the node
+//     A_s = sum_t (t+1) S_(sL+t)           (running sum, high to low)
+//     V_s = L * sum_t S_(sL+t)             (log2 L doublings)
+// and a node for segments [a, a+n) is A = sum_s (U_s + (s-a) L T_s) with
+// V = n L sum_s T_s.  Adjacent nodes merge as
+//     A = A_left + (A_right + V_right),   V = 2 (V_left + V_right),
+// so the root's A is the window sum.  The merges run as a tree of four-lane
+// additions: per workgroup 64 nodes in LDS, as many passes as P needs.
 #include "msm_common.hip.h"
+#include "ec_quad.hip.h"
 #include "launch.h"
 
 namespace bls {
 
-// Segment s of L buckets of group g (bucket b holds digit b+1):
-// H = sum_t (t+1) S_(sL+t) + (sL) * sum_t S_(sL+t), the segment's share of
-// sum_b (b+1) S_b.  The running sum (acc += S, tot += acc, high to low), the
-// double-and-add of sL * acc and the final tot += r share one addition site
-// and one doubling site, which keeps the kernel small.
+// Segment running sums: one lane per segment.  The running sum (acc += S,
+// tot += acc) shares one addition site and the doublings of V one doubling
+// site, which keeps the kernel small.
 __global__ void __launch_bounds__(256)
 k_reduce_segments(const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase,
                   uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets,
-                  xyzz* __restrict__ H) {
+                  xyzz* __restrict__ nodes) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total_segments) return;
     const uint32_t g = find_owner(sbase, ngroups, t);
     const group_t G = groups[g];
     const uint32_t s = t - G.sb;
     const size_t base = (size_t)G.kb + (size_t)s * G.L;
-    const uint32_t m = s * G.L;
-    const int mbits = m ? 32 - __builtin_clz(m) : 0;
-    xyzz acc, tot, r, x, y;
+    xyzz acc, tot, x, y;
     xyzz_set_inf(acc);
     xyzz_set_inf(tot);
-    xyzz_set_inf(r);
-    // steps 0 .. 2L-1: running sum; then per bit of m (high to low): double r
-    // and, for a set bit, r += acc; last: tot += r
-    const int nrun = 2 * (int)G.L;
-    int bit = mbits - 1;
-    bool dbl_pending = true;
 #pragma unroll 1
-    for (int step = 0;; step++) {
-        int kind;                       // 0 acc+=S, 1 tot+=acc, 2 r+=acc, 3 tot+=r
-        if (step < nrun) {
-            kind = step & 1;
-        } else if (bit >= 0) {
-            if (dbl_pending) {
-                xyzz_dbl_inl(r, r);
-                dbl_pending = false;
-                if (!((m >> bit) & 1u)) { bit--; dbl_pending = true; continue; }
-            }
-            kind = 2;
+    for (uint32_t step = 0; step < 2 * G.L; step++) {
+        if (step & 1) {
+            x = tot;
+            y = acc;
         } else {
-            kind = 3;
-        }
-        if (kind == 0) {
-            load_xyzz(y, &buckets[base + (G.L - 1 - (uint32_t)(step >> 1))]);
             x = acc;
-        } else if (kind == 1) {
-            x = tot;
-            y = acc;
-        } else if (kind == 2) {
-            x = r;
-            y = acc;
-        } else {
-            x = tot;
-            y = r;
+            load_xyzz(y, &buckets[base + (G.L - 1 - (step >> 1))]);
         }
         xyzz_add(x, x, y);
-        if (kind == 0) acc = x;
-        else if (kind == 2) { r = x; bit--; dbl_pending = true; }
-        else tot = x;
-        if (kind == 3) break;
+        if (step & 1) tot = x;
+        else acc = x;
     }
-    store_xyzz(&H[t], tot);
+    store_xyzz(&nodes[2 * (size_t)t], tot);
+#pragma unroll 1
+    for (uint32_t l = G.L; l > 1; l >>= 1) xyzz_dbl_inl(acc, acc);
+    store_xyzz(&nodes[2 * (size_t)t + 1], acc);
 }
 
-// within each group: H[sb + s] += H[sb + s + step] for s a multiple of 2*step
+// One workgroup merges n <= 64 consecutive nodes of a group into one.  Quads
+// 0-31 (waves 0-1) compute the A of the merges and quads 32-63 (waves 2-3)
+// the V, so every wave follows one instruction stream; a level with fewer
+// than four merges repeats them on four quads (16 lanes) and stores one.
+// tn[g] = (first input node, input nodes, first output node); blk[g] = first
+// block of group g.  last: store the root's A to W[g] instead.
 __global__ void __launch_bounds__(256)
-k_pair_sum(const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups,
-           uint32_t total_segments, xyzz* __restrict__ H, uint32_t step) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= total_segments) return;
-    const uint32_t g = find_owner(sbase, ngroups, t);
-    const uint32_t s = t - groups[g].sb;
-    if ((s % (2 * step)) || s + step >= groups[g].P) return;
-    xyzz x, y;
-    load_xyzz(x, &H[t]);
-    load_xyzz(y, &H[t + step]);
-    xyzz_add(x, x, y);
-    store_xyzz(&H[t], x);
+k_reduce_tree(const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups,
+              const xyzz* __restrict__ in, xyzz* __restrict__ out, xyzz* __restrict__ W, uint32_t last) {
+    __shared__ xyzz sA[kTreeNodes], sV[kTreeNodes];
+    const uint32_t g = find_owner(blk, ngroups, blockIdx.x);
+    const uint32_t j = blockIdx.x - blk[g];
+    const uint4 t = tn[g];
+    const uint32_t first = t.x + kTreeNodes * j;
+    const uint32_t n = min(kTreeNodes, t.y - kTreeNodes * j);
+    const uint32_t u = threadIdx.x >> 2, ln = threadIdx.x & 3;
+    const bool beta = u >= 32;
+    const uint32_t v = u & 31;
+    if (u < n) {
+        fp a, b;
+        quad_load(a, &in[2 * (size_t)(first + u)]);
+        quad_load(b, &in[2 * (size_t)(first + u) + 1]);
+        reinterpret_cast<fp*>(&sA[u])[ln] = a;
+        reinterpret_cast<fp*>(&sV[u])[ln] = b;
+    }
+    __syncthreads();
+    for (uint32_t m = n >> 1; m >= 1; m >>= 1) {
+        fp r;
+        if (v < max(m, 4u)) {
+            const uint32_t mm = v % m;
+            fp x, y;
+            if (!beta) {
+                x = reinterpret_cast<const fp*>(&sA[2 * mm + 1])[ln];
+                y = reinterpret_cast<const fp*>(&sV[2 * mm + 1])[ln];
+                quad_add(x, x, y);
+                y = reinterpret_cast<const fp*>(&sA[2 * mm])[ln];
+                quad_add(r, y, x);
+            } else {
+                x = reinterpret_cast<const fp*>(&sV[2 * mm])[ln];
+                y = reinterpret_cast<const fp*>(&sV[2 * mm + 1])[ln];
+                quad_add(x, x, y);
+                quad_dbl(r, x);
+            }
+        }
+        __syncthreads();
+        if (v < m) reinterpret_cast<fp*>(beta ? &sV[v] : &sA[v])[ln] = r;
+        __syncthreads();
+    }
+    if (last) {
+        if (u == 0) quad_store(&W[g], reinterpret_cast<const fp*>(&sA[0])[ln]);
+    } else if (u == 0) {
+        quad_store(&out[2 * (size_t)(t.z + j)], reinterpret_cast<const fp*>(&sA[0])[ln]);
+        quad_store(&out[2 * (size_t)(t.z + j) + 1], reinterpret_cast<const fp*>(&sV[0])[ln]);
+    }
 }
 
-// window sums of every group -> W[g]
-__global__ void __launch_bounds__(256)
-k_gather_windows(const group_t* __restrict__ groups, uint32_t ngroups, const xyzz* __restrict__ H,
-                 xyzz* __restrict__ W) {
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    xyzz x;
-    load_xyzz(x, &H[groups[g].sb]);
-    store_xyzz(&W[g], x);
+void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets, xyzz* __restrict__ nodes) {
+    hipLaunchKernelGGL(k_reduce_segments, grid, block, 0, stream, groups, sbase, ngroups, total_segments, buckets, nodes);
 }
 
-void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets, xyzz* __restrict__ H) {
-    hipLaunchKernelGGL(k_reduce_segments, grid, block, 0, stream, groups, sbase, ngroups, total_segments, buckets, H);
-}
-
-void launch_k_pair_sum(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, xyzz* __restrict__ H, uint32_t step) {
-    hipLaunchKernelGGL(k_pair_sum, grid, block, 0, stream, groups, sbase, ngroups, total_segments, H, step);
-}
-
-void launch_k_gather_windows(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, uint32_t ngroups, const xyzz* __restrict__ H, xyzz* __restrict__ W) {
-    hipLaunchKernelGGL(k_gather_windows, grid, block, 0, stream, groups, ngroups, H, W);
+void launch_k_reduce_tree(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups, const xyzz* __restrict__ in, xyzz* __restrict__ out, xyzz* __restrict__ W, uint32_t last) {
+    hipLaunchKernelGGL(k_reduce_tree, grid, dim3(256), 0, stream, blk, tn, ngroups, in, out, W, last);
 }
 
 }  // namespace bls

@@ -137,16 +137,6 @@ k_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const u
     }
 }
 
-// largest histogram entry (the biggest bucket) -> *out (pre-zeroed)
-__global__ void __launch_bounds__(256)
-k_max_count(const uint32_t* __restrict__ hist, uint32_t N, uint32_t* __restrict__ out) {
-    uint32_t m = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x)
-        m = max(m, hist[i]);
-    for (int off = 32; off > 0; off >>= 1) m = max(m, (uint32_t)__shfl_xor(m, off));
-    if ((threadIdx.x & 63) == 0 && m) atomicMax(out, m);
-}
-
 // dst[0..words) = value; a kernel rather than a memset node so the sequence
 // can be captured in a HIP graph
 __global__ void __launch_bounds__(256)
@@ -209,10 +199,6 @@ void launch_k_digits(dim3 grid, dim3 block, hipStream_t stream, const job_t* __r
 
 void launch_k_digits_debug(dim3 grid, dim3 block, hipStream_t stream, const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits, uint32_t bit0, uint32_t c, int32_t* __restrict__ out) {
     hipLaunchKernelGGL(k_digits_debug, grid, block, 0, stream, scalars, n, nbytes, nbits, bit0, c, out);
-}
-
-void launch_k_max_count(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ hist, uint32_t N, uint32_t* __restrict__ out) {
-    hipLaunchKernelGGL(k_max_count, grid, block, 0, stream, hist, N, out);
 }
 
 void launch_k_fill(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restrict__ dst, size_t words, uint32_t value) {

@@ -14,10 +14,12 @@
 //               additions of gathered affine points (the bucket fill of
 //               points_tile_pippenger, multi_scalar.cpp:806-865); buckets that
 //               cross chunk boundaries leave partial sums in two slots/chunk
-//   partials    log-depth pairwise tree over the partial sums of a bucket
+//   partials    buckets crossing one chunk boundary: one lane adds the two
+//               partials; longer spans: one workgroup per bucket, four-lane
+//               additions in strides, then a tree in LDS
 //   reduce      sum_b b * S_b per window (point_integrate_buckets,
-//               multi_scalar.cpp:730-751) as segment running sums plus a
-//               per-segment scalar weight, then a pairwise tree per window
+//               multi_scalar.cpp:730-751): running sums over segments of L
+//               buckets, then a tree of four-lane merges per window
 //
 // Keys are (window * B + bucket) with B = 2^(c-1) buckets per window, so all
 // windows go through one sort and one accumulate launch.
@@ -187,6 +189,9 @@ struct sgroup_t {
 };
 
 constexpr uint32_t kSortThreads = 512;
+
+// nodes one block of the reduction tree merges into one
+constexpr uint32_t kTreeNodes = 64;
 
 // largest j with base[j] <= t (base ascending, base[0] = 0)
 __device__ __forceinline__ uint32_t find_owner(const uint32_t* __restrict__ base, uint32_t cnt, uint32_t t) {

@@ -45,18 +45,14 @@ struct DevBuf {
 // each captured batch plan and each serving slot own one.
 struct Workspace {
     hipStream_t stream = nullptr;
-    DevBuf ent, rank, counts, hist, off, sorted, buckets, slot_key, slots, H, scan_tmp, W, res, tables, maxcnt;
+    DevBuf ent, rank, counts, hist, off, sorted, buckets, slot_key, slots, heavy, H, scan_tmp, W, res, tables;
     std::vector<uint8_t> tables_host;
-    uint32_t* host_word = nullptr;       // pinned readback word
     hipEvent_t ev[8] = {};
     void init() {
         HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto& e : ev) HIPCK(hipEventCreate(&e));
-        HIPCK(hipHostMalloc((void**)&host_word, 64, hipHostMallocDefault));
-        maxcnt.ensure(64);
     }
     ~Workspace() {
-        if (host_word) (void)hipHostFree(host_word);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -93,6 +89,14 @@ struct Layout {
     uint32_t total_points = 0, total_entries = 0, nkeys = 0, total_segments = 0, max_n = 0;
     uint32_t ntiles = 0, max_B = 0;
     size_t count_words = 0;
+    // reduction tree: per pass, blocks and node offsets (in nodes of two
+    // xyzz) of its input and output arrays; per pass and group the first
+    // block and (first input node, input nodes, first output node)
+    std::vector<uint32_t> tree_blocks;
+    std::vector<size_t> tree_in, tree_out;
+    std::vector<uint32_t> tree_blk;
+    std::vector<uint4> tree_tn;
+    size_t tree_nodes = 0;
     uint32_t T = 32;       // sorted entries per accumulate lane
 };
 
@@ -102,6 +106,8 @@ struct DevLayout {
     const uint32_t *pbase, *ebase, *kbase, *sbase, *tbase, *gkb;
     const group_t* groups;
     const sgroup_t* sgroups;
+    const uint32_t* tree_blk;
+    const uint4* tree_tn;
 };
 
 uint32_t device_window(size_t n, size_t nbits);
@@ -110,9 +116,8 @@ Layout make_layout(const std::vector<JobSpec>& specs);
 DevLayout upload_layout(Workspace& cx, const Layout& L);
 void ensure_workspace(Workspace& cx, const Layout& L);
 size_t workspace_bytes(const Layout& L);
-uint32_t bound_rounds(uint32_t max_n, uint32_t T);
 bool tiled_sort(const Layout& L);
-void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, int rounds, bool host_fold, void* host_out,
+void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host_fold, void* host_out,
                    bool timing = true);
 void record_phases(Workspace& cx);
 

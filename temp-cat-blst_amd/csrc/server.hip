@@ -277,7 +277,7 @@ private:
             s.host_res_cap = std::max<size_t>(batch.size(), 64);
             HIPCK(hipHostMalloc((void**)&s.host_res, s.host_res_cap * sizeof(hostec::xyzz6), hipHostMallocDefault));
         }
-        enqueue_batch(s.ws, s.L, D, (int)bound_rounds(s.L.max_n, s.L.T), false, s.host_res, false);
+        enqueue_batch(s.ws, s.L, D, false, s.host_res, false);
         HIPCK(hipEventRecord(s.done, s.ws.stream));
         s.reqs = std::move(batch);
         s.busy = true;

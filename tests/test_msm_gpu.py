@@ -139,6 +139,25 @@ def test_all_scalars_equal_single_heavy_bucket(hip, orc, base_points):
     assert hip.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == bytes([0xC0]) + bytes(47)
 
 
+def test_heavy_bucket_long_span(hip, orc, base_points):
+    # one bucket per window holding all 2^16 entries: its partial sums span
+    # far more chunks than one merge workgroup has quads
+    n = 1 << 16
+    pts = np.ascontiguousarray(np.tile(base_points, (n // base_points.shape[0], 1)))
+    sc = _scal([0x1234567890ABCDEF1122334455667788] * n)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
+
+
+def test_quad_point_ops_match_one_lane(hip, base_points):
+    pts = np.ascontiguousarray(base_points[:1024])
+    d = hip.DeviceBuffer(pts)
+    try:
+        assert hip.point_ops_check(d.ptr, pts.shape[0]) == 0
+    finally:
+        d.close()
+
+
 def test_scalars_beyond_r_256_bits(hip, orc):
     A = np.array([pyref.affine_to_limbs(pyref.G)] * 2, dtype=np.uint64)
     S = np.full((2, 32), 0xFF, np.uint8)
