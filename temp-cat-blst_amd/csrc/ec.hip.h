@@ -142,6 +142,14 @@ __device__ __forceinline__ void xyzz_add(xyzz& r, const xyzz& p, const xyzz& q) 
     r = o;
 }
 
+// canonical coordinates, for points that leave the device
+__device__ __forceinline__ void xyzz_canon(xyzz& r, const xyzz& p) {
+    fp_canon(r.x, p.x);
+    fp_canon(r.y, p.y);
+    fp_canon(r.zz, p.zz);
+    fp_canon(r.zzz, p.zzz);
+}
+
 __device__ __forceinline__ void xyzz_neg(xyzz& r, const xyzz& p) {
     r = p;
     fp_neg(r.y, p.y);

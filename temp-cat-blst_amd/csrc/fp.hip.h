@@ -55,27 +55,45 @@ __device__ __forceinline__ void fp_zero(fp& r) {
     for (int i = 0; i < 12; i++) r.l[i] = 0;
 }
 
-__device__ __forceinline__ bool fp_is_zero(const fp& a) {
-    uint32_t acc = 0;
-#pragma unroll
-    for (int i = 0; i < 12; i++) acc |= a.l[i];
-    return acc == 0;
+__device__ __forceinline__ uint32_t p2_limb(int i) {
+    constexpr uint32_t P2[12] = {0xffff5556u, 0x73fdffffu, 0x62a7ffffu, 0x3d57fffdu, 0xed61ec48u, 0xce61a541u,
+                                 0xe70a257eu, 0xc8ee9709u, 0x869759aeu, 0x96374f6cu, 0x72ffcd34u, 0x340223d4u};
+    return P2[i];
 }
 
-__device__ __forceinline__ bool fp_eq(const fp& a, const fp& b) {
-    uint32_t acc = 0;
-#pragma unroll
-    for (int i = 0; i < 12; i++) acc |= a.l[i] ^ b.l[i];
-    return acc == 0;
-}
+// Values live in [0, 2p) ("2p-redundant"): fp_mul leaves out the final
+// subtraction (inputs below 2p keep the Montgomery output below 2p because
+// 4p < 2^384), additions and subtractions reduce modulo 2p, and a value is
+// brought to [0, p) only where it leaves the device (fp_canon).
 
-// r = (t >= p) ? t - p : t, for t < 2p (twelve limbs, no overflow word)
-__device__ __forceinline__ void fp_reduce_once(fp& r, const uint32_t t[12]) {
+// r = (t >= p) ? t - p : t, for t < 2p: the canonical representative
+__device__ __forceinline__ void fp_canon(fp& r, const fp& t) {
     uint32_t d[12];
     uint32_t borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 12; i++) d[i] = __builtin_subc(t[i], p_limb(i), borrow, &borrow);
+    for (int i = 0; i < 12; i++) d[i] = __builtin_subc(t.l[i], p_limb(i), borrow, &borrow);
     // borrow == 1  <=>  t < p
+#pragma unroll
+    for (int i = 0; i < 12; i++) r.l[i] = borrow ? t.l[i] : d[i];
+}
+
+// zero modulo p: 0 or p
+__device__ __forceinline__ bool fp_is_zero(const fp& a) {
+    uint32_t z = 0, q = 0;
+#pragma unroll
+    for (int i = 0; i < 12; i++) {
+        z |= a.l[i];
+        q |= a.l[i] ^ p_limb(i);
+    }
+    return z == 0 || q == 0;
+}
+
+// r = (t >= 2p) ? t - 2p : t, for t < 4p
+__device__ __forceinline__ void fp_reduce_2p(fp& r, const uint32_t t[12]) {
+    uint32_t d[12];
+    uint32_t borrow = 0;
+#pragma unroll
+    for (int i = 0; i < 12; i++) d[i] = __builtin_subc(t[i], p2_limb(i), borrow, &borrow);
 #pragma unroll
     for (int i = 0; i < 12; i++) r.l[i] = borrow ? t[i] : d[i];
 }
@@ -85,7 +103,7 @@ __device__ __forceinline__ void fp_add(fp& r, const fp& a, const fp& b) {
     uint32_t c = 0;
 #pragma unroll
     for (int i = 0; i < 12; i++) s[i] = __builtin_addc(a.l[i], b.l[i], c, &c);
-    fp_reduce_once(r, s);   // a + b < 2p < 2^382: no carry out of limb 11
+    fp_reduce_2p(r, s);   // a + b < 4p < 2^383: no carry out of limb 11
 }
 
 __device__ __forceinline__ void fp_sub(fp& r, const fp& a, const fp& b) {
@@ -93,24 +111,26 @@ __device__ __forceinline__ void fp_sub(fp& r, const fp& a, const fp& b) {
     uint32_t borrow = 0;
 #pragma unroll
     for (int i = 0; i < 12; i++) d[i] = __builtin_subc(a.l[i], b.l[i], borrow, &borrow);
-    // if a < b add p back: mask is all-ones on borrow
+    // if a < b add 2p back: mask is all-ones on borrow
     const uint32_t mask = 0u - borrow;
     uint32_t c = 0;
 #pragma unroll
-    for (int i = 0; i < 12; i++) r.l[i] = __builtin_addc(d[i], p_limb(i) & mask, c, &c);
+    for (int i = 0; i < 12; i++) r.l[i] = __builtin_addc(d[i], p2_limb(i) & mask, c, &c);
 }
 
 __device__ __forceinline__ void fp_dbl(fp& r, const fp& a) { fp_add(r, a, a); }
 
 __device__ __forceinline__ void fp_neg(fp& r, const fp& a) {
-    // p - a, and 0 stays 0
+    // 2p - a, and 0 stays 0
     uint32_t d[12];
     uint32_t borrow = 0;
 #pragma unroll
-    for (int i = 0; i < 12; i++) d[i] = __builtin_subc(p_limb(i), a.l[i], borrow, &borrow);
-    const bool z = fp_is_zero(a);
+    for (int i = 0; i < 12; i++) d[i] = __builtin_subc(p2_limb(i), a.l[i], borrow, &borrow);
+    uint32_t z = 0;
 #pragma unroll
-    for (int i = 0; i < 12; i++) r.l[i] = z ? 0u : d[i];
+    for (int i = 0; i < 12; i++) z |= a.l[i];
+#pragma unroll
+    for (int i = 0; i < 12; i++) r.l[i] = z ? d[i] : 0u;
 }
 
 // conditional negate: r = neg ? -a : a
@@ -121,10 +141,14 @@ __device__ __forceinline__ void fp_cneg(fp& r, const fp& a, bool neg) {
     for (int i = 0; i < 12; i++) r.l[i] = neg ? n.l[i] : a.l[i];
 }
 
-// acc(64) += x*y with the carry out of bit 64 counted into top.
-// One v_mad_u64_u32 writes the 64-bit sum of the product and the aligned
-// accumulator pair and its carry-out to VCC; one v_addc folds VCC into top.
-__device__ __forceinline__ void mac3(uint64_t& acc, uint32_t& top, uint32_t x, uint32_t y) {
+// Multiply-accumulate groups: acc(64) += sum of x_j*y_j with every carry out
+// of bit 64 counted into top.  One v_mad_u64_u32 adds a product to the
+// aligned accumulator pair and writes its carry to VCC, one v_addc folds VCC
+// into top.  Several products share one asm statement so the compiler's
+// hazard padding between statements is paid once per group; the p limbs go
+// in scalar registers.  The _new forms start the column's top at the first
+// carry instead of adding to it.
+__device__ __forceinline__ void mac1(uint64_t& acc, uint32_t& top, uint32_t x, uint32_t y) {
     asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
         "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
         : "+v"(acc), "+v"(top)
@@ -132,43 +156,185 @@ __device__ __forceinline__ void mac3(uint64_t& acc, uint32_t& top, uint32_t x, u
         : "vcc");
 }
 
+__device__ __forceinline__ void mac1_ps(uint64_t& acc, uint32_t& top, uint32_t x, uint32_t py) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(top)
+        : "v"(x), "s"(py)
+        : "vcc");
+}
+
+// acc += a0*b0 + m0*p0 + a1*b1 + m1*p1 (p0, p1 constants)
+__device__ __forceinline__ void mac4(uint64_t& acc, uint32_t& top, uint32_t a0, uint32_t b0, uint32_t m0,
+                                     uint32_t p0, uint32_t a1, uint32_t b1, uint32_t m1, uint32_t p1) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %4, %5, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %6, %7, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %8, %9, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(top)
+        : "v"(a0), "v"(b0), "v"(m0), "s"(p0), "v"(a1), "v"(b1), "v"(m1), "s"(p1)
+        : "vcc");
+}
+
+__device__ __forceinline__ void mac4_new(uint64_t& acc, uint32_t& top, uint32_t a0, uint32_t b0, uint32_t m0,
+                                         uint32_t p0, uint32_t a1, uint32_t b1, uint32_t m1, uint32_t p1) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_cndmask_b32_e64 %1, 0, 1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %4, %5, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %6, %7, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %8, %9, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "=&v"(top)
+        : "v"(a0), "v"(b0), "v"(m0), "s"(p0), "v"(a1), "v"(b1), "v"(m1), "s"(p1)
+        : "vcc");
+}
+
+#define BLS_MAC8(FIRST)                                         \
+    "v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t" FIRST               \
+    "v_mad_u64_u32 %0, vcc, %4, %5, %0\n\t"                     \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %6, %7, %0\n\t"                     \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %8, %9, %0\n\t"                     \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %10, %11, %0\n\t"                   \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %12, %13, %0\n\t"                   \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %14, %15, %0\n\t"                   \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %16, %17, %0\n\t"                   \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+
+// four pairs a_j*b_j + m_j*p_j
+__device__ __forceinline__ void mac8(uint64_t& acc, uint32_t& top, uint32_t a0, uint32_t b0, uint32_t m0,
+                                     uint32_t p0, uint32_t a1, uint32_t b1, uint32_t m1, uint32_t p1,
+                                     uint32_t a2, uint32_t b2, uint32_t m2, uint32_t p2, uint32_t a3,
+                                     uint32_t b3, uint32_t m3, uint32_t p3) {
+    asm(BLS_MAC8("v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t")
+        : "+v"(acc), "+v"(top)
+        : "v"(a0), "v"(b0), "v"(m0), "s"(p0), "v"(a1), "v"(b1), "v"(m1), "s"(p1),
+          "v"(a2), "v"(b2), "v"(m2), "s"(p2), "v"(a3), "v"(b3), "v"(m3), "s"(p3)
+        : "vcc");
+}
+
+__device__ __forceinline__ void mac8_new(uint64_t& acc, uint32_t& top, uint32_t a0, uint32_t b0, uint32_t m0,
+                                         uint32_t p0, uint32_t a1, uint32_t b1, uint32_t m1, uint32_t p1,
+                                         uint32_t a2, uint32_t b2, uint32_t m2, uint32_t p2, uint32_t a3,
+                                         uint32_t b3, uint32_t m3, uint32_t p3) {
+    asm(BLS_MAC8("v_cndmask_b32_e64 %1, 0, 1, vcc\n\t")
+        : "+v"(acc), "=&v"(top)
+        : "v"(a0), "v"(b0), "v"(m0), "s"(p0), "v"(a1), "v"(b1), "v"(m1), "s"(p1),
+          "v"(a2), "v"(b2), "v"(m2), "s"(p2), "v"(a3), "v"(b3), "v"(m3), "s"(p3)
+        : "vcc");
+}
+
+#undef BLS_MAC8
+
+// acc += a0*b0 + m0*p0
+__device__ __forceinline__ void mac2(uint64_t& acc, uint32_t& top, uint32_t a0, uint32_t b0, uint32_t m0,
+                                     uint32_t p0) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %4, %5, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "+v"(top)
+        : "v"(a0), "v"(b0), "v"(m0), "s"(p0)
+        : "vcc");
+}
+
+__device__ __forceinline__ void mac2_new(uint64_t& acc, uint32_t& top, uint32_t a0, uint32_t b0, uint32_t m0,
+                                         uint32_t p0) {
+    asm("v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t"
+        "v_cndmask_b32_e64 %1, 0, 1, vcc\n\t"
+        "v_mad_u64_u32 %0, vcc, %4, %5, %0\n\t"
+        "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+        : "+v"(acc), "=&v"(top)
+        : "v"(a0), "v"(b0), "v"(m0), "s"(p0)
+        : "vcc");
+}
+
+// the column's pairs a_i*b_(k-i) + m_i*p_(k-i) for i in [lo, hi)
+template <int K, int LO, int HI>
+__device__ __forceinline__ void mac_pairs(uint64_t& acc, uint32_t& top, const fp& a, const fp& b,
+                                          const uint32_t m[12]) {
+    constexpr bool opens = LO == 0 || (K >= 12 && LO == K - 11);
+    if constexpr (HI - LO >= 4) {
+        if constexpr (opens)
+            mac8_new(acc, top, a.l[LO], b.l[K - LO], m[LO], p_limb(K - LO), a.l[LO + 1], b.l[K - LO - 1],
+                     m[LO + 1], p_limb(K - LO - 1), a.l[LO + 2], b.l[K - LO - 2], m[LO + 2], p_limb(K - LO - 2),
+                     a.l[LO + 3], b.l[K - LO - 3], m[LO + 3], p_limb(K - LO - 3));
+        else
+            mac8(acc, top, a.l[LO], b.l[K - LO], m[LO], p_limb(K - LO), a.l[LO + 1], b.l[K - LO - 1],
+                 m[LO + 1], p_limb(K - LO - 1), a.l[LO + 2], b.l[K - LO - 2], m[LO + 2], p_limb(K - LO - 2),
+                 a.l[LO + 3], b.l[K - LO - 3], m[LO + 3], p_limb(K - LO - 3));
+        mac_pairs<K, LO + 4, HI>(acc, top, a, b, m);
+    } else if constexpr (HI - LO >= 2) {
+        if constexpr (opens)
+            mac4_new(acc, top, a.l[LO], b.l[K - LO], m[LO], p_limb(K - LO), a.l[LO + 1], b.l[K - LO - 1],
+                     m[LO + 1], p_limb(K - LO - 1));
+        else
+            mac4(acc, top, a.l[LO], b.l[K - LO], m[LO], p_limb(K - LO), a.l[LO + 1], b.l[K - LO - 1],
+                 m[LO + 1], p_limb(K - LO - 1));
+        mac_pairs<K, LO + 2, HI>(acc, top, a, b, m);
+    } else if constexpr (HI - LO == 1) {
+        if constexpr (opens)
+            mac2_new(acc, top, a.l[LO], b.l[K - LO], m[LO], p_limb(K - LO));
+        else
+            mac2(acc, top, a.l[LO], b.l[K - LO], m[LO], p_limb(K - LO));
+    }
+}
+
+// next column: acc = (top : acc) >> 32 (top is restarted by the column's
+// first group, or explicitly when a column opens with a single product)
+__device__ __forceinline__ void next_column(uint64_t& acc, uint32_t top) {
+    acc = (acc >> 32) | ((uint64_t)top << 32);
+}
+
+template <int K>
+__device__ __forceinline__ void mul_low_column(uint64_t& acc, uint32_t& top, const fp& a, const fp& b,
+                                               uint32_t m[12]) {
+    if constexpr (K == 0) top = 0;
+    mac_pairs<K, 0, K>(acc, top, a, b, m);
+    mac1(acc, top, a.l[K], b.l[0]);
+    m[K] = (uint32_t)acc * BLS_N0;
+    mac1_ps(acc, top, m[K], p_limb(0));
+    next_column(acc, top);
+    if constexpr (K + 1 < 12) mul_low_column<K + 1>(acc, top, a, b, m);
+}
+
+template <int K>
+__device__ __forceinline__ void mul_high_column(uint64_t& acc, uint32_t& top, const fp& a, const fp& b,
+                                                const uint32_t m[12], uint32_t t[12]) {
+    mac_pairs<K, K - 11, 12>(acc, top, a, b, m);
+    t[K - 12] = (uint32_t)acc;
+    next_column(acc, top);
+    if constexpr (K + 1 < 23) mul_high_column<K + 1>(acc, top, a, b, m, t);
+}
+
 // Product-scanning (column-wise) Montgomery multiplication with the
 // reduction interleaved per column (the "FIPS" ordering of Koc et al.):
 // column k gathers a_i*b_(k-i) and m_i*p_(k-i) into a 96-bit accumulator,
 // the low word of column k < 12 fixes m_k = lo * (-1/p) mod 2^32, and the
 // columns 12..23 are the result words.  288 multiply-adds, each with one
-// carry fold, and no per-product register shuffling.
+// carry fold, and no per-product register shuffling.  Inputs below 2p give
+// a result below 2p (no final subtraction).
 __device__ __forceinline__ void fp_mul(fp& r, const fp& a, const fp& b) {
     uint32_t m[12];
     uint32_t t[12];
     uint64_t acc = 0;
     uint32_t top = 0;
-#pragma unroll
-    for (int k = 0; k < 12; k++) {
-#pragma unroll
-        for (int i = 0; i < k; i++) {
-            mac3(acc, top, a.l[i], b.l[k - i]);
-            mac3(acc, top, m[i], p_limb(k - i));
-        }
-        mac3(acc, top, a.l[k], b.l[0]);
-        m[k] = (uint32_t)acc * BLS_N0;
-        mac3(acc, top, m[k], p_limb(0));
-        acc = (acc >> 32) | ((uint64_t)top << 32);
-        top = 0;
-    }
-#pragma unroll
-    for (int k = 12; k < 23; k++) {
-#pragma unroll
-        for (int i = k - 11; i < 12; i++) {
-            mac3(acc, top, a.l[i], b.l[k - i]);
-            mac3(acc, top, m[i], p_limb(k - i));
-        }
-        t[k - 12] = (uint32_t)acc;
-        acc = (acc >> 32) | ((uint64_t)top << 32);
-        top = 0;
-    }
+    mul_low_column<0>(acc, top, a, b, m);
+    mul_high_column<12>(acc, top, a, b, m, t);
     t[11] = (uint32_t)acc;   // result < 2p < 2^382: nothing above word 11
-    fp_reduce_once(r, t);
+#pragma unroll
+    for (int i = 0; i < 12; i++) r.l[i] = t[i];
 }
 
 __device__ __forceinline__ void fp_sqr(fp& r, const fp& a) { fp_mul(r, a, a); }

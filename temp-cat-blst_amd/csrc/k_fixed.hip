@@ -65,6 +65,8 @@ k_to_affine(const xyzz* __restrict__ in, aff* __restrict__ out, uint32_t n, uint
             fp_mul(izzz, it, p.zz);
             fp_mul(o.x, p.x, izz);
             fp_mul(o.y, p.y, izzz);
+            fp_canon(o.x, o.x);
+            fp_canon(o.y, o.y);
         }
         out[i] = o;
     }

@@ -20,6 +20,7 @@ k_fold(const job_t* __restrict__ jobs, uint32_t njobs, const xyzz* __restrict__ 
         load_xyzz(w, &W[J.gbase + win]);
         xyzz_add(acc, acc, w);
     }
+    xyzz_canon(acc, acc);
     store_xyzz(&out[j], acc);
 }
 

@@ -100,7 +100,11 @@ k_reduce_tree(const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, ui
         __syncthreads();
     }
     if (last) {
-        if (u == 0) quad_store(&W[g], reinterpret_cast<const fp*>(&sA[0])[ln]);
+        if (u == 0) {
+            fp c;
+            fp_canon(c, reinterpret_cast<const fp*>(&sA[0])[ln]);
+            quad_store(&W[g], c);
+        }
     } else if (u == 0) {
         quad_store(&out[2 * (size_t)(t.z + j)], reinterpret_cast<const fp*>(&sA[0])[ln]);
         quad_store(&out[2 * (size_t)(t.z + j) + 1], reinterpret_cast<const fp*>(&sV[0])[ln]);
