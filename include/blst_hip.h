@@ -161,6 +161,12 @@ int blst_hip_msm_device_tuned(blst_p1 *ret, const void *device_points, const voi
 int blst_hip_tuning_set(int batch, size_t log2n, size_t window, size_t chunk, size_t segment);
 void blst_hip_tuning_get(int batch, size_t npoints, size_t nbits, size_t out[3]);
 
+/* Window groups per lone MSM: the windows are split into `groups` pipelines
+ * on separate streams (sort, accumulation, merges and reduction of one group
+ * overlap those of another).  0 restores the automatic choice (two groups
+ * from 2^17 points).  Returns 0, or -1 for groups > 16. */
+int blst_hip_set_pipeline(size_t groups);
+
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
  * written to out[npoints] (host).  Returns 0 on success. */

@@ -27,8 +27,8 @@ import numpy as np  # noqa: E402
 
 import blst_hip as bh  # noqa: E402
 
-T_CHOICES = (16, 32, 64)
-L_CHOICES = (4, 8, 16)
+T_CHOICES = (16, 32, 64, 128)
+L_CHOICES = (2, 4, 8, 16)
 BATCH_POINTS = 1 << 21
 
 

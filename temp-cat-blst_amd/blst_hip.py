@@ -26,6 +26,7 @@ SIGNATURES = {
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_point_ops_check": ([_vp, _sz], ctypes.c_int),
+    "blst_hip_set_pipeline": ([_sz], ctypes.c_int),
     "blst_hip_last_phase_ms": ([_vp], None),
     "blst_hip_device_upload": ([_vp, _sz], _vp),
     "blst_hip_device_free": ([_vp], None),
@@ -159,6 +160,12 @@ def booth_digits(scalars, nbits, bit0, window):
     if rc:
         raise RuntimeError("blst_hip_booth_digits: " + last_error())
     return out
+
+
+def set_pipeline(groups):
+    """Window-group pipelines per lone MSM (0 = automatic)."""
+    if load().blst_hip_set_pipeline(groups):
+        raise RuntimeError("blst_hip_set_pipeline: " + last_error())
 
 
 def point_ops_check(d_points_ptr, n):

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -35,6 +36,8 @@ struct Ctx {
     hipStream_t stream = nullptr;        // == ws.stream
     std::mutex mu;
     Workspace ws;
+    std::vector<std::unique_ptr<Workspace>> pipe;   // window-group pipelines
+    hipEvent_t ready = nullptr;                     // inputs written on stream
     DevBuf pts, scl, dbg;
     DevBuf gtable, gen_tmp, gen_out;     // comb table of G, fixed-base scratch
     bool gtable_ready = false;
@@ -53,6 +56,7 @@ Ctx& context() {
     c->device = dev;
     c->ws.init();
     c->stream = c->ws.stream;
+    HIPCK(hipEventCreateWithFlags(&c->ready, hipEventDisableTiming));
     Ctx& ref = *c;
     g_ctx.emplace(dev, std::move(c));
     return ref;
@@ -185,9 +189,9 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
         J.nbits = s.nbits;
         J.nbytes = (s.nbits + 7) / 8;
         J.c = s.c;
-        J.nwin = s.tile ? 1u : 1u + s.nbits / s.c;
+        J.nwin = s.tile ? 1u : (s.nwin ? s.nwin : 1u + s.nbits / s.c);
         J.B = 1u << (s.c - 1);
-        J.bit_base = s.bit_base;
+        J.bit_base = s.tile ? s.bit_base : s.win0 * s.c;
         J.vec = (J.nbytes == 32 && ((uintptr_t)s.scalars % 16) == 0) ? 1u : 0u;
         J.kbase = (uint32_t)keys;
         J.ebase = (uint32_t)ent;
@@ -376,7 +380,7 @@ void ensure_workspace(Workspace& cx, const Layout& L) {
     cx.buckets.ensure((size_t)nkeys * sizeof(xyzz));
     cx.slot_key.ensure((size_t)nslots * 4);
     cx.slots.ensure((size_t)nslots * sizeof(xyzz));
-    cx.heavy.ensure(((size_t)nslots / 2 + 64) * 4);
+    cx.heavy.ensure(((size_t)nslots + 64) * 4);
     cx.H.ensure(L.tree_nodes * 2 * sizeof(xyzz));
     cx.W.ensure((size_t)L.groups.size() * sizeof(xyzz));
     cx.res.ensure((size_t)L.jobs.size() * sizeof(xyzz));
@@ -431,7 +435,8 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host
                          cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off, cx.sorted.as<uint2>());
     }
     HIPCK(hipGetLastError());
-    fill_words(cx, cx.buckets.p, (size_t)nkeys * (sizeof(xyzz) / 4), 0);
+    // buckets are not cleared: every bucket with entries is written below,
+    // and the reduction skips the empty ones by their count
     if (timing) HIPCK(hipEventRecord(cx.ev[2], st));
     launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.sorted.as<const uint2>(), off + nkeys, D.jobs,
                         D.kbase, njobs, cx.buckets.as<xyzz>(), cx.slot_key.as<uint32_t>(), cx.slots.as<xyzz>(), T,
@@ -440,19 +445,24 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host
     if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
 
     // buckets spanning chunks: two partials in one lane, longer spans listed
-    // for one workgroup each
+    // for one wave each, the longest for one workgroup each
     uint32_t* heavy = cx.heavy.as<uint32_t>();
-    fill_words(cx, heavy, 1, 0);
+    uint32_t* wave_list = heavy + 64;
+    uint32_t* group_list = wave_list + nchunks;
+    fill_words(cx, heavy, 2, 0);
     launch_k_partial_light(dim3(div_up(nchunks, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
-                           cx.slots.as<const xyzz>(), off, hist, T, nchunks, cx.buckets.as<xyzz>(), heavy, heavy + 64);
-    launch_k_partial_heavy(dim3(std::min<uint32_t>(nchunks, 256)), st, heavy, heavy + 64, cx.slots.as<const xyzz>(),
-                           off, hist, T, cx.buckets.as<xyzz>());
+                           cx.slots.as<const xyzz>(), off, hist, T, nchunks, cx.buckets.as<xyzz>(), heavy, wave_list,
+                           group_list);
+    launch_k_partial_wave(dim3(std::min<uint32_t>(div_up(nchunks, 16), 2048)), st, heavy, wave_list,
+                          cx.slots.as<const xyzz>(), off, hist, T, cx.buckets.as<xyzz>());
+    launch_k_partial_heavy(dim3(std::min<uint32_t>(div_up(nchunks, 512), 256)), st, heavy + 1, group_list,
+                           cx.slots.as<const xyzz>(), off, hist, T, cx.buckets.as<xyzz>());
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[4], st));
 
     xyzz* Hn = cx.H.as<xyzz>();
     launch_k_reduce_segments(dim3(div_up(L.total_segments, 256)), dim3(256), st, D.groups, D.sbase, ngroups,
-                             L.total_segments, cx.buckets.as<const xyzz>(), Hn);
+                             L.total_segments, cx.buckets.as<const xyzz>(), hist, Hn);
     const size_t npass = L.tree_blocks.size();
     for (size_t p = 0; p < npass; p++)
         launch_k_reduce_tree(dim3(L.tree_blocks[p]), st, D.tree_blk + p * ngroups, D.tree_tn + p * ngroups, ngroups,
@@ -462,8 +472,7 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host
     if (host_fold) {
         HIPCK(hipMemcpyAsync(host_out, cx.W.p, (size_t)ngroups * sizeof(xyzz), hipMemcpyDeviceToHost, st));
     } else {
-        launch_k_fold(dim3(div_up(njobs, 64)), dim3(64), st, D.jobs, njobs, cx.W.as<const xyzz>(),
-                      cx.res.as<xyzz>());
+        launch_k_fold(st, D.jobs, njobs, cx.W.as<const xyzz>(), cx.res.as<xyzz>());
         HIPCK(hipGetLastError());
         HIPCK(hipMemcpyAsync(host_out, cx.res.p, (size_t)njobs * sizeof(xyzz), hipMemcpyDeviceToHost, st));
     }
@@ -501,12 +510,63 @@ void finish_single(const JobSpec& js, const std::vector<hostec::xyzz6>& wins, bl
     g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
-// one MSM (or tile) over device-resident inputs
+// Window groups of one MSM run as separate pipelines on their own streams,
+// so the sort of one group overlaps the accumulation of another and the
+// latency-bound merges and reduction of a group overlap the accumulation of
+// the next.  0 = automatic.
+std::atomic<uint32_t> g_pipeline{0};
+
+uint32_t pipeline_groups(const JobSpec& js) {
+    if (js.tile) return 1;
+    const uint32_t nwin = 1 + js.nbits / js.c;
+    uint32_t G = g_pipeline.load();
+    if (G == 0) G = js.n >= (1u << 17) ? 2 : 1;
+    return std::max<uint32_t>(1, std::min<uint32_t>(G, nwin / 2));
+}
+
+// one MSM (or tile) over device-resident inputs written on cx.stream
 void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
-    std::vector<JobSpec> v(1, js);
-    Layout L = make_layout(v);
+    const uint32_t G = pipeline_groups(js);
+    if (G == 1) {
+        std::vector<JobSpec> v(1, js);
+        Layout L = make_layout(v);
+        std::vector<hostec::xyzz6> wins;
+        run_batch(cx.ws, L, true, wins);
+        finish_single(js, wins, ret);
+        return;
+    }
+    const uint32_t nwin = 1 + js.nbits / js.c;
+    while (cx.pipe.size() < G) {
+        cx.pipe.emplace_back(new Workspace());
+        cx.pipe.back()->init();
+    }
+    // layouts, buffers and tables first (allocation may synchronise), then
+    // the pipelines, each behind the inputs
+    std::vector<Layout> Ls(G);
+    std::vector<DevLayout> Ds(G);
+    std::vector<hostec::xyzz6*> outs(G);
+    for (uint32_t g = 0; g < G; g++) {
+        JobSpec s = js;
+        s.win0 = g * nwin / G;
+        s.nwin = (g + 1) * nwin / G - s.win0;
+        Ls[g] = make_layout(std::vector<JobSpec>(1, s));
+        Workspace& w = *cx.pipe[g];
+        ensure_workspace(w, Ls[g]);
+        Ds[g] = upload_layout(w, Ls[g]);
+        outs[g] = static_cast<hostec::xyzz6*>(w.host_buf(Ls[g].groups.size() * sizeof(hostec::xyzz6)));
+    }
+    HIPCK(hipEventRecord(cx.ready, cx.stream));
+    for (uint32_t g = 0; g < G; g++) {
+        Workspace& w = *cx.pipe[g];
+        HIPCK(hipStreamWaitEvent(w.stream, cx.ready, 0));
+        enqueue_batch(w, Ls[g], Ds[g], true, outs[g], g == 0);
+    }
     std::vector<hostec::xyzz6> wins;
-    run_batch(cx.ws, L, true, wins);
+    for (uint32_t g = 0; g < G; g++) {
+        HIPCK(hipStreamSynchronize(cx.pipe[g]->stream));
+        wins.insert(wins.end(), outs[g], outs[g] + Ls[g].groups.size());
+    }
+    record_phases(*cx.pipe[0]);
     finish_single(js, wins, ret);
 }
 
@@ -988,6 +1048,16 @@ void blst_hip_tuning_get(int batch, size_t npoints, size_t nbits, size_t out[3])
     out[0] = t.c;
     out[1] = t.T;
     out[2] = t.L;
+}
+
+int blst_hip_set_pipeline(size_t groups) {
+    if (groups > 16) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    g_pipeline.store((uint32_t)groups);
+    g_err.clear();
+    return 0;
 }
 
 void blst_hip_last_phase_ms(double out[7]) {

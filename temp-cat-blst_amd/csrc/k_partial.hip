@@ -4,32 +4,80 @@
 // the bucket that started before it in slot 2k (partial j = 0 of a bucket
 // whose first chunk is k0 is slot 2*k0+1, partial j >= 1 is slot 2*(k0+j)).
 // Most spanning buckets cross one boundary (two partials): one lane adds
-// them.  Longer spans go to a list and one workgroup per listed bucket sums
-// its partials with four-lane additions, first in strides per quad, then as
-// a tree in LDS.
+// them.  Buckets with up to kWaveSpan partials go to a list worked by one
+// wave per bucket (16 quads, four-lane additions, strides then a shuffle
+// tree); longer ones to a list worked by one workgroup per bucket (256
+// quads, strides then a tree in LDS).  Short top windows put many entries
+// into few buckets, so a batch of MSMs can hold very many listed buckets.
 #include "msm_common.hip.h"
 #include "ec_quad.hip.h"
 #include "launch.h"
 
 namespace bls {
 
+constexpr uint32_t kWaveSpan = 512;
+
+// counts[0]: buckets listed for a wave each, counts[1]: for a workgroup each
 __global__ void __launch_bounds__(256)
 k_partial_light(const uint32_t* __restrict__ slot_key, const xyzz* __restrict__ slots,
                 const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, uint32_t nchunks,
-                xyzz* __restrict__ buckets, uint32_t* __restrict__ heavy_count, uint32_t* __restrict__ heavy) {
+                xyzz* __restrict__ buckets, uint32_t* __restrict__ counts, uint32_t* __restrict__ wave_list,
+                uint32_t* __restrict__ group_list) {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nchunks) return;
     const uint32_t key = slot_key[2 * k + 1];
     if (key == kInvalid) return;
-    const uint32_t k1 = (off[key] + cnt[key] - 1) / T;
-    if (k1 == k + 1) {
+    const uint32_t m = (off[key] + cnt[key] - 1) / T - k + 1;   // partials
+    if (m == 2) {
         xyzz a, b;
         load_xyzz(a, &slots[2 * k + 1]);
         load_xyzz(b, &slots[2 * k + 2]);
         xyzz_add(a, a, b);
         store_xyzz(&buckets[key], a);
+    } else if (m <= kWaveSpan) {
+        wave_list[atomicAdd(&counts[0], 1u)] = key;
     } else {
-        heavy[atomicAdd(heavy_count, 1u)] = key;
+        group_list[atomicAdd(&counts[1], 1u)] = key;
+    }
+}
+
+template <int D>
+__device__ __forceinline__ void fp_shfl_down(fp& r, const fp& a) {
+#pragma unroll
+    for (int k = 0; k < 12; k++) r.l[k] = (uint32_t)__shfl_down((int)a.l[k], D, 64);
+}
+
+// One wave per listed bucket; the grid strides over the list.  Every quad
+// of the wave runs every tree level (only quads below the level's width
+// keep a meaningful value), so no level runs on a sparse wave.
+__global__ void __launch_bounds__(256)
+k_partial_wave(const uint32_t* __restrict__ count, const uint32_t* __restrict__ list,
+               const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt,
+               uint32_t T, xyzz* __restrict__ buckets) {
+    const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const uint32_t nwaves = (gridDim.x * blockDim.x) >> 6;
+    const uint32_t q = (threadIdx.x & 63) >> 2;
+    const uint32_t n = *count;
+    for (uint32_t h = wave; h < n; h += nwaves) {
+        const uint32_t key = list[h];
+        const uint32_t k0 = off[key] / T;
+        const uint32_t m = (off[key] + cnt[key] - 1) / T - k0 + 1;
+        fp acc, o;
+        fp_zero(acc);
+        for (uint32_t j = q; j < m; j += 16) {
+            fp x;
+            quad_load(x, &slots[j == 0 ? 2 * k0 + 1 : 2 * (k0 + j)]);
+            quad_add(acc, acc, x);
+        }
+        fp_shfl_down<32>(o, acc);
+        quad_add(acc, acc, o);
+        fp_shfl_down<16>(o, acc);
+        quad_add(acc, acc, o);
+        fp_shfl_down<8>(o, acc);
+        quad_add(acc, acc, o);
+        fp_shfl_down<4>(o, acc);
+        quad_add(acc, acc, o);
+        if (q == 0) quad_store(&buckets[key], acc);
     }
 }
 
@@ -78,8 +126,12 @@ k_partial_heavy(const uint32_t* __restrict__ heavy_count, const uint32_t* __rest
     }
 }
 
-void launch_k_partial_light(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ slot_key, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, uint32_t nchunks, xyzz* __restrict__ buckets, uint32_t* __restrict__ heavy_count, uint32_t* __restrict__ heavy) {
-    hipLaunchKernelGGL(k_partial_light, grid, block, 0, stream, slot_key, slots, off, cnt, T, nchunks, buckets, heavy_count, heavy);
+void launch_k_partial_light(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ slot_key, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, uint32_t nchunks, xyzz* __restrict__ buckets, uint32_t* __restrict__ counts, uint32_t* __restrict__ wave_list, uint32_t* __restrict__ group_list) {
+    hipLaunchKernelGGL(k_partial_light, grid, block, 0, stream, slot_key, slots, off, cnt, T, nchunks, buckets, counts, wave_list, group_list);
+}
+
+void launch_k_partial_wave(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ count, const uint32_t* __restrict__ list, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, xyzz* __restrict__ buckets) {
+    hipLaunchKernelGGL(k_partial_wave, grid, dim3(256), 0, stream, count, list, slots, off, cnt, T, buckets);
 }
 
 void launch_k_partial_heavy(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ heavy_count, const uint32_t* __restrict__ heavy, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, xyzz* __restrict__ buckets) {

@@ -22,7 +22,7 @@ namespace bls {
 __global__ void __launch_bounds__(256)
 k_reduce_segments(const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase,
                   uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets,
-                  xyzz* __restrict__ nodes) {
+                  const uint32_t* __restrict__ cnt, xyzz* __restrict__ nodes) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total_segments) return;
     const uint32_t g = find_owner(sbase, ngroups, t);
@@ -39,7 +39,9 @@ k_reduce_segments(const group_t* __restrict__ groups, const uint32_t* __restrict
             y = acc;
         } else {
             x = acc;
-            load_xyzz(y, &buckets[base + (G.L - 1 - (step >> 1))]);
+            const size_t key = base + (G.L - 1 - (step >> 1));
+            if (cnt[key]) load_xyzz(y, &buckets[key]);   // empty buckets hold no data
+            else xyzz_set_inf(y);
         }
         xyzz_add(x, x, y);
         if (step & 1) tot = x;
@@ -111,8 +113,8 @@ k_reduce_tree(const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, ui
     }
 }
 
-void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets, xyzz* __restrict__ nodes) {
-    hipLaunchKernelGGL(k_reduce_segments, grid, block, 0, stream, groups, sbase, ngroups, total_segments, buckets, nodes);
+void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets, const uint32_t* __restrict__ cnt, xyzz* __restrict__ nodes) {
+    hipLaunchKernelGGL(k_reduce_segments, grid, block, 0, stream, groups, sbase, ngroups, total_segments, buckets, cnt, nodes);
 }
 
 void launch_k_reduce_tree(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups, const xyzz* __restrict__ in, xyzz* __restrict__ out, xyzz* __restrict__ W, uint32_t last) {

@@ -48,11 +48,24 @@ struct Workspace {
     DevBuf ent, rank, counts, hist, off, sorted, buckets, slot_key, slots, heavy, H, scan_tmp, W, res, tables;
     std::vector<uint8_t> tables_host;
     hipEvent_t ev[8] = {};
+    void* pinned = nullptr;               // page-locked result staging
+    size_t pinned_cap = 0;
     void init() {
         HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto& e : ev) HIPCK(hipEventCreate(&e));
     }
+    void* host_buf(size_t bytes) {
+        if (bytes > pinned_cap) {
+            if (pinned) HIPCK(hipHostFree(pinned));
+            pinned = nullptr;
+            pinned_cap = 0;
+            HIPCK(hipHostMalloc(&pinned, bytes, hipHostMallocDefault));
+            pinned_cap = bytes;
+        }
+        return pinned;
+    }
     ~Workspace() {
+        if (pinned) (void)hipHostFree(pinned);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -66,6 +79,8 @@ struct JobSpec {
     uint32_t T, L;         // tile shape (the batch uses the T of its largest job)
     uint32_t bit_base;     // tile mode: first bit of the single window
     bool tile;             // one window at bit_base, no window fold
+    uint32_t win0 = 0;     // window range [win0, win0 + nwin) of the job
+    uint32_t nwin = 0;     // (0: all windows)
 };
 
 // Tile shape and window of one MSM size: c window bits, T sorted entries

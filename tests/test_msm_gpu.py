@@ -149,6 +149,23 @@ def test_heavy_bucket_long_span(hip, orc, base_points):
     assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
 
 
+def test_window_group_pipelines(hip, orc, base_points):
+    # the windows split over 1..4 concurrent pipelines give one result
+    n = 1 << 17
+    pts = np.ascontiguousarray(np.tile(base_points, (n // base_points.shape[0], 1)))
+    sc = orc.gen_scalars(n, 0x717E)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        for groups in (1, 2, 3, 4, 0):
+            hip.set_pipeline(groups)
+            assert hip.compress(hip.msm_device(dp.ptr, ds.ptr, n, 255)) == want, groups
+    finally:
+        hip.set_pipeline(0)
+        dp.close()
+        ds.close()
+
+
 def test_quad_point_ops_match_one_lane(hip, base_points):
     pts = np.ascontiguousarray(base_points[:1024])
     d = hip.DeviceBuffer(pts)
