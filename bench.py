@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=1 << 20)
     ap.add_argument("--window", type=int, default=0)
+    ap.add_argument("--pipeline", type=int, default=0,
+                    help="window-group pipelines per MSM in the timed steps (0 = automatic)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workloads")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
@@ -155,6 +157,7 @@ def main():
     ds = bh.DeviceBuffer(scalars)
 
     r = None
+    bh.set_pipeline(args.pipeline)
     for _ in range(args.warmup):
         r = bh.msm_device(dp.ptr, ds.ptr, n, 255, args.window)
     t0 = time.perf_counter()
@@ -176,7 +179,7 @@ def main():
         ph = bh.last_phase_ms()
         acc_ms.append(ph["accumulate"])
         phases.append(ph)
-    bh.set_pipeline(0)
+    bh.set_pipeline(args.pipeline)
     if bh.compress(r1) != bh.compress(r):
         raise SystemExit("one-pipeline and window-group results differ")
 
@@ -204,6 +207,7 @@ def main():
             "n": n,
             "nbits": 255,
             "window": args.window or None,
+            "window_group_pipelines": args.pipeline or "auto",
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -15,7 +15,7 @@
 
 namespace bls {
 
-constexpr uint32_t kWaveSpan = 512;
+constexpr uint32_t kWaveSpan = 128;
 
 // counts[0]: buckets listed for a wave each, counts[1]: for a workgroup each
 __global__ void __launch_bounds__(256)
@@ -105,9 +105,12 @@ k_partial_heavy(const uint32_t* __restrict__ heavy_count, const uint32_t* __rest
         }
         reinterpret_cast<fp*>(&node[u])[ln] = acc;
         __syncthreads();
-        // pairwise tree; at least four quads (16 lanes) run each level, the
-        // extra ones repeating a merge without storing it
-        for (uint32_t half = kHeavyQuads / 2; half >= 1; half >>= 1) {
+        // pairwise tree over the quads that hold partials; at least four
+        // quads (16 lanes) run each level, the extra ones repeating a merge
+        // without storing it
+        uint32_t width = 1;
+        while (width < min(m, kHeavyQuads)) width <<= 1;
+        for (uint32_t half = width / 2; half >= 1; half >>= 1) {
             fp r;
             const bool run = u < max(half, 4u);
             if (run) {
