@@ -129,7 +129,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workloads")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
-    ap.add_argument("--serve-rate", type=float, default=1500.0, help="offered request rate (req/s)")
+    ap.add_argument("--serve-rate", type=float, default=2800.0, help="offered request rate (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (HBM counter calibration)")

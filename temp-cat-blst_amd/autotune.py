@@ -13,7 +13,7 @@ lane) and the reduction tile L (buckets per lane), by coordinate descent
 and writes the table the library loads at start-up (tuning.txt next to
 libblst_hip.so, or the path in BLST_HIP_TUNING).
 
-    python3 temp-cat-blst_amd/autotune.py [--out PATH] [--single 6-20] [--batch 6-18]
+    python3 temp-cat-blst_amd/autotune.py [--out PATH] [--single 6-20] [--batch 6-20]
 """
 import argparse
 import os
@@ -118,7 +118,7 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "tuning.txt"))
     ap.add_argument("--single", default="6-20")
-    ap.add_argument("--batch", default="6-18")
+    ap.add_argument("--batch", default="6-20")
     ap.add_argument("--reps", type=int, default=3)
     args = ap.parse_args()
     if not bh.available():

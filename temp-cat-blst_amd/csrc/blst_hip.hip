@@ -215,11 +215,9 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
             S.w = w;
             S.ch = std::max<uint32_t>(2 * J.B, 4096);
             S.nch = std::max<uint32_t>(div_up(J.n, S.ch), 1);
-            S.tb = (uint32_t)tiles;
             S.cb = 0;
             tiles += S.nch;
             L.sgroups.push_back(S);
-            L.tbase.push_back(S.tb);
             L.gkb.push_back(G.kb);
         }
         L.max_B = std::max(L.max_B, J.B);
@@ -247,6 +245,27 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
     if (cw >= 0xffffffffULL) throw HipFail{"batch too large for one launch"};
     L.count_words = cw;
     L.ntiles = (uint32_t)tiles;
+    // tile table by LDS size class: a launch reserves the LDS of its widest
+    // window for every tile, so narrow windows of a mixed batch get their own
+    // launches and share CUs
+    {
+        const uint32_t limits[4] = {1u << 9, 1u << 12, 1u << 14, 0xffffffffu};
+        L.tiles.reserve(tiles);
+        L.tile_begin.push_back(0);
+        for (int cls = 0; cls < 4; cls++) {
+            uint32_t widest = 0;
+            for (size_t g = 0; g < L.sgroups.size(); g++) {
+                const uint32_t B = L.groups[g].B;
+                const uint32_t lo = cls ? limits[cls - 1] : 0;
+                if (B <= lo || B > limits[cls]) continue;
+                widest = std::max(widest, B);
+                for (uint32_t k = 0; k < L.sgroups[g].nch; k++) L.tiles.push_back(make_uint2((uint32_t)g, k));
+            }
+            if ((uint32_t)L.tiles.size() == L.tile_begin.back()) continue;
+            L.tile_begin.push_back((uint32_t)L.tiles.size());
+            L.tile_lds.push_back((size_t)widest * 4);
+        }
+    }
     // reduction tree passes: each block merges up to kTreeNodes nodes of a
     // group into one, until one node per group is left
     {
@@ -291,12 +310,12 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_jobs = 0, o_p = al(o_jobs + nj * sizeof(job_t)), o_e = al(o_p + nj * 4),
                  o_k = al(o_e + nj * 4), o_g = al(o_k + nj * 4), o_s = al(o_g + ng * sizeof(group_t)),
-                 o_sg = al(o_s + ng * 4), o_tb = al(o_sg + ng * sizeof(sgroup_t)), o_gk = al(o_tb + ng * 4),
+                 o_sg = al(o_s + ng * 4), o_tb = al(o_sg + ng * sizeof(sgroup_t)),
+                 o_gk = al(o_tb + L.tiles.size() * sizeof(uint2)),
                  o_trb = al(o_gk + ng * 4), o_trt = al(o_trb + L.tree_blk.size() * 4),
                  total = al(o_trt + L.tree_tn.size() * sizeof(uint4));
     cx.tables.ensure(total);
-    cx.tables_host.resize(total);
-    uint8_t* h = cx.tables_host.data();
+    uint8_t* h = cx.tables_stage(total);
     std::memcpy(h + o_jobs, L.jobs.data(), nj * sizeof(job_t));
     std::memcpy(h + o_p, L.pbase.data(), nj * 4);
     std::memcpy(h + o_e, L.ebase.data(), nj * 4);
@@ -304,7 +323,7 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     std::memcpy(h + o_g, L.groups.data(), ng * sizeof(group_t));
     std::memcpy(h + o_s, L.sbase.data(), ng * 4);
     std::memcpy(h + o_sg, L.sgroups.data(), ng * sizeof(sgroup_t));
-    std::memcpy(h + o_tb, L.tbase.data(), ng * 4);
+    std::memcpy(h + o_tb, L.tiles.data(), L.tiles.size() * sizeof(uint2));
     std::memcpy(h + o_gk, L.gkb.data(), ng * 4);
     std::memcpy(h + o_trb, L.tree_blk.data(), L.tree_blk.size() * 4);
     std::memcpy(h + o_trt, L.tree_tn.data(), L.tree_tn.size() * sizeof(uint4));
@@ -318,7 +337,7 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     D.groups = reinterpret_cast<const group_t*>(d + o_g);
     D.sbase = reinterpret_cast<const uint32_t*>(d + o_s);
     D.sgroups = reinterpret_cast<const sgroup_t*>(d + o_sg);
-    D.tbase = reinterpret_cast<const uint32_t*>(d + o_tb);
+    D.tiles = reinterpret_cast<const uint2*>(d + o_tb);
     D.gkb = reinterpret_cast<const uint32_t*>(d + o_gk);
     D.tree_blk = reinterpret_cast<const uint32_t*>(d + o_trb);
     D.tree_tn = reinterpret_cast<const uint4*>(d + o_trt);
@@ -414,16 +433,18 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host
 
     if (timing) HIPCK(hipEventRecord(cx.ev[0], st));
     if (tiled_sort(L)) {
-        const size_t lds = (size_t)L.max_B * 4;
         uint32_t* counts = cx.counts.as<uint32_t>();
-        launch_k_count(dim3(L.ntiles), dim3(kSortThreads), lds, st, D.jobs, D.sgroups, D.tbase, ngroups, counts);
+        for (size_t c = 0; c < L.tile_lds.size(); c++)
+            launch_k_count(dim3(L.tile_begin[c + 1] - L.tile_begin[c]), dim3(kSortThreads), L.tile_lds[c], st, D.jobs,
+                           D.sgroups, D.tiles + L.tile_begin[c], counts);
         launch_k_prefix(dim3(div_up(nkeys + 1, 256)), dim3(256), st, D.groups, D.sgroups, D.gkb, ngroups, nkeys,
                         counts, hist);
         HIPCK(hipGetLastError());
         if (timing) HIPCK(hipEventRecord(cx.ev[1], st));
         scan_exclusive(cx, hist, off, nkeys + 1, cx.scan_tmp.as<uint32_t>());
-        launch_k_place(dim3(L.ntiles), dim3(kSortThreads), lds, st, D.jobs, D.sgroups, D.tbase, ngroups, counts,
-                       off, cx.sorted.as<uint2>());
+        for (size_t c = 0; c < L.tile_lds.size(); c++)
+            launch_k_place(dim3(L.tile_begin[c + 1] - L.tile_begin[c]), dim3(kSortThreads), L.tile_lds[c], st, D.jobs,
+                           D.sgroups, D.tiles + L.tile_begin[c], counts, off, cx.sorted.as<uint2>());
     } else {
         fill_words(cx, hist, (size_t)nkeys + 1, 0);
         launch_k_digits(dim3(div_up(L.total_points, 256)), dim3(256), st, D.jobs, D.pbase, njobs, L.total_points,

@@ -53,13 +53,15 @@ __global__ void k_digits_debug(const uint8_t* __restrict__ scalars, uint32_t n, 
     out[i] = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
 }
 
-// Tile = chunk k of the points of group g.  LDS holds the B bucket counters
-// of the group's window.
+// Tile = chunk k of the points of group g, from the tile table (one launch
+// per LDS size class).  LDS holds the B bucket counters of the group's
+// window.
 __global__ void __launch_bounds__(kSortThreads)
-k_count(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ tbase,
-        uint32_t ngroups, uint32_t* __restrict__ counts) {
+k_count(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles,
+        uint32_t* __restrict__ counts) {
     extern __shared__ uint32_t bins[];
-    const uint32_t g = find_owner(tbase, ngroups, blockIdx.x);
+    const uint2 tk = tiles[blockIdx.x];
+    const uint32_t g = tk.x, k = tk.y;
     const sgroup_t S = sg[g];
     const job_t& J = jobs[S.job];
     const uint32_t B = J.B, c = J.c, nbits = J.nbits, nbytes = J.nbytes;
@@ -67,7 +69,6 @@ k_count(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const u
     const uint8_t* scalars = J.scalars;
     for (uint32_t b = threadIdx.x; b < B; b += blockDim.x) bins[b] = 0;
     __syncthreads();
-    const uint32_t k = blockIdx.x - S.tb;
     const uint32_t i0 = k * S.ch, i1 = min(i0 + S.ch, J.n);
     const uint32_t bit0 = J.bit_base + S.w * c;
     if (bit0 <= nbits) {
@@ -108,17 +109,16 @@ k_prefix(const group_t* __restrict__ groups, const sgroup_t* __restrict__ sg, co
 // Per tile again: LDS cursors start at the bucket offset plus the chunk's
 // offset inside the bucket and hand out positions.
 __global__ void __launch_bounds__(kSortThreads)
-k_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ tbase,
-        uint32_t ngroups, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off,
-        uint2* __restrict__ sorted) {
+k_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles,
+        const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off, uint2* __restrict__ sorted) {
     extern __shared__ uint32_t cursor[];
-    const uint32_t g = find_owner(tbase, ngroups, blockIdx.x);
+    const uint2 tk = tiles[blockIdx.x];
+    const uint32_t g = tk.x, k = tk.y;
     const sgroup_t S = sg[g];
     const job_t& J = jobs[S.job];
     const uint32_t B = J.B, c = J.c, nbits = J.nbits, nbytes = J.nbytes;
     const bool vec = J.vec;
     const uint8_t* scalars = J.scalars;
-    const uint32_t k = blockIdx.x - S.tb;
     const uint32_t kb = J.kbase + S.w * B;
     const uint32_t* cnt = counts + S.cb + (size_t)k * B;
     for (uint32_t b = threadIdx.x; b < B; b += blockDim.x) cursor[b] = off[kb + b] + cnt[b];
@@ -217,16 +217,16 @@ void launch_k_scatter(dim3 grid, dim3 block, hipStream_t stream, const job_t* __
     hipLaunchKernelGGL(k_scatter, grid, block, 0, stream, jobs, ebase, njobs, total_entries, ent, rank, off, sorted);
 }
 
-void launch_k_count(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ tbase, uint32_t ngroups, uint32_t* __restrict__ counts) {
-    hipLaunchKernelGGL(k_count, grid, block, lds, stream, jobs, sg, tbase, ngroups, counts);
+void launch_k_count(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, uint32_t* __restrict__ counts) {
+    hipLaunchKernelGGL(k_count, grid, block, lds, stream, jobs, sg, tiles, counts);
 }
 
 void launch_k_prefix(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, uint32_t ngroups, uint32_t nkeys, uint32_t* __restrict__ counts, uint32_t* __restrict__ hist) {
     hipLaunchKernelGGL(k_prefix, grid, block, 0, stream, groups, sg, gkb, ngroups, nkeys, counts, hist);
 }
 
-void launch_k_place(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ tbase, uint32_t ngroups, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off, uint2* __restrict__ sorted) {
-    hipLaunchKernelGGL(k_place, grid, block, lds, stream, jobs, sg, tbase, ngroups, counts, off, sorted);
+void launch_k_place(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off, uint2* __restrict__ sorted) {
+    hipLaunchKernelGGL(k_place, grid, block, lds, stream, jobs, sg, tiles, counts, off, sorted);
 }
 
 // largest dynamic LDS the tiled sort may ask for (0 when it cannot be raised

@@ -178,14 +178,13 @@ struct group_t {
 };
 
 // Sort tiles of one group (a window of a job): the job's points are cut into
-// nch chunks of ch points and tile tb + k covers chunk k.  Chunk k's bucket
-// counts live at counts[cb + k * B + bucket].
+// nch chunks of ch points; chunk k's bucket counts live at
+// counts[cb + k * B + bucket].
 struct sgroup_t {
     uint32_t job, w;
-    uint32_t tb;
     uint32_t cb;
     uint32_t nch, ch;
-    uint32_t pad[2];
+    uint32_t pad[3];
 };
 
 constexpr uint32_t kSortThreads = 512;

@@ -28,17 +28,25 @@ struct HipFail {
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    std::vector<void*> retired;   // outgrown allocations, freed with the buffer
     template <class T> T* as() const { return static_cast<T*>(p); }
     void ensure(size_t bytes) {
         if (bytes <= cap && p) return;
-        if (p) HIPCK(hipFree(p));
+        // hipFree waits for the whole device, which would drain work queued
+        // behind this call: an outgrown buffer is kept until destruction
+        // instead (geometric growth bounds what is kept to the current size)
+        const size_t old_cap = p ? cap : 0;
+        if (p) retired.push_back(p);
         p = nullptr;
         cap = 0;
-        const size_t want = std::max<size_t>(bytes + bytes / 8, 4096);
+        const size_t want = std::max<size_t>(std::max<size_t>(bytes + bytes / 8, 2 * old_cap), 4096);
         HIPCK(hipMalloc(&p, want));
         cap = want;
     }
-    ~DevBuf() { if (p) (void)hipFree(p); }
+    ~DevBuf() {
+        for (void* q : retired) (void)hipFree(q);
+        if (p) (void)hipFree(p);
+    }
 };
 
 // Device memory and stream of one in-flight batch.  The per-thread context,
@@ -46,13 +54,35 @@ struct DevBuf {
 struct Workspace {
     hipStream_t stream = nullptr;
     DevBuf ent, rank, counts, hist, off, sorted, buckets, slot_key, slots, heavy, H, scan_tmp, W, res, tables;
-    std::vector<uint8_t> tables_host;
+    uint8_t* tables_pinned = nullptr;     // page-locked staging of the layout tables
+    size_t tables_cap = 0;
     hipEvent_t ev[8] = {};
     void* pinned = nullptr;               // page-locked result staging
     size_t pinned_cap = 0;
-    void init() {
-        HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    bool owns_stream = false;
+    // own stream, or a stream shared with other workspaces (kernels of
+    // different workspaces then run one after another)
+    void init(hipStream_t shared = nullptr) {
+        if (shared) {
+            stream = shared;
+        } else {
+            HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            owns_stream = true;
+        }
         for (auto& e : ev) HIPCK(hipEventCreate(&e));
+    }
+    // staging for the layout tables: page-locked, so their upload is a true
+    // asynchronous copy that does not wait for work queued on the stream
+    uint8_t* tables_stage(size_t bytes) {
+        if (bytes > tables_cap) {
+            if (tables_pinned) HIPCK(hipHostFree(tables_pinned));
+            tables_pinned = nullptr;
+            tables_cap = 0;
+            const size_t want = std::max<size_t>(2 * bytes, 1 << 16);
+            HIPCK(hipHostMalloc((void**)&tables_pinned, want, hipHostMallocDefault));
+            tables_cap = want;
+        }
+        return tables_pinned;
     }
     void* host_buf(size_t bytes) {
         if (bytes > pinned_cap) {
@@ -66,8 +96,9 @@ struct Workspace {
     }
     ~Workspace() {
         if (pinned) (void)hipHostFree(pinned);
+        if (tables_pinned) (void)hipHostFree(tables_pinned);
         for (auto& e : ev) if (e) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
+        if (stream && owns_stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -100,7 +131,12 @@ struct Layout {
     std::vector<group_t> groups;
     std::vector<uint32_t> sbase;
     std::vector<sgroup_t> sgroups;     // sort tiles per group
-    std::vector<uint32_t> tbase, gkb;  // first tile, first key of every group
+    std::vector<uint32_t> gkb;         // first key of every group
+    // sort tiles (group, chunk), ordered by LDS size class; class k covers
+    // tiles [tile_begin[k], tile_begin[k + 1]) and needs tile_lds[k] bytes
+    std::vector<uint2> tiles;
+    std::vector<uint32_t> tile_begin;
+    std::vector<size_t> tile_lds;
     uint32_t total_points = 0, total_entries = 0, nkeys = 0, total_segments = 0, max_n = 0;
     uint32_t ntiles = 0, max_B = 0;
     size_t count_words = 0;
@@ -118,7 +154,8 @@ struct Layout {
 // Device copies of the layout tables (one upload per batch).
 struct DevLayout {
     const job_t* jobs;
-    const uint32_t *pbase, *ebase, *kbase, *sbase, *tbase, *gkb;
+    const uint32_t *pbase, *ebase, *kbase, *sbase, *gkb;
+    const uint2* tiles;
     const group_t* groups;
     const sgroup_t* sgroups;
     const uint32_t* tree_blk;

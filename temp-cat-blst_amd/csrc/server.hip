@@ -4,24 +4,29 @@
 // fresh bases that travel with the request) and wait for tickets.  A
 // scheduler thread
 //   * admits pending requests in arrival order while their device footprint
-//     (inputs plus the batch workspace the request adds) fits the HBM budget,
-//   * copies the inputs of admitted requests on a copy stream,
-//   * whenever one of two batch slots is free, launches every admitted
+//     (inputs plus the batch workspace the request adds) fits the HBM budget
+//     and the points in flight stay within a few batches,
+//   * has a copier thread move the inputs of admitted requests on a copy
+//     stream (device buffers from a slab pool),
+//   * whenever one of three batch slots is free, launches every admitted
 //     request whose copies have landed as one batch (one digit pass, one
-//     sort, one accumulation, one reduction and a device window fold over all
-//     of them), on that slot's own stream, so a batch can start while the
-//     other is still on the device,
-//   * completes the tickets of finished batches and returns their memory.
+//     sort, one accumulation, one reduction over all of them); the slots
+//     queue their batches on one compute stream, so the device runs them
+//     back to back while copies, batch set-up and completions overlap,
+//   * completes finished batches: the window sums come back to the host and
+//     a thread pool folds them per request, then the tickets are released.
 // Requests that arrive while batches run join the next batch as soon as
 // memory and a slot allow.
 #include "pipeline.h"
 #include "../../include/blst_hip.h"
 
 #include <atomic>
+#include <cstdlib>
 #include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -34,27 +39,36 @@ namespace {
 using clock_t_ = std::chrono::steady_clock;
 
 // Device buffers recycled by capacity: hipFree synchronises the device, so
-// the loop never frees while serving.
+// the loop never frees while serving.  Capacities are powers of two; classes
+// below kSlab are carved from kSlab-sized allocations, so a stream of small
+// requests costs one hipMalloc per slab.  Thread-safe.
 class BufferPool {
 public:
+    static constexpr size_t kSlab = (size_t)64 << 20;
     ~BufferPool() {
-        for (auto& kv : free_) (void)hipFree(kv.second);
+        for (void* p : blocks_) (void)hipFree(p);
     }
     void* get(size_t bytes, size_t& cap) {
         cap = round_up(bytes);
+        std::lock_guard<std::mutex> lk(mu_);
         auto it = free_.find(cap);
-        if (it != free_.end()) {
-            void* p = it->second;
-            free_.erase(it);
-            return p;
+        if (it == free_.end()) {
+            const size_t span = std::max(cap, kSlab);
+            void* p = nullptr;
+            HIPCK(hipMalloc(&p, span));
+            blocks_.push_back(p);
+            allocated_ += span;
+            for (size_t o = 0; o + cap <= span; o += cap) free_.emplace(cap, static_cast<uint8_t*>(p) + o);
+            it = free_.find(cap);
         }
-        void* p = nullptr;
-        HIPCK(hipMalloc(&p, cap));
-        allocated_ += cap;
+        void* p = it->second;
+        free_.erase(it);
         return p;
     }
-    void put(void* p, size_t cap) { free_.emplace(cap, p); }
-    size_t allocated() const { return allocated_; }
+    void put(void* p, size_t cap) {
+        std::lock_guard<std::mutex> lk(mu_);
+        free_.emplace(cap, p);
+    }
 
 private:
     static size_t round_up(size_t b) {
@@ -62,8 +76,70 @@ private:
         while (c < b) c <<= 1;
         return c;
     }
+    std::mutex mu_;
     std::multimap<size_t, void*> free_;
+    std::vector<void*> blocks_;
     size_t allocated_ = 0;
+};
+
+// Host threads for the window folds of finished batches (sum_w 2^(cw) W_w
+// per request: a chain of ~255 doublings, cheap on a CPU core and latency
+// bound on the GPU).  run() hands out indices to the pool and the caller.
+class FoldPool {
+public:
+    FoldPool() {
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        const unsigned n = std::max(1u, std::min(8u, hw / 2));
+        for (unsigned i = 0; i + 1 < n; i++) workers_.emplace_back([this] { loop(); });
+    }
+    ~FoldPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    void run(size_t n, std::function<void(size_t)> f) {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            f_ = std::move(f);
+            n_ = n;
+            next_.store(0);
+            pending_ = workers_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    void drain() {
+        for (size_t i; (i = next_.fetch_add(1)) < n_;) f_(i);
+    }
+    void loop() {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        while (true) {
+            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            lk.unlock();
+            drain();
+            lk.lock();
+            if (--pending_ == 0) done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::function<void(size_t)> f_;
+    size_t n_ = 0, pending_ = 0;
+    std::atomic<size_t> next_{0};
+    uint64_t gen_ = 0;
+    bool stop_ = false;
 };
 
 struct Request {
@@ -79,12 +155,18 @@ struct Request {
     size_t d_scalars_cap = 0;
     hipEvent_t copied = nullptr;
     bool copy_issued = false;         // set by the copier thread
+    bool landed = false;              // copies complete (seen by the scheduler)
     clock_t_::time_point t_submit;
     bool done = false;
     int status = 0;
     blst_p1 result = {};
     double latency_ms = 0;
 };
+
+// batches in flight at once: one finishing (results, merges, folds), one
+// accumulating, one whose inputs have landed and that is ready to launch
+constexpr int kSlots = 3;
+constexpr int kQueued = 2;
 
 struct Slot {
     Workspace ws;
@@ -94,6 +176,7 @@ struct Slot {
     hostec::xyzz6* host_res = nullptr;
     size_t host_res_cap = 0;
     hipEvent_t done = nullptr;
+    bool gpu_done = false;            // batch finished on the device, not yet completed
 };
 
 }  // namespace
@@ -106,9 +189,13 @@ public:
         HIPCK(hipMemGetInfo(&free_b, &total_b));
         budget_ = budget ? budget : (size_t)(0.8 * (double)free_b);
         HIPCK(hipStreamCreateWithFlags(&copy_stream_, hipStreamNonBlocking));
+        // one compute stream for all slots: batches run one after another on
+        // the device (concurrent batches measured slower than batches in
+        // turn), while copies, batch set-up and completions overlap them
+        HIPCK(hipStreamCreateWithFlags(&compute_stream_, hipStreamNonBlocking));
         for (auto& s : slots_) {
             s = std::make_unique<Slot>();
-            s->ws.init();
+            s->ws.init(compute_stream_);
             HIPCK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
         }
         worker_ = std::thread([this] { loop(); });
@@ -130,6 +217,7 @@ public:
         }
         for (auto& b : bases_) (void)hipFree(b.ptr);
         (void)hipStreamDestroy(copy_stream_);
+        if (compute_stream_) (void)hipStreamDestroy(compute_stream_);
     }
 
     int add_bases(const blst_p1_affine* pts, size_t n) {
@@ -209,16 +297,20 @@ private:
     // under mu_: admit in arrival order while the budget allows; the copies
     // are issued by the copier thread (a copy from pageable host memory can
     // block the issuing thread for its whole duration)
+    // A request is admitted when its footprint fits the HBM budget and the
+    // points admitted but not finished stay within a few batches: requests
+    // beyond that would only hold input buffers (and make the pool allocate)
+    // long before a slot could take them.
     void admit() {
+        const size_t lookahead = (kSlots + 1) * max_batch_points_;
         while (!pending_.empty()) {
             auto r = pending_.front();
-            if (used_ + r->need > budget_ && (inflight_reqs_ > 0 || !admitted_.empty())) break;
+            const bool idle = inflight_reqs_ == 0 && admitted_.empty();
+            if (!idle && (used_ + r->need > budget_ || open_points_ + r->n > lookahead)) break;
             pending_.pop_front();
             used_ += r->need;
+            open_points_ += r->n;
             peak_used_ = std::max(peak_used_, used_);
-            const size_t sbytes = r->n * ((r->nbits + 7) / 8);
-            r->d_scalars = pool_.get(sbytes + 16, r->d_scalars_cap);
-            if (r->base_id < 0) r->d_points = pool_.get(r->n * sizeof(blst_p1_affine), r->d_points_cap);
             HIPCK(hipEventCreateWithFlags(&r->copied, hipEventDisableTiming));
             admitted_.push_back(r);
             to_copy_.push_back(r);
@@ -240,6 +332,14 @@ private:
             lk.unlock();
             bool ok = true;
             const size_t sbytes = r->n * ((r->nbits + 7) / 8);
+            // device buffers come from the pool here, off the scheduler thread
+            // (a new slab is a hipMalloc)
+            try {
+                r->d_scalars = pool_.get(sbytes + 16, r->d_scalars_cap);
+                if (r->base_id < 0) r->d_points = pool_.get(r->n * sizeof(blst_p1_affine), r->d_points_cap);
+            } catch (const HipFail&) {
+                ok = false;
+            }
             ok = ok && hipMemcpyAsync(r->d_scalars, r->host_scalars, sbytes, hipMemcpyHostToDevice,
                                       copy_stream_) == hipSuccess;
             if (r->base_id < 0)
@@ -253,14 +353,33 @@ private:
         }
     }
 
-    // under mu_: one batch of copied requests into a free slot
-    void launch(Slot& s) {
+    // under mu_: whether a request's inputs are on the device
+    bool landed(Request& r) {
+        if (!r.landed && r.copy_issued && hipEventQuery(r.copied) == hipSuccess) r.landed = true;
+        return r.landed;
+    }
+
+    // under mu_: points of the leading admitted requests that could form a
+    // batch now (arrival order, up to one batch)
+    size_t ready_points() {
+        size_t pts = 0;
+        for (auto& r : admitted_) {
+            if (!landed(*r) || pts >= max_batch_points_) break;
+            pts += r->n;
+        }
+        return pts;
+    }
+
+    // one batch of copied requests into a free slot: chosen under mu_, set
+    // up and enqueued with mu_ released (the copier and submitters keep
+    // going; only this thread touches the slot)
+    void launch(Slot& s, std::unique_lock<std::mutex>& lk) {
         std::vector<std::shared_ptr<Request>> batch;
         std::vector<JobSpec> specs;
         size_t pts = 0;
         for (auto it = admitted_.begin(); it != admitted_.end();) {
             auto& r = *it;
-            if (!r->copy_issued || hipEventQuery(r->copied) != hipSuccess) break;   // keep arrival order
+            if (!landed(*r)) break;   // keep arrival order
             if (!batch.empty() && pts + r->n > max_batch_points_) break;
             specs.push_back(make_spec(r->base_id >= 0 ? bases_[r->base_id].ptr : r->d_points, r->d_scalars, r->n,
                                       r->nbits, 0, true));
@@ -269,33 +388,55 @@ private:
             it = admitted_.erase(it);
         }
         if (batch.empty()) return;
-        s.L = make_layout(specs);
-        ensure_workspace(s.ws, s.L);
-        const DevLayout D = upload_layout(s.ws, s.L);
-        if (s.host_res_cap < batch.size()) {
-            if (s.host_res) HIPCK(hipHostFree(s.host_res));
-            s.host_res_cap = std::max<size_t>(batch.size(), 64);
-            HIPCK(hipHostMalloc((void**)&s.host_res, s.host_res_cap * sizeof(hostec::xyzz6), hipHostMallocDefault));
-        }
-        enqueue_batch(s.ws, s.L, D, false, s.host_res, false);
-        HIPCK(hipEventRecord(s.done, s.ws.stream));
+        const size_t nreq = batch.size();
         s.reqs = std::move(batch);
         s.busy = true;
-        inflight_reqs_ += s.reqs.size();
+        inflight_reqs_ += nreq;
         batches_++;
         int inflight = 0;
         for (auto& o : slots_) inflight += o->busy ? 1 : 0;
         max_inflight_ = std::max<uint64_t>(max_inflight_, inflight);
+        lk.unlock();
+        try {
+            s.L = make_layout(specs);
+            ensure_workspace(s.ws, s.L);
+            const DevLayout D = upload_layout(s.ws, s.L);
+            const size_t ngroups = s.L.groups.size();
+            if (s.host_res_cap < ngroups) {
+                if (s.host_res) HIPCK(hipHostFree(s.host_res));
+                s.host_res = nullptr;
+                s.host_res_cap = 0;
+                HIPCK(hipHostMalloc((void**)&s.host_res, std::max<size_t>(2 * ngroups, 64) * sizeof(hostec::xyzz6),
+                                    hipHostMallocDefault));
+                s.host_res_cap = std::max<size_t>(2 * ngroups, 64);
+            }
+            // window sums come back to the host; the folds run on the fold
+            // pool when the batch completes, so the device moves on
+            enqueue_batch(s.ws, s.L, D, true, s.host_res, false);
+            HIPCK(hipEventRecord(s.done, s.ws.stream));
+        } catch (...) {
+            lk.lock();
+            throw;
+        }
+        lk.lock();
     }
 
-    // under mu_: completion of a finished slot
-    void complete(Slot& s) {
+    // completion of a finished slot: window folds with mu_ released (only
+    // this thread touches the slot), then the tickets under mu_
+    void complete(Slot& s, std::unique_lock<std::mutex>& lk) {
+        lk.unlock();
+        fold_pool_.run(s.reqs.size(), [&s](size_t k) {
+            const job_t& J = s.L.jobs[k];
+            hostec::xyzz6 r;
+            hostec::fold_windows(r, s.host_res + J.gbase, (int)J.nwin, (int)J.c);
+            hostec::jac6 j;
+            hostec::to_jac(j, r);
+            std::memcpy(&s.reqs[k]->result, &j, sizeof(j));
+        });
+        lk.lock();
         const auto now = clock_t_::now();
         for (size_t k = 0; k < s.reqs.size(); k++) {
             auto& r = s.reqs[k];
-            hostec::jac6 j;
-            hostec::to_jac(j, s.host_res[k]);
-            std::memcpy(&r->result, &j, sizeof(j));
             r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
             release(*r);
             r->done = true;
@@ -304,6 +445,7 @@ private:
         inflight_reqs_ -= s.reqs.size();
         s.reqs.clear();
         s.busy = false;
+        s.gpu_done = false;
         done_cv_.notify_all();
     }
 
@@ -314,6 +456,7 @@ private:
         if (r.copied) (void)hipEventDestroy(r.copied);
         r.copied = nullptr;
         used_ -= r.need;
+        open_points_ -= r.n;
     }
 
     void fail_all(const std::string& msg) {
@@ -333,6 +476,7 @@ private:
             for (auto& r : s->reqs) fail(r);
             s->reqs.clear();
             s->busy = false;
+            s->gpu_done = false;
         }
         inflight_reqs_ = 0;
         done_cv_.notify_all();
@@ -345,17 +489,33 @@ private:
             try {
                 if (copy_failed_) throw HipFail{"host to device copy failed"};
                 bool progress = false;
+                int on_device = 0;
                 for (auto& s : slots_) {
-                    if (s->busy && hipEventQuery(s->done) == hipSuccess) {
-                        complete(*s);
+                    if (s->busy && !s->gpu_done && hipEventQuery(s->done) == hipSuccess) s->gpu_done = true;
+                    on_device += (s->busy && !s->gpu_done) ? 1 : 0;
+                }
+                admit();
+                // Batch formation: an idle device takes whatever has landed;
+                // behind a running batch, the next one is queued only once it
+                // holds a quarter of a full batch, so requests arriving under
+                // moderate load gather into batches large enough to amortise
+                // the per-batch stages; never more than kQueued on the device.
+                // Launching comes before completing, so the device has work
+                // while finished batches are folded.
+                for (auto& s : slots_) {
+                    if (s->busy || admitted_.empty() || on_device >= kQueued) continue;
+                    const size_t ready = ready_points();
+                    if (ready == 0 || (on_device > 0 && ready < max_batch_points_ / 4)) break;
+                    launch(*s, lk);
+                    if (s->busy) {
+                        on_device++;
                         progress = true;
                     }
                 }
-                admit();
                 for (auto& s : slots_) {
-                    if (!s->busy && !admitted_.empty()) {
-                        launch(*s);
-                        progress = progress || s->busy;
+                    if (s->busy && s->gpu_done) {
+                        complete(*s, lk);
+                        progress = true;
                     }
                 }
                 const bool busy = inflight_reqs_ > 0 || !admitted_.empty();
@@ -372,8 +532,11 @@ private:
 
     int device_ = 0;
     size_t budget_ = 0, used_ = 0, peak_used_ = 0, max_batch_points_;
+    size_t open_points_ = 0;   // points of admitted requests not yet finished
     hipStream_t copy_stream_ = nullptr;
-    std::unique_ptr<Slot> slots_[2];
+    hipStream_t compute_stream_ = nullptr;   // shared by the slots
+    FoldPool fold_pool_;
+    std::unique_ptr<Slot> slots_[kSlots];
     std::vector<Base> bases_;
     BufferPool pool_;
     std::mutex mu_;

@@ -163,8 +163,11 @@ def _serve(data, trace, burst):
                 return srv.submit(sc, 255, base_id=sid, n=4096)
             return srv.submit(sc, 255, points=data.fresh[lg][fi])
 
-        # warm the server's workspaces and code paths on a few requests
-        for t in [submit(k) for k in range(min(8, len(trace)))]:
+        # warm-up before traffic, as a serving process would: a burst of the
+        # trace's own first requests (about four full batches) sizes every
+        # batch slot's device workspace, which a fresh server allocates on
+        # its first full batch
+        for t in [submit(k) for k in range(min(900, len(trace)))]:
             srv.wait(t)
         tickets = [None] * len(trace)
 
@@ -191,7 +194,7 @@ def _serve(data, trace, burst):
     return secs, res, lat, stats
 
 
-def serving(rate=1500.0, duration=2.0, check_every=97):
+def serving(rate=2800.0, duration=2.0, check_every=97):
     data = _ServingData()
     trace = serving_trace(rate, duration)
     pts_total = sum(1 << lg for _, lg, _, _ in trace)
