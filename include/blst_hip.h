@@ -19,6 +19,9 @@
 
 #include <stddef.h>
 #include <stdint.h>
+#ifndef __cplusplus
+#include <stdbool.h>
+#endif
 
 #ifdef __cplusplus
 extern "C" {
@@ -32,6 +35,7 @@ typedef uint64_t blst_limb_t;
 typedef struct { blst_limb_t l[384 / 8 / sizeof(blst_limb_t)]; } blst_fp;
 typedef struct { blst_fp x, y, z; } blst_p1;
 typedef struct { blst_fp x, y; } blst_p1_affine;
+typedef struct { blst_limb_t l[256 / 8 / sizeof(blst_limb_t)]; } blst_fr;
 #endif
 
 /* ------------------------------------------------------------------ drop-in */
@@ -68,6 +72,34 @@ void blst_p1s_tile_pippenger(blst_p1 *ret, const blst_p1_affine *const points[],
                              size_t npoints, const blst_byte *const scalars[],
                              size_t nbits, blst_limb_t *scratch, size_t bit0,
                              size_t window);
+
+/* Replaces blst_p1s_mult_pippenger_fr_scratch_sizeof (bindings/blst_scalar.h:541,
+ * blst/c_api.cpp:1420-1423, 272-299): 32 bytes per scalar plus, for more than
+ * one point, 192 * 2^(window-1) bucket bytes; 0 for no points or an
+ * unrepresentable size. */
+size_t blst_p1s_mult_pippenger_fr_scratch_sizeof(size_t npoints);
+
+/* Replaces blst_p1s_mult_pippenger_fr (bindings/blst_scalar.h:549,
+ * blst/c_api.cpp:1425-1435, 532-580): contiguous points and reduced
+ * Montgomery-form blst_fr scalars (converted to canonical scalars on the
+ * device).  Returns false, leaving ret unchanged, for a NULL ret, NULL inputs
+ * or scratch, or scratch_size below the queried size (and on a device error);
+ * npoints == 0 writes infinity and returns true.  The scratch is not written. */
+bool blst_p1s_mult_pippenger_fr(blst_p1 *ret, const blst_p1_affine points[], const blst_fr scalars[],
+                                size_t npoints, blst_limb_t *scratch, size_t scratch_size);
+
+/* Replaces blst_p1s_mult_fr_scratch_sizeof (bindings/blst_scalar.h:592,
+ * blst/c_api.cpp:1502-1506, 758-807): a capacity query, the largest of the
+ * Pippenger and batch-affine workspaces the reference's plan may pick for up
+ * to max_points. */
+size_t blst_p1s_mult_fr_scratch_sizeof(size_t max_points);
+
+/* Replaces blst_p1s_mult_fr (bindings/blst_scalar.h:597,
+ * blst/c_api.cpp:1508-1520, 809-851): the same MSM and contract as
+ * blst_p1s_mult_pippenger_fr with this function's scratch size; identity
+ * points are accepted. */
+bool blst_p1s_mult_fr(blst_p1 *ret, const blst_p1_affine points[], const blst_fr scalars[], size_t npoints,
+                      blst_limb_t *scratch, size_t scratch_size);
 
 /* ------------------------------------------------------- device extensions */
 

@@ -16,6 +16,7 @@
 #include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <stdint.h>
 
 typedef uint64_t u64;
 typedef unsigned __int128 u128;
@@ -547,6 +548,137 @@ size_t or_pippenger_scratch_sizeof(size_t npoints) {
 size_t or_pippenger_window_scratch_sizeof(size_t window) {
     if (window < 1 || window > 24) return 0;
     return (size_t)192 << (window - 1);
+}
+
+/* ------------------------------------------------------------ blst_fr */
+
+/* Fr::to_scaler (blst/scalar/fields/Fr.h:445-450) = Fr::from_mont
+ * (Fr.h:634-637, from_mont_256 of blst/backend/backend.h:91-98): the
+ * Montgomery form x of a scalar (R = 2^256 mod r) to k = x * 2^-256 mod r,
+ * as four rounds of limb-wise Montgomery reduction by r and one conditional
+ * subtraction, written out little-endian.  Accepts any 256-bit x: the
+ * reduction leaves t <= r before the subtraction. */
+void or_fr_to_scalar(uint8_t out[32], const u64 fr[4]) {
+    const u64 r0 = 0xfffffffeffffffffULL;   /* -r^-1 mod 2^64 */
+    u64 t[5] = {fr[0], fr[1], fr[2], fr[3], 0};
+    for (int k = 0; k < 4; k++) {
+        const u64 m = t[0] * r0;
+        u128 acc = (u128)m * RORD[0] + t[0];
+        acc >>= 64;
+        for (int j = 1; j < 4; j++) {
+            acc += (u128)m * RORD[j] + t[j];
+            t[j - 1] = (u64)acc;
+            acc >>= 64;
+        }
+        acc += t[4];
+        t[3] = (u64)acc;
+        t[4] = (u64)(acc >> 64);
+    }
+    u64 d[4], borrow = 0;
+    for (int j = 0; j < 4; j++) {
+        const u128 v = (u128)t[j] - RORD[j] - borrow;
+        d[j] = (u64)v;
+        borrow = (u64)(v >> 64) & 1;
+    }
+    const int ge = t[4] != 0 || borrow == 0;
+    for (int j = 0; j < 4; j++) {
+        const u64 v = ge ? d[j] : t[j];
+        for (int b = 0; b < 8; b++) out[8 * j + b] = (uint8_t)(v >> (8 * b));
+    }
+}
+
+/* c_api_scalars_from_fr (blst/c_api.cpp:315-333) over n contiguous values */
+void or_frs_to_scalars(uint8_t *out, const u64 *fr, size_t n) {
+    for (size_t i = 0; i < n; i++) or_fr_to_scalar(out + 32 * i, fr + 4 * i);
+}
+
+/* c_api_pippenger_fr_scratch_sizeof (blst/c_api.cpp:272-299): 32 bytes per
+ * canonical scalar, and for more than one point 2^(window-1) XYZZ buckets of
+ * 192 bytes (the G1PointZZ of Pippenger), 0 on overflow. */
+size_t or_pippenger_fr_scratch_sizeof(size_t npoints) {
+    if (npoints == 0) return 0;
+    if (npoints > SIZE_MAX / 32) return 0;
+    const size_t scalar_bytes = npoints * 32;
+    if (npoints == 1) return scalar_bytes;
+    const size_t window = or_pippenger_window_size(npoints);
+    if (window == 0 || window - 1 >= 64) return 0;
+    const size_t buckets = (size_t)1 << (window - 1);
+    if (buckets > (SIZE_MAX - scalar_bytes) / 192) return 0;
+    return scalar_bytes + buckets * 192;
+}
+
+/* The batch-affine workspace element types for G1
+ * (blst/scalar/curve/multi_scalar_batch_affine.cpp:16-36): Fp is six 64-bit
+ * limbs, Point<Fp> is Jacobian (X, Y, Z). */
+typedef struct { u64 x[6], y[6]; uint32_t generation, queued_epoch; } ba_affine_bucket_t;
+typedef struct { u64 point[18]; uint32_t generation; } ba_projective_bucket_t;
+typedef struct { uint32_t point_index, bucket_index, subtract; } ba_schedule_point_t;
+typedef struct { u64 l[6]; } ba_fp_t;
+
+static size_t ba_align(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
+
+/* batch_affine_scratch_size (multi_scalar_batch_affine.cpp:149-168): the
+ * five arrays in order, each at its element's alignment, the total rounded
+ * to the limb size; window outside [1, 24] gives 0 */
+static size_t batch_affine_scratch_size(size_t window, size_t batch) {
+    if (window < 1 || window > 24) return 0;
+    const size_t B = (size_t)1 << (window - 1);
+    size_t off = 0;
+    off = ba_align(off, _Alignof(ba_affine_bucket_t)) + B * sizeof(ba_affine_bucket_t);
+    off = ba_align(off, _Alignof(ba_projective_bucket_t)) + B * sizeof(ba_projective_bucket_t);
+    off = ba_align(off, _Alignof(ba_schedule_point_t)) + batch * sizeof(ba_schedule_point_t);
+    off = ba_align(off, _Alignof(ba_fp_t)) + batch * sizeof(ba_fp_t);
+    off = ba_align(off, _Alignof(ba_fp_t)) + batch * sizeof(ba_fp_t);
+    return ba_align(off, sizeof(u64));
+}
+
+/* c_api_batch_affine_fr_scratch_sizeof (blst/c_api.cpp:593-627) */
+size_t or_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t batch) {
+    if (npoints == 0 || npoints > 0xffffffffULL) return 0;
+    if (batch != 128 && batch != 384) return 0;
+    const size_t alg = batch_affine_scratch_size(window, batch);
+    if (alg == 0) return 0;
+    if (npoints > SIZE_MAX / 32) return 0;
+    const size_t scalar_bytes = npoints * 32;
+    if (npoints == 1) return scalar_bytes;
+    if (alg > SIZE_MAX - scalar_bytes) return 0;
+    return scalar_bytes + alg;
+}
+
+/* c_api_msm_fr_plan (blst/c_api.cpp:719-755) for G1: batch-affine from 2^12
+ * to 2^18 points, window by size, batch 384 from 2^18 points; 0 = Pippenger */
+static size_t msm_fr_plan(size_t n, size_t *batch) {
+    if (n < ((size_t)1 << 12) || n > ((size_t)1 << 18)) return 0;
+    size_t w;
+    if (n <= ((size_t)1 << 12)) w = 10;
+    else if (n <= ((size_t)1 << 14)) w = 11;
+    else if (n <= ((size_t)1 << 15)) w = 12;
+    else if (n <= ((size_t)1 << 16)) w = 13;
+    else if (n <= ((size_t)1 << 17)) w = 14;
+    else w = 15;
+    *batch = n >= ((size_t)1 << 18) ? 384 : 128;
+    return w;
+}
+
+/* c_api_msm_fr_scratch_sizeof (blst/c_api.cpp:758-807): a capacity query */
+size_t or_msm_fr_scratch_sizeof(size_t max_points) {
+    if (max_points == 0) return 0;
+    const size_t pip = or_pippenger_fr_scratch_sizeof(max_points);
+    if (pip == 0) return 0;
+    size_t required = pip, batch = 0, w = msm_fr_plan(max_points, &batch);
+    if (w) {
+        const size_t b = or_batch_affine_fr_scratch_sizeof(max_points, w, batch);
+        if (b == 0) return 0;
+        if (b > required) required = b;
+    }
+    const size_t measured_max = (size_t)1 << 18;
+    if (max_points > measured_max) {
+        w = msm_fr_plan(measured_max, &batch);
+        const size_t b = or_batch_affine_fr_scratch_sizeof(measured_max, w, batch);
+        if (b == 0) return 0;
+        if (b > required) required = b;
+    }
+    return required;
 }
 
 /* get_wval_limb (ec_mult.h:63-82): up to 25 bits starting at bit |off|,

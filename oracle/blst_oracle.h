@@ -49,6 +49,13 @@ size_t or_pippenger_scratch_sizeof(size_t npoints);
 size_t or_pippenger_window_scratch_sizeof(size_t window);
 uint64_t or_booth_digit(const uint8_t *scalar, size_t nbits, size_t bit0, size_t window);
 
+/* blst_fr scalars and the scratch sizes of the blst_fr MSM entry points */
+void or_fr_to_scalar(uint8_t out[32], const uint64_t fr[4]);
+void or_frs_to_scalars(uint8_t *out, const uint64_t *fr, size_t n);
+size_t or_pippenger_fr_scratch_sizeof(size_t npoints);
+size_t or_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t batch);
+size_t or_msm_fr_scratch_sizeof(size_t max_points);
+
 /* MSM over contiguous arrays (points: n x 12 limbs, scalars: n x ceil(nbits/8)) */
 void or_msm_pippenger(uint64_t ret[18], const uint64_t *points, const uint8_t *scalars,
                       size_t npoints, size_t nbits, size_t window);

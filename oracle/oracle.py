@@ -55,6 +55,8 @@ def lib():
             "or_gen_points": [_u64p, _sz, ctypes.c_uint64, ctypes.c_int],
             "or_gen_srs": [_u64p, _sz, ctypes.c_uint64, ctypes.c_int],
             "or_apply_edge_cases": [_u64p, _u8p, _sz, ctypes.c_uint64],
+            "or_fr_to_scalar": [_u8p, _u64p],
+            "or_frs_to_scalars": [_u8p, _u64p, _sz],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -68,6 +70,9 @@ def lib():
             ("or_pippenger_scratch_sizeof", [_sz], _sz),
             ("or_pippenger_window_scratch_sizeof", [_sz], _sz),
             ("or_booth_digit", [_u8p, _sz, _sz, _sz], ctypes.c_uint64),
+            ("or_pippenger_fr_scratch_sizeof", [_sz], _sz),
+            ("or_batch_affine_fr_scratch_sizeof", [_sz, _sz, _sz], _sz),
+            ("or_msm_fr_scratch_sizeof", [_sz], _sz),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -212,6 +217,38 @@ def msm_naive(points, scalars, n, nbits):
     sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
     lib().or_msm_naive(u64p(r), u64p(pts), u8p(sc), n, nbits)
     return r
+
+
+# ---- blst_fr ---------------------------------------------------------------
+
+R_ORDER = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+
+
+def fr_to_scalars(fr):
+    """(n, 4) uint64 Montgomery-form blst_fr -> (n, 32) uint8 canonical scalars."""
+    fr = np.ascontiguousarray(fr, dtype=np.uint64).reshape(-1, 4)
+    out = np.zeros((fr.shape[0], 32), np.uint8)
+    if fr.shape[0]:
+        lib().or_frs_to_scalars(u8p(out.reshape(-1)), u64p(fr.reshape(-1)), fr.shape[0])
+    return out
+
+
+def fr_from_values(vals):
+    """Python ints below r -> (n, 4) uint64 Montgomery form (x * 2^256 mod r)."""
+    return np.stack([limbs((int(v) << 256) % R_ORDER, 4) for v in vals]) if len(vals) else \
+        np.zeros((0, 4), np.uint64)
+
+
+def pippenger_fr_scratch_sizeof(n):
+    return lib().or_pippenger_fr_scratch_sizeof(n)
+
+
+def batch_affine_fr_scratch_sizeof(n, window, batch):
+    return lib().or_batch_affine_fr_scratch_sizeof(n, window, batch)
+
+
+def msm_fr_scratch_sizeof(n):
+    return lib().or_msm_fr_scratch_sizeof(n)
 
 
 # ---- synthetic inputs ------------------------------------------------------

@@ -21,6 +21,10 @@ SIGNATURES = {
     "blst_p1s_mult_pippenger": ([_vp, _vp, _sz, _vp, _sz, _vp], None),
     "blst_p1s_mult_pippenger_window": ([_vp, _vp, _sz, _vp, _sz, _vp, _sz], None),
     "blst_p1s_tile_pippenger": ([_vp, _vp, _sz, _vp, _sz, _vp, _sz, _sz], None),
+    "blst_p1s_mult_pippenger_fr_scratch_sizeof": ([_sz], _sz),
+    "blst_p1s_mult_pippenger_fr": ([_vp, _vp, _vp, _sz, _vp, _sz], ctypes.c_bool),
+    "blst_p1s_mult_fr_scratch_sizeof": ([_sz], _sz),
+    "blst_p1s_mult_fr": ([_vp, _vp, _vp, _sz, _vp, _sz], ctypes.c_bool),
     "blst_hip_last_error": ([], ctypes.c_char_p),
     "blst_hip_available": ([], ctypes.c_int),
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
@@ -137,6 +141,44 @@ def p1s_mult_pippenger(points, scalars, nbits, window=0, scratch=True, mode="con
     if err and n > 0:
         raise RuntimeError("blst_p1s_mult_pippenger: " + err)
     return ret
+
+
+def fr_scratch_sizeof(n, entry="pippenger"):
+    """Scratch bytes of blst_p1s_mult_pippenger_fr (entry "pippenger") or of
+    blst_p1s_mult_fr (entry "msm")."""
+    L = load()
+    if entry == "pippenger":
+        return L.blst_p1s_mult_pippenger_fr_scratch_sizeof(n)
+    if entry == "msm":
+        return L.blst_p1s_mult_fr_scratch_sizeof(n)
+    raise ValueError(entry)
+
+
+def p1s_mult_fr(points, fr, entry="pippenger", scratch_size=None, ret=None, null=()):
+    """G1 MSM of contiguous points and Montgomery-form Fr scalars (n x 4
+    uint64 limbs) through blst_p1s_mult_pippenger_fr / blst_p1s_mult_fr.
+
+    scratch_size defaults to the queried size; null names arguments passed as
+    NULL ("ret", "points", "scalars", "scratch").  Returns (ok, ret): ret is
+    the 18-limb blst_p1 buffer, untouched by a call that returns False."""
+    L = load()
+    points = np.ascontiguousarray(points, dtype=np.uint64)
+    fr = np.ascontiguousarray(fr, dtype=np.uint64)
+    n = points.shape[0]
+    assert fr.shape[0] == n
+    if ret is None:
+        ret = np.zeros(18, np.uint64)
+    need = fr_scratch_sizeof(n, entry)
+    if scratch_size is None:
+        scratch_size = need
+    scr = np.zeros(max((scratch_size + 7) // 8, 1), np.uint64)
+    args = {"ret": _ptr(ret), "points": _ptr(points) if n else _ptr(np.zeros(12, np.uint64)),
+            "scalars": _ptr(fr) if n else _ptr(np.zeros(4, np.uint64)), "scratch": _ptr(scr)}
+    for k in null:
+        args[k] = None
+    f = L.blst_p1s_mult_pippenger_fr if entry == "pippenger" else L.blst_p1s_mult_fr
+    ok = f(args["ret"], args["points"], args["scalars"], n, args["scratch"], scratch_size)
+    return bool(ok), ret
 
 
 def p1s_tile_pippenger(points, scalars, nbits, bit0, window):

@@ -733,6 +733,147 @@ struct blst_hip_batch_plan {
     PlanImpl impl;
 };
 
+// ---------------------------------------------------------------- Fr scalars
+//
+// Scratch sizes of the reference's blst_fr entry points, which callers
+// allocate from: 32 bytes per canonical scalar plus the Pippenger buckets
+// (blst/c_api.cpp:272-299), and for blst_p1s_mult_fr the largest of that and
+// the batch-affine plan's workspace (c_api.cpp:593-627, 705-807,
+// blst/scalar/curve/multi_scalar_batch_affine.cpp:16-36, 149-168).  Overflow
+// yields 0.
+
+namespace {
+
+size_t pippenger_fr_scratch(size_t n) {
+    if (n == 0) return 0;
+    size_t scalar_bytes;
+    if (__builtin_mul_overflow(n, (size_t)32, &scalar_bytes)) return 0;
+    if (n == 1) return scalar_bytes;
+    const size_t w = ref_window_size(n);
+    if (w == 0 || w - 1 >= 8 * sizeof(size_t)) return 0;
+    size_t buckets, total;
+    if (__builtin_mul_overflow((size_t)1 << (w - 1), (size_t)192, &buckets) ||
+        __builtin_add_overflow(scalar_bytes, buckets, &total))
+        return 0;
+    return total;
+}
+
+// batch-affine workspace for G1: affine buckets (x, y, two u32: 104 B),
+// projective collision buckets (Jacobian point + u32: 152 B), the schedule
+// (3 x u32 per entry) and two Fp arrays of the batch size, each array aligned
+// to its element
+size_t batch_affine_algorithm_scratch(size_t window, size_t batch) {
+    if (window < 1 || window > 24) return 0;
+    auto align = [](size_t v, size_t a) { return (v + a - 1) & ~(a - 1); };
+    const size_t B = (size_t)1 << (window - 1);
+    size_t off = B * 104;
+    off = align(off, 8) + B * 152;
+    off = align(off, 4) + batch * 12;
+    off = align(off, 8) + batch * 48;
+    off = align(off, 8) + batch * 48;
+    return align(off, 8);
+}
+
+size_t batch_affine_fr_scratch(size_t n, size_t window, size_t batch) {
+    if (n == 0 || n > 0xffffffffULL) return 0;
+    if (batch != 128 && batch != 384) return 0;
+    const size_t alg = batch_affine_algorithm_scratch(window, batch);
+    if (alg == 0) return 0;
+    size_t scalar_bytes, total;
+    if (__builtin_mul_overflow(n, (size_t)32, &scalar_bytes)) return 0;
+    if (n == 1) return scalar_bytes;
+    if (__builtin_add_overflow(scalar_bytes, alg, &total)) return 0;
+    return total;
+}
+
+// the reference's algorithm choice for blst_p1s_mult_fr: batch-affine from
+// 2^12 to 2^18 points (window by size, batch 384 from 2^18), else Pippenger
+bool msm_fr_batch_plan(size_t n, size_t& window, size_t& batch) {
+    if (n < ((size_t)1 << 12) || n > ((size_t)1 << 18)) return false;
+    if (n <= ((size_t)1 << 12)) window = 10;
+    else if (n <= ((size_t)1 << 14)) window = 11;
+    else if (n <= ((size_t)1 << 15)) window = 12;
+    else if (n <= ((size_t)1 << 16)) window = 13;
+    else if (n <= ((size_t)1 << 17)) window = 14;
+    else window = 15;
+    batch = n >= ((size_t)1 << 18) ? 384 : 128;
+    return true;
+}
+
+size_t msm_fr_scratch(size_t max_points) {
+    if (max_points == 0) return 0;
+    size_t required = pippenger_fr_scratch(max_points);
+    if (required == 0) return 0;
+    size_t window, batch;
+    if (msm_fr_batch_plan(max_points, window, batch)) {
+        const size_t b = batch_affine_fr_scratch(max_points, window, batch);
+        if (b == 0) return 0;
+        required = std::max(required, b);
+    }
+    // a capacity query: storage for max_points also runs every smaller
+    // input, including the largest batch-affine plan
+    const size_t measured_max = (size_t)1 << 18;
+    if (max_points > measured_max) {
+        msm_fr_batch_plan(measured_max, window, batch);
+        const size_t b = batch_affine_fr_scratch(measured_max, window, batch);
+        if (b == 0) return 0;
+        required = std::max(required, b);
+    }
+    return required;
+}
+
+// the device MSM of contiguous points and Montgomery-form Fr scalars; ret is
+// written only on success
+bool fr_msm(blst_p1* ret, const blst_p1_affine points[], const void* fr_scalars, size_t n) {
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        cx.pts.ensure(n * sizeof(blst_p1_affine));
+        cx.scl.ensure(n * 32 + 16);
+        HIPCK(hipMemcpyAsync(cx.pts.p, points, n * sizeof(blst_p1_affine), hipMemcpyHostToDevice, cx.stream));
+        HIPCK(hipMemcpyAsync(cx.scl.p, fr_scalars, n * 32, hipMemcpyHostToDevice, cx.stream));
+        launch_k_fr_to_scalar(cx.stream, cx.scl.as<uint32_t>(), (uint32_t)n);
+        HIPCK(hipGetLastError());
+        const JobSpec js = make_spec(cx.pts.p, cx.scl.p, n, 255, 0, false);
+        blst_p1 r;
+        device_msm(cx, js, &r);
+        *ret = r;
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return false;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return false;
+    }
+    g_err.clear();
+    return true;
+}
+
+// checks of c_api_pippenger_fr / c_api_msm_fr (c_api.cpp:532-552, 809-828)
+bool fr_entry(blst_p1* ret, const blst_p1_affine points[], const void* scalars, size_t n, blst_limb_t* scratch,
+              size_t scratch_size, size_t required) {
+    if (ret == nullptr) {
+        g_err = "ret is NULL";
+        return false;
+    }
+    if (n == 0) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err.clear();
+        return true;
+    }
+    if (required == 0 || points == nullptr || scalars == nullptr || scratch == nullptr || scratch_size < required) {
+        g_err = "invalid arguments";
+        return false;
+    }
+    if (n >= 0x7fffffffULL) {   // point indices are 31-bit on the device
+        g_err = "npoints beyond 2^31 - 2";
+        return false;
+    }
+    return fr_msm(ret, points, scalars, n);
+}
+
+}  // namespace
+
 extern "C" {
 
 size_t blst_p1s_mult_pippenger_scratch_sizeof(size_t npoints) {
@@ -743,6 +884,20 @@ size_t blst_p1s_mult_pippenger_scratch_sizeof(size_t npoints) {
 size_t blst_p1s_mult_pippenger_window_scratch_sizeof(size_t window) {
     if (window < 1 || window > 24) return 0;
     return (size_t)192 << (window - 1);
+}
+
+size_t blst_p1s_mult_pippenger_fr_scratch_sizeof(size_t npoints) { return pippenger_fr_scratch(npoints); }
+
+bool blst_p1s_mult_pippenger_fr(blst_p1* ret, const blst_p1_affine points[], const blst_fr scalars[], size_t npoints,
+                                blst_limb_t* scratch, size_t scratch_size) {
+    return fr_entry(ret, points, scalars, npoints, scratch, scratch_size, pippenger_fr_scratch(npoints));
+}
+
+size_t blst_p1s_mult_fr_scratch_sizeof(size_t max_points) { return msm_fr_scratch(max_points); }
+
+bool blst_p1s_mult_fr(blst_p1* ret, const blst_p1_affine points[], const blst_fr scalars[], size_t npoints,
+                      blst_limb_t* scratch, size_t scratch_size) {
+    return fr_entry(ret, points, scalars, npoints, scratch, scratch_size, msm_fr_scratch(npoints));
 }
 
 void blst_p1s_mult_pippenger(blst_p1* ret, const blst_p1_affine* const points[], size_t npoints,

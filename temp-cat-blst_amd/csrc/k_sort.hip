@@ -137,6 +137,45 @@ k_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const u
     }
 }
 
+// Montgomery-form Fr scalars (blst_fr: x = k * 2^256 mod r, four 64-bit
+// little-endian limbs) to canonical 32-byte little-endian scalars k =
+// x * 2^-256 mod r, in place: eight rounds of word-wise Montgomery reduction
+// and one conditional subtraction (Fr::to_scaler, blst/scalar/fields/Fr.h:445).
+__global__ void __launch_bounds__(256)
+k_fr_to_scalar(uint32_t* __restrict__ io, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    constexpr uint32_t R[8] = {0x00000001u, 0xffffffffu, 0xfffe5bfeu, 0x53bda402u,
+                               0x09a1d805u, 0x3339d808u, 0x299d7d48u, 0x73eda753u};
+    constexpr uint32_t N0 = 0xffffffffu;   // -r^-1 mod 2^32
+    uint32_t* w = io + 8 * (size_t)i;
+    uint32_t t[9];
+#pragma unroll
+    for (int j = 0; j < 8; j++) t[j] = w[j];
+    t[8] = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint32_t m = t[0] * N0;
+        uint64_t c = ((uint64_t)m * R[0] + t[0]) >> 32;
+#pragma unroll
+        for (int j = 1; j < 8; j++) {
+            c += (uint64_t)m * R[j] + t[j];
+            t[j - 1] = (uint32_t)c;
+            c >>= 32;
+        }
+        c += t[8];
+        t[7] = (uint32_t)c;
+        t[8] = (uint32_t)(c >> 32);
+    }
+    // t <= r here (the input is below 2^256): subtract r once if t >= r
+    uint32_t d[8], borrow = 0;
+#pragma unroll
+    for (int j = 0; j < 8; j++) d[j] = __builtin_subc(t[j], R[j], borrow, &borrow);
+    const bool ge = t[8] || !borrow;
+#pragma unroll
+    for (int j = 0; j < 8; j++) w[j] = ge ? d[j] : t[j];
+}
+
 // dst[0..words) = value; a kernel rather than a memset node so the sequence
 // can be captured in a HIP graph
 __global__ void __launch_bounds__(256)
@@ -241,6 +280,10 @@ size_t sort_lds_limit() {
         return 0;
     }
     return (size_t)v;
+}
+
+void launch_k_fr_to_scalar(hipStream_t stream, uint32_t* __restrict__ io, uint32_t n) {
+    hipLaunchKernelGGL(k_fr_to_scalar, dim3((n + 255) / 256), dim3(256), 0, stream, io, n);
 }
 
 }  // namespace bls

@@ -16,6 +16,7 @@ void launch_k_count(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const
 void launch_k_prefix(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, uint32_t ngroups, uint32_t nkeys, uint32_t* __restrict__ counts, uint32_t* __restrict__ hist);
 void launch_k_place(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off, uint2* __restrict__ sorted);
 size_t sort_lds_limit();
+void launch_k_fr_to_scalar(hipStream_t stream, uint32_t* __restrict__ io, uint32_t n);
 void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks);
 void launch_k_partial_light(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ slot_key, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, uint32_t nchunks, xyzz* __restrict__ buckets, uint32_t* __restrict__ counts, uint32_t* __restrict__ wave_list, uint32_t* __restrict__ group_list);
 void launch_k_partial_heavy(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ heavy_count, const uint32_t* __restrict__ heavy, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, xyzz* __restrict__ buckets);

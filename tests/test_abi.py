@@ -114,3 +114,36 @@ def test_fail_closed_without_gpu_work():
     L.blst_p1s_tile_pippenger(blst_hip._ptr(ret), blst_hip._ptr(d.pp), 2, blst_hip._ptr(d.ss), 255,
                               blst_hip._ptr(scr), 255, 8)
     assert not ret.any()
+
+
+def test_fr_scratch_sizes_match_oracle(orc):
+    import blst_hip
+    for n in [0, 1, 2, 3, 5, 64, 4095, 4096, 4097, 16384, 16385, 1 << 17, (1 << 18) - 1, 1 << 18,
+              (1 << 18) + 1, 1 << 20, (1 << 32) + 1, 1 << 58]:
+        assert blst_hip.fr_scratch_sizeof(n, "pippenger") == orc.pippenger_fr_scratch_sizeof(n), n
+        assert blst_hip.fr_scratch_sizeof(n, "msm") == orc.msm_fr_scratch_sizeof(n), n
+    # the full capacity range of the reference's monotonicity test
+    L = blst_hip.load()
+    prev = 0
+    for n in range(1, (1 << 18) + 2):
+        cur = L.blst_p1s_mult_fr_scratch_sizeof(n)
+        assert cur >= prev, n
+        prev = cur
+
+
+def test_fr_fail_closed_without_gpu_work():
+    """Argument checks of c_api_pippenger_fr / c_api_msm_fr
+    (c_api.cpp:532-552, 809-828) that return before any device call: false
+    with ret untouched, or true with infinity for no points."""
+    import blst_hip
+    pts = np.ones((4, 12), np.uint64)
+    fr = np.ones((4, 4), np.uint64)
+    for entry in ("pippenger", "msm"):
+        for kw in ({"null": ("ret",)}, {"null": ("points",)}, {"null": ("scalars",)}, {"null": ("scratch",)},
+                   {"scratch_size": blst_hip.fr_scratch_sizeof(4, entry) - 1}, {"scratch_size": 0}):
+            ret = np.full(18, 7, np.uint64)
+            ok, out = blst_hip.p1s_mult_fr(pts, fr, entry, ret=ret, **kw)
+            assert not ok and (out == 7).all(), (entry, kw)
+        ret = np.full(18, 7, np.uint64)
+        ok, out = blst_hip.p1s_mult_fr(pts[:0], fr[:0], entry, ret=ret, null=("points", "scalars", "scratch"))
+        assert ok and not out.any()

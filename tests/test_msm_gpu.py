@@ -364,3 +364,84 @@ def test_serving_loop_results_match_alone(hip, orc, base_points):
         assert st["batches"] < 40                      # requests were batched together
     finally:
         srv.close()
+
+
+# ---------------------------------------------------------------- blst_fr entry points
+
+FR_ENTRIES = ["pippenger", "msm"]
+
+
+@pytest.mark.parametrize("entry", FR_ENTRIES)
+def test_fr_known_answer_ten_g(hip, orc, entry):
+    """Scalars 1..4 in Montgomery form on four copies of G give [10]G, and a
+    scratch one byte short is refused with ret untouched
+    (BindingHeader_tests.cpp:642-690)."""
+    g = orc.generator_affine()
+    pts = np.ascontiguousarray(np.tile(g, (4, 1)))
+    fr = orc.fr_from_values([1, 2, 3, 4])
+    ok, ret = hip.p1s_mult_fr(pts, fr, entry)
+    assert ok
+    assert hip.compress(ret) == pyref.compress(pyref.mul(10, pyref.G))
+    short = np.full(18, 7, np.uint64)
+    ok, out = hip.p1s_mult_fr(pts, fr, entry, scratch_size=hip.fr_scratch_sizeof(4, entry) - 1, ret=short)
+    assert not ok and (out == 7).all()
+
+
+@pytest.mark.parametrize("entry", FR_ENTRIES)
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 4096, 6000])
+def test_fr_random_matches_oracle(hip, orc, entry, n):
+    """Random Montgomery-form scalars with edge values (0, one, r - 1 and
+    unreduced 256-bit words) against the oracle's conversion and MSM."""
+    rnd = random.Random(n * 7 + len(entry))
+    pts = orc.gen_points(n, 0xF00 + n, threads=8)
+    vals = [rnd.randrange(R) for _ in range(n)]
+    for k, v in enumerate([0, 1, R - 1, 2]):
+        if k < n:
+            vals[k] = v
+    fr = orc.fr_from_values(vals)
+    if n > 8:
+        fr[5] = np.uint64(0xFFFFFFFFFFFFFFFF)       # 2^256 - 1, above r
+        fr[6] = orc.limbs(R, 4)                       # r itself
+        fr[7] = orc.limbs(R + 12345, 4)
+    sc = orc.fr_to_scalars(fr)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    ok, ret = hip.p1s_mult_fr(pts, fr, entry)
+    assert ok and hip.compress(ret) == want
+
+
+def test_fr_identity_point_and_capacity_scratch(hip, orc):
+    """An identity point among 2^12 (which sends the reference's selected
+    algorithm back to Pippenger) and a capacity-sized scratch for a smaller
+    input, as BindingHeader_tests.cpp:1205-1242 does."""
+    cap, n = 1 << 12, 1 << 11
+    pts = orc.gen_points(cap, 0xCAFE, threads=8)
+    rnd = random.Random(99)
+    fr = orc.fr_from_values([rnd.randrange(R) for _ in range(cap)])
+    big = hip.fr_scratch_sizeof(cap, "msm")
+    ok_p, want_small = hip.p1s_mult_fr(pts[:n], fr[:n], "pippenger")
+    ok_m, got_small = hip.p1s_mult_fr(pts[:n], fr[:n], "msm", scratch_size=big)
+    assert ok_p and ok_m and hip.compress(got_small) == hip.compress(want_small)
+    assert hip.compress(want_small) == orc.compress(orc.msm_pippenger(pts[:n], orc.fr_to_scalars(fr[:n]), n, 255))
+    pts[17] = 0
+    ok, got = hip.p1s_mult_fr(pts, fr, "msm", scratch_size=big)
+    assert ok
+    assert hip.compress(got) == orc.compress(orc.msm_pippenger(pts, orc.fr_to_scalars(fr), cap, 255))
+    short = np.full(18, 7, np.uint64)
+    ok, out = hip.p1s_mult_fr(pts, fr, "msm", scratch_size=big - 1, ret=short)
+    assert not ok and (out == 7).all()
+
+
+def test_fr_2pow20_equals_canonical_scalars(hip, orc):
+    """At 2^20 points the blst_fr entry equals the byte-scalar entry on the
+    oracle-converted scalars (the byte path is pinned at this size against
+    the oracle above)."""
+    n = 1 << 20
+    pts = hip.synth_points(n, 0x51)
+    rnd = np.random.default_rng(3)
+    fr = rnd.integers(0, 1 << 63, size=(n, 4), dtype=np.uint64)
+    fr[:, 3] &= np.uint64(0x0FFFFFFFFFFFFFFF)      # most values below r, a few above
+    fr[::4096, 3] |= np.uint64(0xF000000000000000)
+    sc = orc.fr_to_scalars(fr)
+    ok, got = hip.p1s_mult_fr(pts, fr, "pippenger")
+    assert ok
+    assert hip.compress(got) == hip.compress(hip.p1s_mult_pippenger(pts, sc, 255))

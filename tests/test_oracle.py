@@ -250,3 +250,79 @@ def test_edge_case_generator(orc):
     changed = sum(not np.array_equal(a, b) for a, b in zip(pts, p0))
     assert 1 <= changed <= 5
     assert sum(not np.array_equal(a, b) for a, b in zip(sc, s0)) >= 25
+
+
+def test_fr_to_scalar_against_bigint(orc):
+    """Fr::to_scaler is x * 2^-256 mod r for the Montgomery form x; any
+    256-bit x reduces to its canonical class (blst_fr inputs above r)."""
+    r = pyref.R_ORDER
+    rinv = pow(2, -256, r)
+    rnd = random.Random(31)
+    xs = [0, 1, 2, r - 1, r, r + 1, (1 << 256) - 1, (1 << 256) % r, (1 << 255)]
+    xs += [rnd.getrandbits(256) for _ in range(200)] + [rnd.randrange(r) for _ in range(200)]
+    fr = np.stack([orc.limbs(x, 4) for x in xs])
+    got = orc.fr_to_scalars(fr)
+    for x, g in zip(xs, got):
+        assert int.from_bytes(g.tobytes(), "little") == x * rinv % r, hex(x)
+    # values -> Montgomery form -> values (blst_fr_from_uint64 then to_scaler)
+    vals = [0, 1, 10, r - 1] + [rnd.randrange(r) for _ in range(50)]
+    back = orc.fr_to_scalars(orc.fr_from_values(vals))
+    assert [int.from_bytes(b.tobytes(), "little") for b in back] == vals
+
+
+def test_fr_scratch_sizes(orc):
+    """c_api.cpp:272-299, 593-627, 758-807, pinned by the reference's own
+    binding tests (BindingHeader_tests.cpp:652-653, 1190-1203), by the
+    Pippenger bucket size already pinned above and by the batch-affine
+    element sizes that C's layout rules give the structs of
+    multi_scalar_batch_affine.cpp:16-36 (computed here by ctypes)."""
+    import ctypes
+    assert orc.pippenger_fr_scratch_sizeof(0) == 0
+    assert orc.pippenger_fr_scratch_sizeof(1) == 32
+    assert orc.msm_fr_scratch_sizeof(0) == 0
+    assert orc.msm_fr_scratch_sizeof(1) == 32
+    # Pippenger: 32-byte canonical scalars, then the plain entry's buckets
+    for n in (2, 3, 5, 64, 100, 4095, 4096, 4097, 1 << 16, 1 << 20):
+        assert orc.pippenger_fr_scratch_sizeof(n) == 32 * n + orc.scratch_sizeof(n), n
+
+    class AffBucket(ctypes.Structure):
+        _fields_ = [("x", ctypes.c_uint64 * 6), ("y", ctypes.c_uint64 * 6),
+                    ("generation", ctypes.c_uint32), ("queued_epoch", ctypes.c_uint32)]
+
+    class ProjBucket(ctypes.Structure):
+        _fields_ = [("point", ctypes.c_uint64 * 18), ("generation", ctypes.c_uint32)]
+
+    class Sched(ctypes.Structure):
+        _fields_ = [("point_index", ctypes.c_uint32), ("bucket_index", ctypes.c_uint32),
+                    ("subtract", ctypes.c_uint32)]
+
+    def align(v, a):
+        return -(-v // a) * a
+
+    def ba(window, batch):
+        B = 1 << (window - 1)
+        off = B * ctypes.sizeof(AffBucket)
+        off = align(off, ctypes.alignment(ProjBucket)) + B * ctypes.sizeof(ProjBucket)
+        off = align(off, ctypes.alignment(Sched)) + batch * ctypes.sizeof(Sched)
+        off = align(off, 8) + batch * 48
+        off = align(off, 8) + batch * 48
+        return align(off, 8)
+
+    for n, w, b in ((4096, 10, 128), (2, 1, 128), (5000, 11, 384), (1 << 18, 15, 384), (7, 24, 128)):
+        assert orc.batch_affine_fr_scratch_sizeof(n, w, b) == 32 * n + ba(w, b)
+    assert orc.batch_affine_fr_scratch_sizeof(1, 10, 128) == 32
+    for bad in ((0, 10, 128), (5, 0, 128), (5, 25, 128), (5, 10, 256), (1 << 32, 10, 128)):
+        assert orc.batch_affine_fr_scratch_sizeof(*bad) == 0
+    # the selected-algorithm query: the larger of the two plans inside
+    # [2^12, 2^18], Pippenger outside, and past 2^18 never below the 2^18 plan
+    assert orc.msm_fr_scratch_sizeof(4095) == orc.pippenger_fr_scratch_sizeof(4095)
+    assert orc.msm_fr_scratch_sizeof(4096) == max(orc.pippenger_fr_scratch_sizeof(4096), 32 * 4096 + ba(10, 128))
+    top = 32 * (1 << 18) + ba(15, 384)
+    assert orc.msm_fr_scratch_sizeof(1 << 18) == max(orc.pippenger_fr_scratch_sizeof(1 << 18), top)
+    assert orc.msm_fr_scratch_sizeof((1 << 18) + 1) == max(orc.pippenger_fr_scratch_sizeof((1 << 18) + 1), top)
+    # capacity is monotone in n (BindingHeader_tests.cpp:1190-1203), sampled
+    prev = 0
+    for n in list(range(1, 5000)) + list(range(5000, (1 << 18) + 2, 97)) + [(1 << 18) + 1, 1 << 20]:
+        cur = orc.msm_fr_scratch_sizeof(n)
+        assert cur >= prev and cur >= orc.pippenger_fr_scratch_sizeof(n), n
+        prev = cur
