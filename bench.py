@@ -61,9 +61,10 @@ def load_traffic():
     return None
 
 
-def cpu_baseline(points, scalars, n, gpu_compressed, samples):
+def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None):
     """The oracle (C port of blst_p1s_mult_pippenger) on the host cores, and
-    the checker of sampled batched results."""
+    the checker of sampled batched results; with jac, also the oracle's batch
+    normalisation on a 2^18-point sample of it."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import blst_hip as bh
@@ -84,6 +85,15 @@ def cpu_baseline(points, scalars, n, gpu_compressed, samples):
     r_mt = oracle.msm_pippenger(points, scalars, n, 255, threads=threads)
     dt_mt = time.perf_counter() - t
     assert oracle.compress(r_mt) == cpu_bytes
+    aff_rep = None
+    if jac is not None:
+        m = min(1 << 18, jac.shape[0])
+        t = time.perf_counter()
+        a = oracle.p1s_to_affine(jac[:m])
+        da = time.perf_counter() - t
+        aff_rep = {"value": m / da, "unit": "points/s", "cores": 1, "kind": "port",
+                   "sample": "oracle points_to_affine over the first %d Jacobian points, 1 thread, %.3f s" % (m, da),
+                   "equals_input": bool(np.array_equal(a, points[:m]))}
     return {
         "value": n / dt,
         "unit": "points/s",
@@ -95,7 +105,61 @@ def cpu_baseline(points, scalars, n, gpu_compressed, samples):
         "bit_exact_vs_gpu": cpu_bytes == gpu_compressed,
         "compressed": cpu_bytes.hex(),
         "batched_samples_bit_exact": checked,
+        "p1s_to_affine": aff_rep,
     }
+
+
+def entry_points(bh, points, scalars, r, n, reps=5):
+    """The other drop-in entry points on the same n-point workload, each
+    checked against the plain MSM result or its own input: the blst_fr MSM and
+    the plain MSM from host buffers (PCIe-inclusive), and batch normalisation
+    of Jacobian representatives of the points, on device memory and host to
+    host.  Returns (report, Jacobian points)."""
+    fr = bh.synth_fr(scalars)
+    bh.p1s_mult_fr(points[:4096], fr[:4096])
+    t = time.perf_counter()
+    for _ in range(reps):
+        ok, rf = bh.p1s_mult_fr(points, fr, "pippenger")
+    fr_s = (time.perf_counter() - t) / reps
+    t = time.perf_counter()
+    for _ in range(reps):
+        rp = bh.p1s_mult_pippenger(points, scalars, 255)
+    plain_s = (time.perf_counter() - t) / reps
+    fr_same = ok and bh.compress(rf) == bh.compress(r) == bh.compress(rp)
+
+    jac = bh.synth_jacobian(points, SEED ^ 4)
+    d_in = bh.DeviceBuffer(jac)
+    d_out = bh.DeviceBuffer(np.zeros_like(points))
+    try:
+        bh.p1s_to_affine_device(d_out.ptr, d_in.ptr, n)
+        t = time.perf_counter()
+        for _ in range(4 * reps):
+            bh.p1s_to_affine_device(d_out.ptr, d_in.ptr, n)
+        dev_s = (time.perf_counter() - t) / (4 * reps)
+        dev_same = np.array_equal(d_out.read().view(np.uint64).reshape(points.shape), points)
+    finally:
+        d_in.close()
+        d_out.close()
+    bh.p1s_to_affine(jac[:4096])
+    t = time.perf_counter()
+    for _ in range(reps):
+        aff = bh.p1s_to_affine(jac)
+    host_s = (time.perf_counter() - t) / reps
+    rep = {
+        "p1s_mult_pippenger_fr": {
+            "workload": "the same n points and scalars as blst_fr (Montgomery form), host buffers in and out",
+            "points_per_s": n / fr_s, "ms": fr_s * 1e3,
+            "plain_host_points_per_s": n / plain_s, "plain_host_ms": plain_s * 1e3,
+            "equals_plain": fr_same,
+        },
+        "p1s_to_affine": {
+            "workload": "n Jacobian representatives (x l^2, y l^3, l) of the same points, seeded l",
+            "device_points_per_s": n / dev_s, "device_ms": dev_s * 1e3,
+            "host_points_per_s": n / host_s, "host_ms": host_s * 1e3,
+            "equals_input": dev_same and np.array_equal(aff, points),
+        },
+    }
+    return rep, jac
 
 
 def copy_calibration(nbytes, reps=3):
@@ -129,6 +193,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-batched", action="store_true", help="skip the batched workloads")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
+    ap.add_argument("--no-entry-points", action="store_true",
+                    help="skip the blst_fr MSM and batch-normalisation measurements")
     ap.add_argument("--serve-rate", type=float, default=2800.0, help="offered request rate (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -234,6 +300,13 @@ def main():
     out["compressed"] = gpu_bytes.hex()
     dp.close()
     ds.close()
+    jac = None
+    if not args.no_entry_points:
+        ep, jac = entry_points(bh, points, scalars, r, n)
+        out["entry_points"] = ep
+        print("fr: %.2f ms host, to_affine: %.3f ms device" % (ep["p1s_mult_pippenger_fr"]["ms"],
+                                                               ep["p1s_to_affine"]["device_ms"]),
+              file=sys.stderr, flush=True)
 
     import workloads
 
@@ -254,9 +327,13 @@ def main():
                                                            sv["speedup"]), file=sys.stderr, flush=True)
         out["serving"] = sv
     if not args.no_cpu:
-        cb = cpu_baseline(points, scalars, n, gpu_bytes, samples)
+        cb = cpu_baseline(points, scalars, n, gpu_bytes, samples, jac)
         out["cpu_baseline"] = cb
         out["bit_exact"] = cb["bit_exact_vs_gpu"] and all(cb["batched_samples_bit_exact"].values())
+        if "entry_points" in out:
+            out["bit_exact"] = (out["bit_exact"] and out["entry_points"]["p1s_mult_pippenger_fr"]["equals_plain"]
+                                and out["entry_points"]["p1s_to_affine"]["equals_input"]
+                                and cb["p1s_to_affine"]["equals_input"])
     print(json.dumps(out), flush=True)
 
 

@@ -101,6 +101,16 @@ size_t blst_p1s_mult_fr_scratch_sizeof(size_t max_points);
 bool blst_p1s_mult_fr(blst_p1 *ret, const blst_p1_affine points[], const blst_fr scalars[], size_t npoints,
                       blst_limb_t *scratch, size_t scratch_size);
 
+/* Replaces blst_p1s_to_affine (bindings/blst_scalar.h:521,
+ * blst/c_api.cpp:1277-1289, multi_scalar.cpp:160-238): Jacobian points in
+ * the reference's descriptor convention (a NULL entry after the head
+ * continues contiguously) to affine, canonical Montgomery coordinates; points
+ * with Z = 0 become the all-zero affine encoding.  npoints == 0 writes
+ * nothing; a NULL array or head writes npoints all-zero points.  Coordinates
+ * must be reduced (below p), as every blst_p1 the reference returns is.  A
+ * device error writes all-zero points and sets blst_hip_last_error(). */
+void blst_p1s_to_affine(blst_p1_affine dst[], const blst_p1 *const points[], size_t npoints);
+
 /* ------------------------------------------------------- device extensions */
 
 /* Last error message of the calling thread ("" when the last call succeeded). */
@@ -120,6 +130,7 @@ int blst_hip_msm_device(blst_p1 *ret, const void *device_points, const void *dev
  * buffer on the current device.  Returns NULL on failure. */
 void *blst_hip_device_upload(const void *host, size_t bytes);
 int blst_hip_device_write(void *device_dst, const void *host_src, size_t bytes);
+int blst_hip_device_read(void *host_dst, const void *device_src, size_t bytes);
 void blst_hip_device_free(void *device_ptr);
 
 /* One MSM of a batch; points and scalars are device pointers (points
@@ -199,6 +210,11 @@ void blst_hip_tuning_get(int batch, size_t npoints, size_t nbits, size_t out[3])
  * from 2^17 points).  Returns 0, or -1 for groups > 16. */
 int blst_hip_set_pipeline(size_t groups);
 
+/* blst_p1s_to_affine on device memory: npoints Jacobian points (144 bytes
+ * each) to affine (96 bytes each), both 16-byte aligned and not overlapping;
+ * returns when done, 0 or -1 (npoints below 2^31 - 1). */
+int blst_hip_p1s_to_affine_device(void *device_dst, const void *device_points, size_t npoints);
+
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
  * written to out[npoints] (host).  Returns 0 on success. */
@@ -222,10 +238,14 @@ int blst_hip_p1s_mult_generator(blst_p1_affine *out, const blst_byte *scalars, s
  * 32-byte little-endian scalars (low64_pct percent keep only their low 64
  * bits); the KZG exponents tau^i mod r with tau drawn from the seed; and the
  * edge-case mix (1% zero, 1% one, 1% r-1 scalars, 0.5% repeated points)
- * applied in place at seeded distinct positions. */
+ * applied in place at seeded distinct positions; blst_fr (Montgomery form)
+ * of canonical scalars below r; Jacobian representatives (x l^2, y l^3, l) of
+ * affine points at seeded nonzero l (all-zero points stay all-zero). */
 void blst_hip_synth_scalars(blst_byte *out32, size_t n, uint64_t seed, int low64_pct);
 void blst_hip_synth_srs_scalars(blst_byte *out32, size_t n, uint64_t seed);
 void blst_hip_synth_edge_cases(void *points96, blst_byte *scalars32, size_t n, uint64_t seed);
+void blst_hip_synth_fr(uint64_t *out4, const blst_byte *scalars32, size_t n);
+void blst_hip_synth_jacobian(void *points144, const void *affine96, size_t n, uint64_t seed);
 
 /* 48-byte ZCash compressed encoding of a G1 Jacobian point, the format of
  * blst_p1_compress (bindings/blst_scalar.h:609, blst/c_api.cpp:1618-1621),

@@ -506,6 +506,27 @@ void or_p1s_to_affine(u64 *out_affine, const u64 *jacobian, size_t n) {
     free(pre);
 }
 
+/* Test inputs for batch normalisation: affine (x, y) to the Jacobian
+ * representative (x l^2, y l^3, l) with l = lambda[i] (Montgomery form);
+ * an all-zero affine point gives Z = 0. */
+void or_p1s_from_affine_scaled(u64 *jacobian, const u64 *affine, const u64 *lambda, size_t n) {
+    for (size_t i = 0; i < n; i++) {
+        const aff_t *a = (const aff_t *)(affine + 12 * i);
+        jac_t *j = (jac_t *)(jacobian + 18 * i);
+        const u64 *l = lambda + 6 * i;
+        if (fp_is_zero(a->x) && fp_is_zero(a->y)) {
+            memset(j, 0, sizeof(*j));
+            continue;
+        }
+        u64 l2[6], l3[6];
+        or_fp_mul(l2, l, l);
+        or_fp_mul(l3, l2, l);
+        or_fp_mul(j->x, a->x, l2);
+        or_fp_mul(j->y, a->y, l3);
+        fp_copy(j->z, l);
+    }
+}
+
 /* Variable-time double-and-add over the low nbits of the scalar (the value
  * mult_vartime computes, point.cpp:1546-1565). */
 void or_p1_mult(u64 out[18], const u64 p[18], const uint8_t *scalar, size_t nbits) {

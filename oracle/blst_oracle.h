@@ -42,6 +42,7 @@ int or_p1_is_equal(const uint64_t a[18], const uint64_t b[18]);
 int or_p1_affine_on_curve(const uint64_t a[12]);
 void or_p1_compress(uint8_t out[48], const uint64_t p[18]);
 void or_p1s_to_affine(uint64_t *out_affine, const uint64_t *jacobian, size_t n);
+void or_p1s_from_affine_scaled(uint64_t *jacobian, const uint64_t *affine, const uint64_t *lambda, size_t n);
 
 /* Pippenger pieces */
 size_t or_pippenger_window_size(size_t npoints);

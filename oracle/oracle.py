@@ -47,6 +47,7 @@ def lib():
             "or_p1_to_affine": [_u64p, _u64p],
             "or_p1_compress": [_u8p, _u64p],
             "or_p1s_to_affine": [_u64p, _u64p, _sz],
+            "or_p1s_from_affine_scaled": [_u64p, _u64p, _u64p, _sz],
             "or_msm_pippenger": [_u64p, _u64p, _u8p, _sz, _sz, _sz],
             "or_msm_pippenger_mt": [_u64p, _u64p, _u8p, _sz, _sz, _sz, ctypes.c_int],
             "or_msm_tile": [_u64p, _u64p, _u8p, _sz, _sz, _sz, _sz],
@@ -156,6 +157,27 @@ def to_affine(p):
     r = np.zeros(12, np.uint64)
     lib().or_p1_to_affine(u64p(r), u64p(np.ascontiguousarray(p)))
     return r
+
+
+def p1s_to_affine(jac):
+    """(n, 18) Jacobian -> (n, 12) affine (points_to_affine)."""
+    jac = np.ascontiguousarray(jac, dtype=np.uint64)
+    out = np.zeros((jac.shape[0], 12), np.uint64)
+    if jac.shape[0]:
+        lib().or_p1s_to_affine(u64p(out.reshape(-1)), u64p(jac.reshape(-1)), jac.shape[0])
+    return out
+
+
+def p1s_from_affine_scaled(aff, lam):
+    """(n, 12) affine and (n, 6) Montgomery lambdas -> (n, 18) Jacobian
+    (x l^2, y l^3, l); all-zero affine -> Z = 0."""
+    aff = np.ascontiguousarray(aff, dtype=np.uint64)
+    lam = np.ascontiguousarray(lam, dtype=np.uint64)
+    out = np.zeros((aff.shape[0], 18), np.uint64)
+    if aff.shape[0]:
+        lib().or_p1s_from_affine_scaled(u64p(out.reshape(-1)), u64p(aff.reshape(-1)), u64p(lam.reshape(-1)),
+                                        aff.shape[0])
+    return out
 
 
 def compress(p):

@@ -25,6 +25,8 @@ SIGNATURES = {
     "blst_p1s_mult_pippenger_fr": ([_vp, _vp, _vp, _sz, _vp, _sz], ctypes.c_bool),
     "blst_p1s_mult_fr_scratch_sizeof": ([_sz], _sz),
     "blst_p1s_mult_fr": ([_vp, _vp, _vp, _sz, _vp, _sz], ctypes.c_bool),
+    "blst_p1s_to_affine": ([_vp, _vp, _sz], None),
+    "blst_hip_p1s_to_affine_device": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_last_error": ([], ctypes.c_char_p),
     "blst_hip_available": ([], ctypes.c_int),
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
@@ -35,10 +37,13 @@ SIGNATURES = {
     "blst_hip_device_upload": ([_vp, _sz], _vp),
     "blst_hip_device_free": ([_vp], None),
     "blst_hip_device_write": ([_vp, _vp, _sz], ctypes.c_int),
+    "blst_hip_device_read": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
     "blst_hip_synth_scalars": ([_vp, _sz, ctypes.c_uint64, ctypes.c_int], None),
     "blst_hip_synth_srs_scalars": ([_vp, _sz, ctypes.c_uint64], None),
     "blst_hip_synth_edge_cases": ([_vp, _vp, _sz, ctypes.c_uint64], None),
+    "blst_hip_synth_fr": ([_vp, _vp, _sz], None),
+    "blst_hip_synth_jacobian": ([_vp, _vp, _sz, ctypes.c_uint64], None),
     "blst_hip_p1_compress": ([_vp, _vp], None),
     "blst_hip_msm_batch_device": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_batch_plan_create": ([_vp, _sz], _vp),
@@ -181,6 +186,35 @@ def p1s_mult_fr(points, fr, entry="pippenger", scratch_size=None, ret=None, null
     return bool(ok), ret
 
 
+def p1s_to_affine(points, mode="contiguous", k=1):
+    """(n, 18) uint64 Jacobian points -> (n, 12) affine through
+    blst_p1s_to_affine, descriptors as in Descriptors."""
+    L = load()
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    n = pts.shape[0]
+    base = pts.ctypes.data
+    if mode == "contiguous":
+        pp = [base, 0]
+    elif mode == "exact":
+        pp = [base + 144 * i for i in range(n)]
+    elif mode == "mixed":
+        pp = [base + 144 * i if i < k else 0 for i in range(n)]
+    else:
+        raise ValueError(mode)
+    pp = np.array(pp, dtype=np.uint64)
+    out = np.zeros((max(n, 1), 12), np.uint64)
+    L.blst_p1s_to_affine(_ptr(out), _ptr(pp), n)
+    err = last_error()
+    if err and n > 0:
+        raise RuntimeError("blst_p1s_to_affine: " + err)
+    return out[:n]
+
+
+def p1s_to_affine_device(d_out_ptr, d_points_ptr, n):
+    if load().blst_hip_p1s_to_affine_device(d_out_ptr, d_points_ptr, n) != 0:
+        raise RuntimeError("blst_hip_p1s_to_affine_device: " + last_error())
+
+
 def p1s_tile_pippenger(points, scalars, nbits, bit0, window):
     L = load()
     n = points.shape[0]
@@ -263,6 +297,22 @@ def synth_srs_scalars(n, seed):
 def synth_edge_cases(points, scalars, seed):
     assert points.flags.c_contiguous and scalars.flags.c_contiguous
     load().blst_hip_synth_edge_cases(_ptr(points), _ptr(scalars), points.shape[0], seed)
+
+
+def synth_fr(scalars):
+    """Montgomery-form blst_fr (n x 4 uint64) of canonical scalars below r."""
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8)
+    out = np.zeros((sc.shape[0], 4), np.uint64)
+    load().blst_hip_synth_fr(_ptr(out), _ptr(sc), sc.shape[0])
+    return out
+
+
+def synth_jacobian(affine, seed):
+    """Jacobian representatives (n x 18 uint64) of affine points at seeded Z."""
+    a = np.ascontiguousarray(affine, dtype=np.uint64)
+    out = np.zeros((a.shape[0], 18), np.uint64)
+    load().blst_hip_synth_jacobian(_ptr(out), _ptr(a), a.shape[0], seed)
+    return out
 
 
 def synth_points(n, seed):
@@ -402,6 +452,13 @@ class DeviceBuffer:
         assert host.nbytes == self.nbytes
         if load().blst_hip_device_write(ctypes.c_void_p(self.ptr), _ptr(host), host.nbytes):
             raise RuntimeError("blst_hip_device_write: " + last_error())
+
+    def read(self):
+        """The buffer's contents as a uint8 host array."""
+        out = np.zeros(self.nbytes, np.uint8)
+        if load().blst_hip_device_read(_ptr(out), ctypes.c_void_p(self.ptr), self.nbytes):
+            raise RuntimeError("blst_hip_device_read: " + last_error())
+        return out
 
     def close(self):
         if self.ptr:

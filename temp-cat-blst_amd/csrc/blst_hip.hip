@@ -40,6 +40,7 @@ struct Ctx {
     hipEvent_t ready = nullptr;                     // inputs written on stream
     DevBuf pts, scl, dbg;
     DevBuf gtable, gen_tmp, gen_out;     // comb table of G, fixed-base scratch
+    DevBuf jac_in, aff_out;              // batch normalisation staging
     bool gtable_ready = false;
 };
 
@@ -724,6 +725,18 @@ bool specs_from_jobs(const blst_hip_msm_job* jobs, size_t njobs, std::vector<Job
     return true;
 }
 
+// Lanes of the Jacobian -> affine kernel: four waves per CU, each lane
+// inverting once for every n / 65536 points.
+constexpr size_t kAffineLanes = 65536;
+// points per host round trip of blst_p1s_to_affine (2.4 GB in, 1.6 GB out)
+constexpr size_t kAffineChunk = (size_t)1 << 24;
+
+void enqueue_to_affine(Ctx& cx, const void* d_in, void* d_out, size_t n) {
+    const uint32_t T = (uint32_t)std::min(n, kAffineLanes);
+    launch_k_jac_to_affine(cx.stream, static_cast<const uint4*>(d_in), static_cast<aff*>(d_out), (uint32_t)n, T);
+    HIPCK(hipGetLastError());
+}
+
 }  // namespace host
 }  // namespace bls
 
@@ -1166,8 +1179,75 @@ void blst_hip_host_free(void* host_ptr) {
     if (host_ptr) (void)hipHostFree(host_ptr);
 }
 
+int blst_hip_device_read(void* host_dst, const void* device_src, size_t bytes) {
+    if (!host_dst || !device_src) { g_err = "invalid arguments"; return -1; }
+    if (hipMemcpy(host_dst, device_src, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
+        g_err = "hipMemcpy failed";
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
 void blst_hip_device_free(void* device_ptr) {
     if (device_ptr) (void)hipFree(device_ptr);
+}
+
+void blst_p1s_to_affine(blst_p1_affine dst[], const blst_p1* const points[], size_t npoints) {
+    if (npoints == 0) { g_err.clear(); return; }
+    if (dst == nullptr) { g_err = "dst is NULL"; return; }
+    if (points == nullptr || points[0] == nullptr) {
+        std::memset(dst, 0, npoints * sizeof(blst_p1_affine));
+        g_err.clear();
+        return;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        std::vector<uint8_t> stage;
+        const uint8_t* hp = resolve(points, npoints, sizeof(blst_p1), stage);
+        const size_t chunk = std::min(npoints, kAffineChunk);
+        cx.jac_in.ensure(chunk * sizeof(blst_p1));
+        cx.aff_out.ensure(chunk * sizeof(blst_p1_affine));
+        for (size_t b = 0; b < npoints; b += chunk) {
+            const size_t m = std::min(chunk, npoints - b);
+            HIPCK(hipMemcpyAsync(cx.jac_in.p, hp + b * sizeof(blst_p1), m * sizeof(blst_p1), hipMemcpyHostToDevice,
+                                 cx.stream));
+            enqueue_to_affine(cx, cx.jac_in.p, cx.aff_out.p, m);
+            HIPCK(hipMemcpyAsync(dst + b, cx.aff_out.p, m * sizeof(blst_p1_affine), hipMemcpyDeviceToHost,
+                                 cx.stream));
+        }
+        HIPCK(hipStreamSynchronize(cx.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        std::memset(dst, 0, npoints * sizeof(blst_p1_affine));
+        return;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        std::memset(dst, 0, npoints * sizeof(blst_p1_affine));
+        return;
+    }
+    g_err.clear();
+}
+
+int blst_hip_p1s_to_affine_device(void* device_dst, const void* device_points, size_t npoints) {
+    if (npoints == 0) { g_err.clear(); return 0; }
+    if (!device_dst || !device_points || npoints >= 0x7fffffffULL || ((uintptr_t)device_dst % 16) ||
+        ((uintptr_t)device_points % 16)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        enqueue_to_affine(cx, device_points, device_dst, npoints);
+        HIPCK(hipStreamSynchronize(cx.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
 }
 
 int blst_hip_p1s_mult_generator(blst_p1_affine* out, const blst_byte* scalars, size_t npoints,

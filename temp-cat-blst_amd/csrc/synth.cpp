@@ -1,11 +1,14 @@
 // Seeded synthetic MSM workloads for benchmarks and the serving loop's load
-// generator: uniform scalars of Fr, KZG setup exponents tau^i and the
-// edge-case mix of the plain-MSM workload.  Random source: xorshift64*
+// generator: uniform scalars of Fr, KZG setup exponents tau^i, the edge-case
+// mix of the plain-MSM workload, Montgomery-form blst_fr scalars and Jacobian
+// representatives of affine points.  Random source: xorshift64*
 // (Vigna), the generator of the reference's tests (test/InsecureRng.h).
 #include <stdint.h>
 #include <string.h>
 
 #include <vector>
+
+#include "host_ec.h"
 
 namespace {
 
@@ -82,6 +85,42 @@ void mul_mod_r(uint64_t r[4], const uint64_t a[4], const uint64_t b[4]) {
     memcpy(r, rem, 32);
 }
 
+// Montgomery multiplication mod r on four 64-bit limbs (CIOS), a * b / 2^256
+void mont_mul_r(uint64_t out[4], const uint64_t a[4], const uint64_t b[4]) {
+    const uint64_t n0 = 0xfffffffeffffffffULL;   // -r^-1 mod 2^64
+    uint64_t t[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 4; i++) {
+        u128 c = 0;
+        for (int j = 0; j < 4; j++) {
+            c += (u128)a[j] * b[i] + t[j];
+            t[j] = (uint64_t)c;
+            c >>= 64;
+        }
+        c += t[4];
+        t[4] = (uint64_t)c;
+        t[5] = (uint64_t)(c >> 64);
+        const uint64_t m = t[0] * n0;
+        c = ((u128)m * kR[0] + t[0]) >> 64;
+        for (int j = 1; j < 4; j++) {
+            c += (u128)m * kR[j] + t[j];
+            t[j - 1] = (uint64_t)c;
+            c >>= 64;
+        }
+        c += t[4];
+        t[3] = (uint64_t)c;
+        t[4] = t[5] + (uint64_t)(c >> 64);
+    }
+    uint64_t d[4];
+    uint64_t borrow = 0;
+    for (int j = 0; j < 4; j++) {
+        const u128 v = (u128)t[j] - kR[j] - borrow;
+        d[j] = (uint64_t)v;
+        borrow = (uint64_t)(v >> 64) & 1;
+    }
+    const bool ge = t[4] != 0 || borrow == 0;
+    for (int j = 0; j < 4; j++) out[j] = ge ? d[j] : t[j];
+}
+
 }  // namespace
 
 extern "C" {
@@ -136,6 +175,44 @@ void blst_hip_synth_edge_cases(void* points96, uint8_t* scalars32, size_t n, uin
         size_t src = (size_t)(g.next() % (uint64_t)n);
         if (src == dst) src = (src + 1) % n;
         memcpy(pts + 96 * dst, pts + 96 * src, 96);
+    }
+}
+
+// blst_fr (Montgomery form, k * 2^256 mod r as four little-endian 64-bit
+// limbs) of n canonical 32-byte scalars below r
+void blst_hip_synth_fr(uint64_t* out4, const uint8_t* scalars32, size_t n) {
+    static const uint64_t kRR[4] = {0xc999e990f3f29c6dULL, 0x2b6cedcb87925c23ULL, 0x05d314967254398fULL,
+                                    0x0748d9d99f59ff11ULL};   // 2^512 mod r
+    for (size_t i = 0; i < n; i++) {
+        uint64_t k[4];
+        memcpy(k, scalars32 + 32 * i, 32);
+        mont_mul_r(out4 + 4 * i, k, kRR);
+    }
+}
+
+// Jacobian representatives (x l^2, y l^3, l) of n affine points, l drawn per
+// point from the seeded stream (five random 64-bit words: nonzero and below
+// p); all-zero affine points give the all-zero Jacobian point
+void blst_hip_synth_jacobian(void* points144, const void* affine96, size_t n, uint64_t seed) {
+    Rng g(seed);
+    const hostec::fp6* a = static_cast<const hostec::fp6*>(affine96);
+    hostec::jac6* out = static_cast<hostec::jac6*>(points144);
+    for (size_t i = 0; i < n; i++) {
+        const hostec::fp6& x = a[2 * i];
+        const hostec::fp6& y = a[2 * i + 1];
+        hostec::fp6 l;
+        for (int w = 0; w < 5; w++) l.l[w] = g.next() | 1;
+        l.l[5] = 0;
+        if (hostec::is_zero(x) && hostec::is_zero(y)) {
+            memset(&out[i], 0, sizeof(out[i]));
+            continue;
+        }
+        hostec::fp6 l2, l3;
+        hostec::mul(l2, l, l);
+        hostec::mul(l3, l2, l);
+        hostec::mul(out[i].x, x, l2);
+        hostec::mul(out[i].y, y, l3);
+        out[i].z = l;
     }
 }
 

@@ -147,3 +147,39 @@ def test_fr_fail_closed_without_gpu_work():
         ret = np.full(18, 7, np.uint64)
         ok, out = blst_hip.p1s_mult_fr(pts[:0], fr[:0], entry, ret=ret, null=("points", "scalars", "scratch"))
         assert ok and not out.any()
+
+
+def test_to_affine_without_gpu_work():
+    """blst_p1s_to_affine's early returns (c_api.cpp:1277-1285): no points
+    writes nothing, a NULL array or head writes all-zero points."""
+    import blst_hip
+    L = blst_hip.load()
+    dst = np.full((3, 12), 7, np.uint64)
+    L.blst_p1s_to_affine(blst_hip._ptr(dst), None, 0)
+    assert (dst == 7).all()
+    L.blst_p1s_to_affine(blst_hip._ptr(dst), None, 3)
+    assert not dst.any()
+    dst[:] = 7
+    heads = np.zeros(2, np.uint64)
+    L.blst_p1s_to_affine(blst_hip._ptr(dst), blst_hip._ptr(heads), 3)
+    assert not dst.any()
+
+
+def test_fr_and_jacobian_generators(orc):
+    """The Montgomery-form and Jacobian workload generators: blst_fr values
+    equal k * 2^256 mod r (Python big integers) and convert back through the
+    oracle; Jacobian representatives normalise back to their affine points."""
+    import blst_hip
+    sc = blst_hip.synth_scalars(500, 11)
+    sc[0] = 0
+    fr = blst_hip.synth_fr(sc)
+    for s, f in zip(sc, fr):
+        k = int.from_bytes(s.tobytes(), "little")
+        assert orc.value(f) == (k << 256) % pyref.R_ORDER
+    assert np.array_equal(orc.fr_to_scalars(fr), sc)
+    aff = orc.gen_points(300, 12, threads=2)
+    aff[::17] = 0
+    jac = blst_hip.synth_jacobian(aff, 13)
+    assert np.array_equal(orc.p1s_to_affine(jac), aff)
+    assert all(j[12:].any() for j, a in zip(jac, aff) if a.any())
+    assert not jac[::17].any()

@@ -445,3 +445,64 @@ def test_fr_2pow20_equals_canonical_scalars(hip, orc):
     ok, got = hip.p1s_mult_fr(pts, fr, "pippenger")
     assert ok
     assert hip.compress(got) == hip.compress(hip.p1s_mult_pippenger(pts, sc, 255))
+
+
+# ---------------------------------------------------------------- batch normalisation
+
+def _jacobian_inputs(orc, n, seed, inf_every=0):
+    """n Jacobian points (random multiples of G at random Z), with Z = 0
+    every inf_every-th point."""
+    rnd = np.random.default_rng(seed)
+    aff = orc.gen_points(n, seed, threads=8)
+    if inf_every:
+        aff[::inf_every] = 0
+    lam = np.zeros((n, 6), np.uint64)
+    lam[:, :5] = rnd.integers(1, 1 << 63, size=(n, 5), dtype=np.uint64)   # nonzero, below p
+    return aff, orc.p1s_from_affine_scaled(aff, lam)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 1536, 1537, 65536, 65537, 200001])
+def test_to_affine_matches_oracle(hip, orc, n):
+    aff, jac = _jacobian_inputs(orc, n, 0xAF + n, inf_every=13)
+    got = hip.p1s_to_affine(jac)
+    assert np.array_equal(got, orc.p1s_to_affine(jac))
+    assert np.array_equal(got, aff)
+
+
+@pytest.mark.parametrize("mode", ["exact", "mixed"])
+def test_to_affine_descriptors(hip, orc, mode):
+    aff, jac = _jacobian_inputs(orc, 300, 0xD5, inf_every=7)
+    assert np.array_equal(hip.p1s_to_affine(jac, mode=mode, k=37), aff)
+
+
+def test_to_affine_all_infinity_and_mixed_z(hip, orc):
+    jac = np.zeros((70000, 18), np.uint64)
+    assert not hip.p1s_to_affine(jac).any()
+    g = orc.from_affine(orc.generator_affine())
+    jac[5] = orc.p1_mult(g, (12345).to_bytes(32, "little"), 255)   # Z from the formulas
+    jac[69999] = g                                                  # Z = one
+    got = hip.p1s_to_affine(jac)
+    assert np.array_equal(got[5], orc.to_affine(jac[5])) and np.array_equal(got[69999], orc.generator_affine())
+    assert not np.delete(got, [5, 69999], axis=0).any()
+
+
+def test_to_affine_device_2pow20_round_trip(hip, orc):
+    """At 2^20 points: affine -> scaled Jacobian -> the device kernel gives
+    back the same affine points (a size-independent round trip)."""
+    n = 1 << 20
+    aff = hip.synth_points(n, 0x77)
+    aff[::4099] = 0
+    rnd = np.random.default_rng(5)
+    lam = np.zeros((n, 6), np.uint64)
+    lam[:, :5] = rnd.integers(1, 1 << 63, size=(n, 5), dtype=np.uint64)
+    jac = orc.p1s_from_affine_scaled(aff, lam)
+    d_in = hip.DeviceBuffer(jac)
+    d_out = hip.DeviceBuffer(np.zeros((n, 12), np.uint64))
+    try:
+        hip.p1s_to_affine_device(d_out.ptr, d_in.ptr, n)
+        got = d_out.read().view(np.uint64).reshape(n, 12)
+    finally:
+        d_in.close()
+        d_out.close()
+    assert np.array_equal(got, aff)
+    assert np.array_equal(hip.p1s_to_affine(jac), aff)
