@@ -111,6 +111,35 @@ bool blst_p1s_mult_fr(blst_p1 *ret, const blst_p1_affine points[], const blst_fr
  * device error writes all-zero points and sets blst_hip_last_error(). */
 void blst_p1s_to_affine(blst_p1_affine dst[], const blst_p1 *const points[], size_t npoints);
 
+/* Replaces blst_p1s_mult_wbits_precompute_sizeof (bindings/blst_scalar.h:527,
+ * blst/c_api.cpp:1310-1316, multi_scalar.h:244-256): 96 * npoints *
+ * 2^(wbits-1) bytes; 0 for wbits outside [2, 14] or on overflow. */
+size_t blst_p1s_mult_wbits_precompute_sizeof(size_t wbits, size_t npoints);
+
+/* Replaces blst_p1s_mult_wbits_precompute (bindings/blst_scalar.h:529,
+ * blst/c_api.cpp:1318-1330, multi_scalar.cpp:268-286, 481-580): row i of the
+ * table holds [1]P_i ... [2^(wbits-1)]P_i in affine form (all-zero rows for
+ * an all-zero point), built on the device.  Points use the descriptor
+ * convention; a packed points array may occupy the table's last npoints
+ * entries.  Invalid arguments write nothing. */
+void blst_p1s_mult_wbits_precompute(blst_p1_affine table[], size_t wbits, const blst_p1_affine *const points[],
+                                    size_t npoints);
+
+/* Replaces blst_p1s_mult_wbits_scratch_sizeof (bindings/blst_scalar.h:531,
+ * blst/c_api.cpp:1332-1335, multi_scalar.h:310-315): 144 bytes per point,
+ * at most 8192 points' worth. */
+size_t blst_p1s_mult_wbits_scratch_sizeof(size_t npoints);
+
+/* Replaces blst_p1s_mult_wbits (bindings/blst_scalar.h:533,
+ * blst/c_api.cpp:1337-1369, multi_scalar.cpp:620-707): the MSM over a table
+ * from blst_p1s_mult_wbits_precompute with the same wbits.  npoints == 0 or
+ * nbits == 0 writes infinity; nbits > 256, wbits outside [2, 14], or a NULL
+ * table, scalar head or scratch writes infinity.  The host gathers the signed
+ * table entries the scalars' Booth digits select; the device sums them per
+ * window.  The scratch is not written. */
+void blst_p1s_mult_wbits(blst_p1 *ret, const blst_p1_affine table[], size_t wbits, size_t npoints,
+                         const blst_byte *const scalars[], size_t nbits, blst_limb_t *scratch);
+
 /* ------------------------------------------------------- device extensions */
 
 /* Last error message of the calling thread ("" when the last call succeeded). */
@@ -214,6 +243,16 @@ int blst_hip_set_pipeline(size_t groups);
  * each) to affine (96 bytes each), both 16-byte aligned and not overlapping;
  * returns when done, 0 or -1 (npoints below 2^31 - 1). */
 int blst_hip_p1s_to_affine_device(void *device_dst, const void *device_points, size_t npoints);
+
+/* Fixed-base tables resident in device memory (the table of a resident
+ * setup never crosses PCIe): blst_p1s_mult_wbits_precompute from device
+ * points into a device table of 96 * npoints * 2^(wbits-1) bytes, and
+ * blst_p1s_mult_wbits over it with device scalars (npoints contiguous
+ * ceil(nbits/8)-byte scalars).  Both return when done, 0 or -1. */
+int blst_hip_p1s_mult_wbits_precompute_device(void *device_table, size_t wbits, const void *device_points,
+                                              size_t npoints);
+int blst_hip_p1s_mult_wbits_device(blst_p1 *ret, const void *device_table, size_t wbits, size_t npoints,
+                                   const void *device_scalars, size_t nbits);
 
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),

@@ -1078,3 +1078,100 @@ void or_apply_edge_cases(u64 *points, uint8_t *scalars32, size_t n, uint64_t see
     }
     free(perm);
 }
+
+/* ------------------------------------------------- fixed-base wbits tables */
+
+/* points_mult_wbits_precompute_sizeof (multi_scalar.h:244-256) with the
+ * c_api.cpp:1310-1316 check: wbits in [2, 14], 96 * n * 2^(wbits-1) bytes,
+ * 0 on overflow. */
+size_t or_wbits_precompute_sizeof(size_t wbits, size_t npoints) {
+    if (wbits < 2 || wbits > 14) return 0;
+    if (npoints > SIZE_MAX / 96) return 0;
+    const size_t bytes = 96 * npoints;
+    if (bytes > (SIZE_MAX >> (wbits - 1))) return 0;
+    return bytes << (wbits - 1);
+}
+
+/* points_mult_wbits_scratch_sizeof (multi_scalar.h:310-315): one
+ * BatchAffineNode<Fp> (affine point + Fp, 144 bytes) per point, at most
+ * fixed_table_scratch_capacity<Fp>() = 8192 (multi_scalar.h:218-222). */
+size_t or_wbits_scratch_sizeof(size_t npoints) {
+    return 144 * (npoints < 8192 ? npoints : 8192);
+}
+
+/* point_precompute_row (multi_scalar.cpp:268-286): row[0] = P, row[1] = 2P,
+ * then row[i] = row[i-1] + P and row[i+1] = 2 row[j] for i = 2, 4, ...,
+ * j = 1, 2, ...; an all-zero P leaves [0, 0, 1] entries, which normalise to
+ * the all-zero affine point.  msm_precompute_wbits (multi_scalar.cpp:481-580)
+ * then writes the rows to affine, row i at table[i * 2^(wbits-1)]. */
+void or_wbits_precompute(u64 *table, size_t wbits, const u64 *points, size_t npoints) {
+    const size_t m = (size_t)1 << (wbits - 1);
+    jac_t *row = malloc(m * sizeof(jac_t));
+    for (size_t i = 0; i < npoints; i++) {
+        const aff_t *p = (const aff_t *)(points + 12 * i);
+        aff_t *dst = (aff_t *)(table + 12 * m * i);
+        if (aff_is_inf(p)) {
+            memset(dst, 0, m * sizeof(aff_t));
+            continue;
+        }
+        jac_from_aff(&row[0], p);
+        jac_double(&row[1], &row[0]);
+        for (size_t k = 2, j = 1; k < m; k += 2, j++) {
+            jac_add(&row[k], &row[k - 1], &row[0]);
+            jac_double(&row[k + 1], &row[j]);
+        }
+        or_p1s_to_affine((u64 *)dst, (const u64 *)row, m);
+    }
+    free(row);
+}
+
+/* point_gather_booth_wbits (multi_scalar.h:262-276): magnitude in the low
+ * wbits bits of the encoding, sign at bit wbits; 0 gathers infinity. */
+static void gather_booth(jac_t *out, const aff_t *row, size_t wbits, u64 enc) {
+    const u64 sign = (enc >> wbits) & 1;
+    const u64 idx = enc & (((u64)1 << wbits) - 1);
+    if (idx == 0) {
+        memset(out, 0, sizeof(*out));
+        return;
+    }
+    aff_t a = row[idx - 1];
+    if (sign) or_fp_neg(a.y, a.y);
+    jac_from_aff(out, &a);
+}
+
+/* msm_mult_table (multi_scalar.cpp:620-707): the top window holds
+ * nbits % wbits bits (with the overlap bit below it), then full windows from
+ * the top, the lowest without an overlap bit; per window the gathered
+ * entries are summed into ret, then ret is doubled wbits times. */
+void or_mult_wbits(u64 ret[18], const u64 *table, size_t wbits, size_t npoints, const uint8_t *scalars,
+                   size_t nbits) {
+    jac_t *r = (jac_t *)ret;
+    memset(r, 0, sizeof(*r));
+    if (npoints == 0 || nbits == 0 || nbits > 256 || wbits < 2 || wbits > 14) return;
+    const size_t m = (size_t)1 << (wbits - 1), nbytes = (nbits + 7) / 8;
+    const aff_t *tab = (const aff_t *)table;
+    size_t cur = nbits % wbits;
+    u64 mask = ((u64)1 << (cur + 1)) - 1;
+    size_t rem = nbits - cur;
+    const size_t no_overlap = rem == 0, overlap = no_overlap ^ 1;
+    jac_t g;
+    for (size_t i = 0; i < npoints; i++) {
+        const uint8_t *s = scalars + nbytes * i;
+        u64 wval = (get_wval_limb(s, rem - overlap, cur + overlap) << no_overlap) & mask;
+        gather_booth(&g, tab + m * i, wbits, booth_encode(wval, wbits));
+        jac_add(r, r, &g);
+    }
+    while (rem > 0) {
+        for (size_t k = 0; k < wbits; k++) jac_double(r, r);
+        cur = wbits;
+        mask = ((u64)1 << (cur + 1)) - 1;
+        rem -= cur;
+        for (size_t i = 0; i < npoints; i++) {
+            const uint8_t *s = scalars + nbytes * i;
+            const u64 wval = rem ? get_wval_limb(s, rem - 1, cur + 1) & mask
+                                 : (get_wval_limb(s, 0, cur) << 1) & mask;
+            gather_booth(&g, tab + m * i, wbits, booth_encode(wval, wbits));
+            jac_add(r, r, &g);
+        }
+    }
+}

@@ -57,6 +57,14 @@ size_t or_pippenger_fr_scratch_sizeof(size_t npoints);
 size_t or_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t batch);
 size_t or_msm_fr_scratch_sizeof(size_t max_points);
 
+/* fixed-base tables: n rows of 2^(wbits-1) affine multiples, and the MSM
+ * over them (contiguous scalars of ceil(nbits/8) bytes) */
+size_t or_wbits_precompute_sizeof(size_t wbits, size_t npoints);
+size_t or_wbits_scratch_sizeof(size_t npoints);
+void or_wbits_precompute(uint64_t *table, size_t wbits, const uint64_t *points, size_t npoints);
+void or_mult_wbits(uint64_t ret[18], const uint64_t *table, size_t wbits, size_t npoints,
+                   const uint8_t *scalars, size_t nbits);
+
 /* MSM over contiguous arrays (points: n x 12 limbs, scalars: n x ceil(nbits/8)) */
 void or_msm_pippenger(uint64_t ret[18], const uint64_t *points, const uint8_t *scalars,
                       size_t npoints, size_t nbits, size_t window);

@@ -58,6 +58,8 @@ def lib():
             "or_apply_edge_cases": [_u64p, _u8p, _sz, ctypes.c_uint64],
             "or_fr_to_scalar": [_u8p, _u64p],
             "or_frs_to_scalars": [_u8p, _u64p, _sz],
+            "or_wbits_precompute": [_u64p, _sz, _u64p, _sz],
+            "or_mult_wbits": [_u64p, _u64p, _sz, _sz, _u8p, _sz],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -74,6 +76,8 @@ def lib():
             ("or_pippenger_fr_scratch_sizeof", [_sz], _sz),
             ("or_batch_affine_fr_scratch_sizeof", [_sz, _sz, _sz], _sz),
             ("or_msm_fr_scratch_sizeof", [_sz], _sz),
+            ("or_wbits_precompute_sizeof", [_sz, _sz], _sz),
+            ("or_wbits_scratch_sizeof", [_sz], _sz),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -271,6 +275,34 @@ def batch_affine_fr_scratch_sizeof(n, window, batch):
 
 def msm_fr_scratch_sizeof(n):
     return lib().or_msm_fr_scratch_sizeof(n)
+
+
+# ---- fixed-base tables -----------------------------------------------------
+
+def wbits_precompute_sizeof(wbits, n):
+    return lib().or_wbits_precompute_sizeof(wbits, n)
+
+
+def wbits_scratch_sizeof(n):
+    return lib().or_wbits_scratch_sizeof(n)
+
+
+def wbits_precompute(points, wbits):
+    """(n, 12) affine -> (n * 2^(wbits-1), 12) affine table, rows of [j]P."""
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    n = pts.shape[0]
+    out = np.zeros((n << (wbits - 1), 12), np.uint64)
+    if n:
+        lib().or_wbits_precompute(u64p(out.reshape(-1)), wbits, u64p(pts.reshape(-1)), n)
+    return out
+
+
+def mult_wbits(table, wbits, scalars, n, nbits):
+    r = np.zeros(18, np.uint64)
+    tab = np.ascontiguousarray(table, dtype=np.uint64).reshape(-1)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
+    lib().or_mult_wbits(u64p(r), u64p(tab), wbits, n, u8p(sc), nbits)
+    return r
 
 
 # ---- synthetic inputs ------------------------------------------------------

@@ -26,6 +26,12 @@ SIGNATURES = {
     "blst_p1s_mult_fr_scratch_sizeof": ([_sz], _sz),
     "blst_p1s_mult_fr": ([_vp, _vp, _vp, _sz, _vp, _sz], ctypes.c_bool),
     "blst_p1s_to_affine": ([_vp, _vp, _sz], None),
+    "blst_p1s_mult_wbits_precompute_sizeof": ([_sz, _sz], _sz),
+    "blst_p1s_mult_wbits_precompute": ([_vp, _sz, _vp, _sz], None),
+    "blst_p1s_mult_wbits_scratch_sizeof": ([_sz], _sz),
+    "blst_p1s_mult_wbits": ([_vp, _vp, _sz, _sz, _vp, _sz, _vp], None),
+    "blst_hip_p1s_mult_wbits_precompute_device": ([_vp, _sz, _vp, _sz], ctypes.c_int),
+    "blst_hip_p1s_mult_wbits_device": ([_vp, _vp, _sz, _sz, _vp, _sz], ctypes.c_int),
     "blst_hip_p1s_to_affine_device": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_last_error": ([], ctypes.c_char_p),
     "blst_hip_available": ([], ctypes.c_int),
@@ -213,6 +219,65 @@ def p1s_to_affine(points, mode="contiguous", k=1):
 def p1s_to_affine_device(d_out_ptr, d_points_ptr, n):
     if load().blst_hip_p1s_to_affine_device(d_out_ptr, d_points_ptr, n) != 0:
         raise RuntimeError("blst_hip_p1s_to_affine_device: " + last_error())
+
+
+def wbits_precompute(points, wbits, mode="contiguous", k=1):
+    """(n, 12) affine points -> (n * 2^(wbits-1), 12) table through
+    blst_p1s_mult_wbits_precompute (descriptors as in Descriptors)."""
+    L = load()
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    n = pts.shape[0]
+    d = Descriptors(pts, np.zeros((n, 1), np.uint8), mode, k)
+    size = L.blst_p1s_mult_wbits_precompute_sizeof(wbits, n)
+    table = np.zeros((max(size // 96, 1), 12), np.uint64)
+    L.blst_p1s_mult_wbits_precompute(_ptr(table), wbits, _ptr(d.pp), n)
+    err = last_error()
+    if err and n > 0:
+        raise RuntimeError("blst_p1s_mult_wbits_precompute: " + err)
+    return table[:size // 96]
+
+
+def mult_wbits(table, wbits, scalars, nbits, mode="contiguous", k=1):
+    """MSM over a wbits table through blst_p1s_mult_wbits."""
+    L = load()
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8)
+    n = sc.shape[0]
+    tab = np.ascontiguousarray(table, dtype=np.uint64)
+    d = Descriptors(np.zeros((n, 12), np.uint64), sc, mode, k)
+    scr = np.zeros(max(L.blst_p1s_mult_wbits_scratch_sizeof(n) // 8, 1), np.uint64)
+    ret = np.zeros(18, np.uint64)
+    L.blst_p1s_mult_wbits(_ptr(ret), _ptr(tab), wbits, n, _ptr(d.ss), nbits, _ptr(scr))
+    err = last_error()
+    if err and n > 0:
+        raise RuntimeError("blst_p1s_mult_wbits: " + err)
+    return ret
+
+
+def wbits_precompute_device(d_table_ptr, wbits, d_points_ptr, n):
+    if load().blst_hip_p1s_mult_wbits_precompute_device(d_table_ptr, wbits, d_points_ptr, n) != 0:
+        raise RuntimeError("blst_hip_p1s_mult_wbits_precompute_device: " + last_error())
+
+
+def mult_wbits_device(d_table_ptr, wbits, n, d_scalars_ptr, nbits):
+    ret = np.zeros(18, np.uint64)
+    if load().blst_hip_p1s_mult_wbits_device(_ptr(ret), d_table_ptr, wbits, n, d_scalars_ptr, nbits) != 0:
+        raise RuntimeError("blst_hip_p1s_mult_wbits_device: " + last_error())
+    return ret
+
+
+class DeviceAlloc:
+    """Uninitialised-looking device memory of a given byte size (zeroed)."""
+
+    def __init__(self, nbytes):
+        self.buf = DeviceBuffer(np.zeros(nbytes, np.uint8))
+        self.ptr = self.buf.ptr
+        self.nbytes = nbytes
+
+    def read(self):
+        return self.buf.read()
+
+    def close(self):
+        self.buf.close()
 
 
 def p1s_tile_pippenger(points, scalars, nbits, bit0, window):

@@ -20,6 +20,7 @@
 #include <memory>
 #include <atomic>
 #include <mutex>
+#include <thread>
 #include <string>
 #include <vector>
 
@@ -41,6 +42,9 @@ struct Ctx {
     DevBuf pts, scl, dbg;
     DevBuf gtable, gen_tmp, gen_out;     // comb table of G, fixed-base scratch
     DevBuf jac_in, aff_out;              // batch normalisation staging
+    DevBuf wb_pts, wb_rows, wb_table, wb_list, wb_scl, wb_part, wb_W;   // fixed-base tables
+    void* wb_host = nullptr;             // page-locked gather staging
+    size_t wb_host_cap = 0;
     bool gtable_ready = false;
 };
 
@@ -737,6 +741,139 @@ void enqueue_to_affine(Ctx& cx, const void* d_in, void* d_out, size_t n) {
     HIPCK(hipGetLastError());
 }
 
+// ---------------------------------------------------------------- fixed-base tables
+
+constexpr size_t kWbitsRowEntries = (size_t)1 << 22;   // table entries per row-building launch
+constexpr size_t kWbitsSumLanes = 65536;
+constexpr size_t kWbitsListBytes = (size_t)256 << 20;   // gathered entries per host round trip
+
+bool valid_wbits(size_t wbits) { return wbits >= 2 && wbits <= 14; }
+
+// rows of n points (device, affine) into table (device, n * 2^(wbits-1)
+// affine entries)
+void enqueue_wbits_table(Ctx& cx, const void* d_pts, size_t n, size_t wbits, void* d_table) {
+    const size_t m = (size_t)1 << (wbits - 1);
+    const uint32_t L = (uint32_t)std::min<size_t>(m, 64);
+    const size_t R = std::max<size_t>(1, kWbitsRowEntries / m);
+    cx.wb_rows.ensure(std::min(n, R) * m * sizeof(xyzz));
+    for (size_t row0 = 0; row0 < n; row0 += R) {
+        const size_t nr = std::min(R, n - row0);
+        const size_t count = nr * m;
+        launch_k_wbits_rows(cx.stream, static_cast<const uint4*>(d_pts), (uint32_t)row0, (uint32_t)nr, (uint32_t)m, L,
+                            cx.wb_rows.as<xyzz>());
+        const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, count / kAffineLanes));
+        launch_k_to_affine(dim3(div_up(div_up(count, Q), 256)), dim3(256), cx.stream, cx.wb_rows.as<const xyzz>(),
+                           static_cast<aff*>(d_table) + row0 * m, (uint32_t)count, Q);
+        HIPCK(hipGetLastError());
+    }
+}
+
+// window sums S_w of n (window, point) entries, from a device table and
+// device scalars or from a gathered list (window-major, D * n entries),
+// added into W (D windows)
+void wbits_window_sums(Ctx& cx, const void* d_table, const void* d_scl, size_t nbits, const void* d_list, size_t n,
+                       size_t wbits, std::vector<hostec::xyzz6>& W) {
+    const size_t D = 1 + nbits / wbits;
+    const size_t nbytes = (nbits + 7) / 8;
+    const size_t C = std::min(n, std::max<size_t>(4, (n * D + kWbitsSumLanes - 1) / kWbitsSumLanes));
+    const size_t nch = (n + C - 1) / C;
+    cx.wb_part.ensure(D * nch * sizeof(xyzz));
+    cx.wb_W.ensure(D * sizeof(xyzz));
+    const uint32_t vec = (nbytes == 32 && ((uintptr_t)d_scl % 16) == 0) ? 1u : 0u;
+    launch_k_wbits_sum(cx.stream, static_cast<const uint4*>(d_table), static_cast<const uint8_t*>(d_scl),
+                       (uint32_t)nbytes, (uint32_t)nbits, vec, static_cast<const uint4*>(d_list), (uint32_t)n,
+                       (uint32_t)wbits, (uint32_t)D, (uint32_t)C, (uint32_t)nch, cx.wb_part.as<xyzz>());
+    launch_k_wbits_tree(cx.stream, cx.wb_part.as<const xyzz>(), (uint32_t)nch, (uint32_t)D, cx.wb_W.as<xyzz>());
+    HIPCK(hipGetLastError());
+    std::vector<hostec::xyzz6> part(D);
+    HIPCK(hipMemcpyAsync(part.data(), cx.wb_W.p, D * sizeof(xyzz), hipMemcpyDeviceToHost, cx.stream));
+    HIPCK(hipStreamSynchronize(cx.stream));
+    if (W.empty()) W.assign(D, hostec::xyzz6{});
+    for (size_t w = 0; w < D; w++) hostec::addp(W[w], W[w], part[w]);
+}
+
+// Booth digit of the window at bit0 (<= nbits) of width c, from the scalar's
+// bytes (the device's booth_digit_at)
+int32_t host_booth_digit(const uint8_t* s, uint32_t bit0, uint32_t c, uint32_t nbits) {
+    auto bits = [&](uint32_t pos, uint32_t len) -> uint32_t {
+        const uint32_t b0 = pos >> 3, b1 = (pos + len - 1) >> 3;
+        uint64_t v = 0;
+        for (uint32_t b = b0; b <= b1; b++) v |= (uint64_t)s[b] << (8 * (b - b0));
+        return (uint32_t)(v >> (pos & 7)) & ((1u << len) - 1u);
+    };
+    const uint32_t e = std::min(nbits - bit0 + 1, c);
+    const uint32_t end = std::min(bit0 + c, nbits);
+    const uint32_t wval = bit0 == 0 ? bits(0, end) << 1 : bits(bit0 - 1, end - bit0 + 1);
+    const uint32_t sign = (wval >> e) & 1u;
+    const int32_t val = (int32_t)((wval + 1) >> 1);
+    return sign ? val - (int32_t)(1u << e) : val;
+}
+
+void* wbits_host_stage(Ctx& cx, size_t bytes) {
+    if (bytes > cx.wb_host_cap) {
+        if (cx.wb_host) HIPCK(hipHostFree(cx.wb_host));
+        cx.wb_host = nullptr;
+        cx.wb_host_cap = 0;
+        HIPCK(hipHostMalloc(&cx.wb_host, bytes, hipHostMallocDefault));
+        cx.wb_host_cap = bytes;
+    }
+    return cx.wb_host;
+}
+
+// The host-table MSM: the host computes digits and gathers the signed
+// entries of a run of points for every window into page-locked memory (the
+// table stays where the caller keeps it), the device sums them.
+void host_mult_wbits(Ctx& cx, blst_p1* ret, const blst_p1_affine* table, size_t wbits, size_t n,
+                     const uint8_t* scalars, size_t nbits) {
+    const size_t D = 1 + nbits / wbits, nbytes = (nbits + 7) / 8, m = (size_t)1 << (wbits - 1);
+    const size_t run = std::max<size_t>(1, std::min(n, kWbitsListBytes / (D * sizeof(blst_p1_affine))));
+    uint8_t* stage = static_cast<uint8_t*>(wbits_host_stage(cx, run * D * sizeof(blst_p1_affine)));
+    cx.wb_list.ensure(run * D * sizeof(blst_p1_affine));
+    std::vector<hostec::xyzz6> W;
+    const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    for (size_t i0 = 0; i0 < n; i0 += run) {
+        const size_t cnt = std::min(run, n - i0);
+        auto gather = [&](size_t a, size_t b) {
+            for (size_t i = a; i < b; i++) {
+                const uint8_t* sc = scalars + (i0 + i) * nbytes;
+                const blst_p1_affine* row = table + (i0 + i) * m;
+                for (size_t w = 0; w < D; w++) {
+                    hostec::fp6* dst = reinterpret_cast<hostec::fp6*>(stage + (w * cnt + i) * sizeof(blst_p1_affine));
+                    const int32_t d = host_booth_digit(sc, (uint32_t)(w * wbits), (uint32_t)wbits, (uint32_t)nbits);
+                    if (d == 0) {
+                        std::memset(dst, 0, sizeof(blst_p1_affine));
+                        continue;
+                    }
+                    std::memcpy(dst, &row[(d < 0 ? -d : d) - 1], sizeof(blst_p1_affine));
+                    if (d < 0 && !hostec::is_zero(dst[1])) {
+                        hostec::fp6 zero{};
+                        hostec::sub(dst[1], zero, dst[1]);
+                    }
+                }
+            }
+        };
+        const size_t per = (cnt + nthreads - 1) / nthreads;
+        if (cnt * D < 4096 || nthreads == 1) {
+            gather(0, cnt);
+        } else {
+            std::vector<std::thread> th;
+            for (unsigned t = 0; t < nthreads; t++) {
+                const size_t a = t * per, b = std::min(cnt, a + per);
+                if (a < b) th.emplace_back(gather, a, b);
+            }
+            for (auto& t : th) t.join();
+        }
+        HIPCK(hipMemcpyAsync(cx.wb_list.p, stage, cnt * D * sizeof(blst_p1_affine), hipMemcpyHostToDevice,
+                             cx.stream));
+        wbits_window_sums(cx, nullptr, nullptr, nbits, cx.wb_list.p, cnt, wbits, W);
+    }
+    hostec::xyzz6 r;
+    hostec::fold_windows(r, W.data(), (int)D, (int)wbits);
+    hostec::jac6 j;
+    hostec::to_jac(j, r);
+    std::memcpy(ret, &j, sizeof(j));
+}
+
 }  // namespace host
 }  // namespace bls
 
@@ -1244,6 +1381,148 @@ int blst_hip_p1s_to_affine_device(void* device_dst, const void* device_points, s
         HIPCK(hipStreamSynchronize(cx.stream));
     } catch (const HipFail& f) {
         g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+size_t blst_p1s_mult_wbits_precompute_sizeof(size_t wbits, size_t npoints) {
+    if (!valid_wbits(wbits)) return 0;
+    size_t bytes;
+    if (__builtin_mul_overflow(npoints, sizeof(blst_p1_affine), &bytes)) return 0;
+    if (bytes > (SIZE_MAX >> (wbits - 1))) return 0;
+    return bytes << (wbits - 1);
+}
+
+size_t blst_p1s_mult_wbits_scratch_sizeof(size_t npoints) {
+    return 144 * std::min<size_t>(npoints, 8192);
+}
+
+void blst_p1s_mult_wbits_precompute(blst_p1_affine table[], size_t wbits, const blst_p1_affine* const points[],
+                                    size_t npoints) {
+    if (!valid_wbits(wbits) || (npoints != 0 && table == nullptr) ||
+        (npoints != 0 && (points == nullptr || points[0] == nullptr))) {
+        g_err = "invalid arguments";
+        return;
+    }
+    if (npoints == 0) { g_err.clear(); return; }
+    if (blst_p1s_mult_wbits_precompute_sizeof(wbits, npoints) == 0 || npoints >= 0x7fffffffULL) {
+        g_err = "table size out of range";
+        return;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        std::vector<uint8_t> stage;
+        const uint8_t* hp = resolve(points, npoints, sizeof(blst_p1_affine), stage);
+        const size_t m = (size_t)1 << (wbits - 1);
+        cx.wb_pts.ensure(npoints * sizeof(blst_p1_affine));
+        // every point is on the device before any row is written back: a
+        // packed input may occupy the table's tail
+        HIPCK(hipMemcpyAsync(cx.wb_pts.p, hp, npoints * sizeof(blst_p1_affine), hipMemcpyHostToDevice, cx.stream));
+        const size_t R = std::max<size_t>(1, kWbitsRowEntries / m);
+        cx.wb_table.ensure(std::min(npoints, R) * m * sizeof(blst_p1_affine));
+        for (size_t row0 = 0; row0 < npoints; row0 += R) {
+            const size_t nr = std::min(R, npoints - row0);
+            const uint8_t* rows = static_cast<const uint8_t*>(cx.wb_pts.p) + row0 * sizeof(blst_p1_affine);
+            enqueue_wbits_table(cx, rows, nr, wbits, cx.wb_table.p);
+            HIPCK(hipMemcpyAsync(table + row0 * m, cx.wb_table.p, nr * m * sizeof(blst_p1_affine),
+                                 hipMemcpyDeviceToHost, cx.stream));
+            HIPCK(hipStreamSynchronize(cx.stream));
+        }
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return;
+    }
+    g_err.clear();
+}
+
+void blst_p1s_mult_wbits(blst_p1* ret, const blst_p1_affine table[], size_t wbits, size_t npoints,
+                         const blst_byte* const scalars[], size_t nbits, blst_limb_t* scratch) {
+    if (npoints == 0 || nbits == 0) { std::memset(ret, 0, sizeof(*ret)); g_err.clear(); return; }
+    if (nbits > 256 || !valid_wbits(wbits) || table == nullptr || scalars == nullptr || scalars[0] == nullptr ||
+        scratch == nullptr) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "invalid arguments";
+        return;
+    }
+    if (blst_p1s_mult_wbits_precompute_sizeof(wbits, npoints) == 0 || npoints >= 0x7fffffffULL) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "table size out of range";
+        return;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        std::vector<uint8_t> stage;
+        const uint8_t* hs = resolve(scalars, npoints, (nbits + 7) / 8, stage);
+        host_mult_wbits(cx, ret, table, wbits, npoints, hs, nbits);
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        std::memset(ret, 0, sizeof(*ret));
+        return;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        std::memset(ret, 0, sizeof(*ret));
+        return;
+    }
+    g_err.clear();
+}
+
+int blst_hip_p1s_mult_wbits_precompute_device(void* device_table, size_t wbits, const void* device_points,
+                                              size_t npoints) {
+    if (npoints == 0) { g_err.clear(); return 0; }
+    if (!device_table || !device_points || !valid_wbits(wbits) || npoints >= 0x7fffffffULL ||
+        ((uintptr_t)device_table % 16) || ((uintptr_t)device_points % 16)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        const size_t m = (size_t)1 << (wbits - 1);
+        const size_t R = std::max<size_t>(1, kWbitsRowEntries / m);
+        for (size_t row0 = 0; row0 < npoints; row0 += R) {
+            const size_t nr = std::min(R, npoints - row0);
+            enqueue_wbits_table(cx, static_cast<const uint8_t*>(device_points) + row0 * sizeof(blst_p1_affine), nr,
+                                wbits, static_cast<uint8_t*>(device_table) + row0 * m * sizeof(blst_p1_affine));
+        }
+        HIPCK(hipStreamSynchronize(cx.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_p1s_mult_wbits_device(blst_p1* ret, const void* device_table, size_t wbits, size_t npoints,
+                                   const void* device_scalars, size_t nbits) {
+    if (!ret) { g_err = "invalid arguments"; return -1; }
+    std::memset(ret, 0, sizeof(*ret));
+    if (npoints == 0 || nbits == 0) { g_err.clear(); return 0; }
+    if (!device_table || !device_scalars || !valid_wbits(wbits) || nbits > 256 || npoints >= 0x7fffffffULL ||
+        ((uintptr_t)device_table % 16)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        std::vector<hostec::xyzz6> W;
+        wbits_window_sums(cx, device_table, device_scalars, nbits, nullptr, npoints, wbits, W);
+        hostec::xyzz6 r;
+        hostec::fold_windows(r, W.data(), (int)W.size(), (int)wbits);
+        hostec::jac6 j;
+        hostec::to_jac(j, r);
+        std::memcpy(ret, &j, sizeof(j));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        std::memset(ret, 0, sizeof(*ret));
         return -1;
     }
     g_err.clear();

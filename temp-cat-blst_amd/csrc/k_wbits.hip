@@ -1,0 +1,160 @@
+// Fixed-base tables (blst_p1s_mult_wbits_precompute / blst_p1s_mult_wbits,
+// blst/scalar/curve/multi_scalar.cpp:268-286, 481-707).
+//
+// A table holds, for each point P_i, the row [1]P_i ... [m]P_i (m =
+// 2^(wbits-1)) in affine form.  The MSM over a table needs no buckets: with
+// the Booth digits d_{i,w} of the scalars (windows of wbits bits),
+//   sum_i k_i P_i = sum_w 2^(wbits w) S_w,   S_w = sum_i sign(d) row_i[|d| - 1],
+// so every (window, point) pair is one gathered entry and one mixed
+// addition into its window's sum.
+//
+//   rows        L lanes per row, each lane one run of m / L consecutive
+//               multiples: [t (m/L) + 1]P by double-and-add, then + P per
+//               entry (XYZZ); k_to_affine then normalises the rows
+//   sum         one lane per (window, run of C points): gathers and mixed
+//               additions into one XYZZ partial sum
+//   tree        one workgroup per window: 256 quads fold the window's
+//               partials (four-lane additions, strides then a tree in LDS);
+//               the window sum is written canonical for the host's fold
+#include "msm_common.hip.h"
+#include "ec_quad.hip.h"
+#include "launch.h"
+
+namespace bls {
+
+__device__ __forceinline__ void load_aff_at(aff& a, const uint4* __restrict__ base, size_t idx) {
+    const uint4* s = base + idx * 6;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint4 u = s[k], v = s[k + 3];
+        a.x.l[4 * k] = u.x; a.x.l[4 * k + 1] = u.y; a.x.l[4 * k + 2] = u.z; a.x.l[4 * k + 3] = u.w;
+        a.y.l[4 * k] = v.x; a.y.l[4 * k + 1] = v.y; a.y.l[4 * k + 2] = v.z; a.y.l[4 * k + 3] = v.w;
+    }
+}
+
+// rows [row0, row0 + nrows) of the table, as XYZZ multiples: out[r * m + j]
+// = [j + 1] P_{row0 + r}; all-zero points give infinity entries.
+__global__ void __launch_bounds__(256)
+k_wbits_rows(const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint32_t m, uint32_t L,
+             xyzz* __restrict__ out) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nrows * L) return;
+    const uint32_t r = g / L, t = g % L, S = m / L;
+    aff P;
+    load_aff_at(P, pts, (size_t)row0 + r);
+    xyzz* dst = out + (size_t)r * m + (size_t)t * S;
+    xyzz acc;
+    xyzz_set_inf(acc);
+    if (aff_is_inf(P)) {
+        for (uint32_t s = 0; s < S; s++) store_xyzz(&dst[s], acc);
+        return;
+    }
+    // acc = [t S + 1] P, most significant bit first
+    const uint32_t k = t * S + 1;
+    for (int b = 31 - __builtin_clz(k); b >= 0; b--) {
+        xyzz_dbl_inl(acc, acc);
+        if ((k >> b) & 1u) xyzz_madd(acc, P);
+    }
+    store_xyzz(&dst[0], acc);
+    for (uint32_t s = 1; s < S; s++) {
+        xyzz_madd(acc, P);
+        store_xyzz(&dst[s], acc);
+    }
+}
+
+// Partial window sums.  Lane (w, c) adds the entries of points
+// [c C, min((c + 1) C, n)) in window w into part[w * nch + c].  With a table
+// (device memory, n rows of m entries) the lane computes the Booth digit of
+// each scalar itself; without one, list holds the D * n entries already
+// gathered and signed (window-major, all-zero = nothing to add).
+__global__ void __launch_bounds__(256)
+k_wbits_sum(const uint4* __restrict__ table, const uint8_t* __restrict__ scalars, uint32_t nbytes,
+            uint32_t nbits, uint32_t vec, const uint4* __restrict__ list, uint32_t n, uint32_t wbits, uint32_t D,
+            uint32_t C, uint32_t nch, xyzz* __restrict__ part) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= D * nch) return;
+    const uint32_t w = g / nch, c = g % nch;
+    const uint32_t i0 = c * C, i1 = min(i0 + C, n);
+    const size_t m = (size_t)1 << (wbits - 1);
+    xyzz acc;
+    xyzz_set_inf(acc);
+    for (uint32_t i = i0; i < i1; i++) {
+        aff a;
+        if (table) {
+            const int32_t d = booth_digit_at(scalars + (size_t)i * nbytes, vec != 0, w * wbits, wbits, nbits);
+            if (d == 0) continue;
+            load_aff_at(a, table, (size_t)i * m + (size_t)(d < 0 ? -d : d) - 1);
+            if (aff_is_inf(a)) continue;
+            if (d < 0) fp_neg(a.y, a.y);
+        } else {
+            load_aff_at(a, list, (size_t)w * n + i);
+            if (aff_is_inf(a)) continue;
+        }
+        xyzz_madd(acc, a);
+    }
+    store_xyzz(&part[g], acc);
+}
+
+constexpr uint32_t kWbitsTreeThreads = 1024;
+constexpr uint32_t kWbitsTreeQuads = kWbitsTreeThreads / 4;
+
+// Window w's nch partials -> W[w] (canonical XYZZ); one workgroup per window.
+__global__ void __launch_bounds__(kWbitsTreeThreads)
+k_wbits_tree(const xyzz* __restrict__ part, uint32_t nch, xyzz* __restrict__ W) {
+    __shared__ xyzz node[kWbitsTreeQuads];
+    const uint32_t w = blockIdx.x;
+    const uint32_t u = threadIdx.x >> 2, ln = threadIdx.x & 3;
+    const xyzz* src = part + (size_t)w * nch;
+    fp acc;
+    fp_zero(acc);
+    for (uint32_t j = u; j < nch; j += kWbitsTreeQuads) {
+        fp x;
+        quad_load(x, &src[j]);
+        quad_add(acc, acc, x);
+    }
+    reinterpret_cast<fp*>(&node[u])[ln] = acc;
+    __syncthreads();
+    // pairwise tree over the quads that hold partials; at least four quads
+    // (16 lanes) run each level, the extra ones repeating a merge without
+    // storing it
+    uint32_t width = 1;
+    while (width < min(nch, kWbitsTreeQuads)) width <<= 1;
+    for (uint32_t half = width / 2; half >= 1; half >>= 1) {
+        fp r;
+        if (u < max(half, 4u)) {
+            const uint32_t v = u % half;
+            const fp a = reinterpret_cast<const fp*>(&node[v])[ln];
+            const fp b = reinterpret_cast<const fp*>(&node[v + half])[ln];
+            quad_add(r, a, b);
+        }
+        __syncthreads();
+        if (u < half) reinterpret_cast<fp*>(&node[u])[ln] = r;
+        __syncthreads();
+    }
+    if (u == 0) {
+        fp r = reinterpret_cast<const fp*>(&node[0])[ln];
+        fp_canon(r, r);
+        quad_store(&W[w], r);
+    }
+}
+
+void launch_k_wbits_rows(hipStream_t stream, const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint32_t m,
+                         uint32_t L, xyzz* __restrict__ out) {
+    const uint32_t lanes = nrows * L;
+    hipLaunchKernelGGL(k_wbits_rows, dim3((lanes + 255) / 256), dim3(256), 0, stream, pts, row0, nrows, m, L, out);
+}
+
+void launch_k_wbits_sum(hipStream_t stream, const uint4* __restrict__ table, const uint8_t* __restrict__ scalars,
+                        uint32_t nbytes, uint32_t nbits, uint32_t vec, const uint4* __restrict__ list, uint32_t n,
+                        uint32_t wbits, uint32_t D, uint32_t C, uint32_t nch, xyzz* __restrict__ part) {
+    const uint32_t lanes = D * nch;
+    hipLaunchKernelGGL(k_wbits_sum, dim3((lanes + 255) / 256), dim3(256), 0, stream, table, scalars, nbytes, nbits,
+                       vec, list, n, wbits, D, C, nch, part);
+}
+
+void launch_k_wbits_tree(hipStream_t stream, const xyzz* __restrict__ part, uint32_t nch, uint32_t D,
+                         xyzz* __restrict__ W) {
+    hipLaunchKernelGGL(k_wbits_tree, dim3(D), dim3(kWbitsTreeThreads), 0, stream, part, nch, W);
+}
+
+}  // namespace bls

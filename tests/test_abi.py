@@ -183,3 +183,38 @@ def test_fr_and_jacobian_generators(orc):
     assert np.array_equal(orc.p1s_to_affine(jac), aff)
     assert all(j[12:].any() for j, a in zip(jac, aff) if a.any())
     assert not jac[::17].any()
+
+
+def test_wbits_sizes_and_checks_without_gpu_work(orc):
+    """Sizes of the fixed-base entry points against the oracle, and the
+    argument checks of c_api.cpp:1318-1356 that return before any device call."""
+    import blst_hip
+    L = blst_hip.load()
+    for wb in range(0, 17):
+        for n in (0, 1, 5, 4096, 1 << 20, (1 << 64) - 1, (1 << 50)):
+            assert L.blst_p1s_mult_wbits_precompute_sizeof(wb, n) == orc.wbits_precompute_sizeof(wb, n), (wb, n)
+    for n in (0, 1, 100, 8191, 8192, 8193, 1 << 30):
+        assert L.blst_p1s_mult_wbits_scratch_sizeof(n) == orc.wbits_scratch_sizeof(n)
+    sc = np.zeros((2, 32), np.uint8)
+    d = blst_hip.Descriptors(np.zeros((2, 12), np.uint64), sc)
+    tab = np.zeros((4, 12), np.uint64)
+    scr = np.zeros(64, np.uint64)
+    ret = np.full(18, 7, np.uint64)
+    P = blst_hip._ptr
+    L.blst_p1s_mult_wbits(P(ret), P(tab), 2, 0, P(d.ss), 255, P(scr))
+    assert not ret.any()
+    for args in ((P(tab), 2, 2, P(d.ss), 0, P(scr)), (P(tab), 2, 2, P(d.ss), 257, P(scr)),
+                 (P(tab), 1, 2, P(d.ss), 255, P(scr)), (P(tab), 15, 2, P(d.ss), 255, P(scr)),
+                 (None, 2, 2, P(d.ss), 255, P(scr)), (P(tab), 2, 2, None, 255, P(scr)),
+                 (P(tab), 2, 2, P(np.zeros(2, np.uint64)), 255, P(scr)), (P(tab), 2, 2, P(d.ss), 255, None)):
+        ret[:] = 7
+        L.blst_p1s_mult_wbits(P(ret), *args)
+        assert not ret.any(), args
+    # precompute: invalid wbits, NULL table or head write nothing
+    pts = np.ones((2, 12), np.uint64)
+    dp = blst_hip.Descriptors(pts, sc)
+    tab[:] = 7
+    for args in ((P(tab), 1, P(dp.pp), 2), (P(tab), 15, P(dp.pp), 2), (None, 2, P(dp.pp), 2),
+                 (P(tab), 2, None, 2), (P(tab), 2, P(np.zeros(2, np.uint64)), 2), (P(tab), 2, P(dp.pp), 0)):
+        L.blst_p1s_mult_wbits_precompute(*args)
+        assert (tab == 7).all(), args

@@ -506,3 +506,105 @@ def test_to_affine_device_2pow20_round_trip(hip, orc):
         d_out.close()
     assert np.array_equal(got, aff)
     assert np.array_equal(hip.p1s_to_affine(jac), aff)
+
+
+# ---------------------------------------------------------------- fixed-base tables
+
+@pytest.mark.parametrize("wbits", [2, 3, 4, 8, 10, 14])
+def test_wbits_table_matches_oracle(hip, orc, wbits):
+    n = 3 if wbits == 14 else 40
+    pts = orc.gen_points(n, 0x7B + wbits, threads=8)
+    pts[1] = 0
+    want = orc.wbits_precompute(pts, wbits)
+    assert np.array_equal(hip.wbits_precompute(pts, wbits), want)
+
+
+@pytest.mark.parametrize("wbits", [2, 3, 5, 8, 11, 14])
+@pytest.mark.parametrize("nbits", [1, 8, 64, 255, 256])
+def test_wbits_msm_matches_oracle(hip, orc, wbits, nbits):
+    n = 3 if wbits == 14 else 37
+    rnd = random.Random(wbits * 1000 + nbits)
+    pts = orc.gen_points(n, 0x5A + wbits, threads=8)
+    pts[2] = 0
+    nb = (nbits + 7) // 8
+    S = _scal([rnd.getrandbits(8 * nb) for _ in range(n)], nb)   # bits above nbits are ignored
+    S[0] = 0
+    tab = orc.wbits_precompute(pts, wbits)
+    want = orc.compress(orc.mult_wbits(tab, wbits, S, n, nbits))
+    assert want == orc.compress(orc.msm_pippenger(pts, S, n, nbits))
+    assert hip.compress(hip.mult_wbits(tab, wbits, S, nbits)) == want
+
+
+def test_wbits_known_answers(hip, orc):
+    """C_api_tests.cpp:1606-1690: nbits 8, wbits 4 and 10."""
+    g = orc.from_affine(orc.generator_affine())
+    for ks, sc, wb, total in (([3, 5, 7], [2, 3, 5], 4, 56), ([3, 5, 7, 9, 11], [2, 3, 5, 7, 11], 10, 240)):
+        pts = np.stack([orc.to_affine(orc.p1_mult(g, k.to_bytes(32, "little"), 255)) for k in ks])
+        tab = hip.wbits_precompute(pts, wb)
+        r = hip.mult_wbits(tab, wb, np.array(sc, np.uint8).reshape(-1, 1), 8)
+        assert hip.compress(r) == pyref.compress(pyref.mul(total, pyref.G))
+
+
+@pytest.mark.parametrize("mode", ["exact", "mixed"])
+def test_wbits_descriptors_and_packed_tail(hip, orc, mode):
+    n, wbits = 21, 5
+    pts = orc.gen_points(n, 0x33, threads=8)
+    sc = orc.gen_scalars(n, 0x34)
+    want_tab = orc.wbits_precompute(pts, wbits)
+    assert np.array_equal(hip.wbits_precompute(pts, wbits, mode=mode, k=6), want_tab)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    assert hip.compress(hip.mult_wbits(want_tab, wbits, sc, 255, mode=mode, k=6)) == want
+    # the points packed into the table's last n entries (an allowed overlap)
+    L = hip.load()
+    m = 1 << (wbits - 1)
+    table = np.zeros((n * m, 12), np.uint64)
+    table[n * m - n:] = pts
+    heads = np.array([table[n * m - n:].ctypes.data, 0], np.uint64)
+    L.blst_p1s_mult_wbits_precompute(hip._ptr(table), wbits, hip._ptr(heads), n)
+    assert hip.last_error() == ""
+    assert np.array_equal(table, want_tab)
+
+
+def test_wbits_device_resident_4096(hip, orc):
+    """A resident 4096-point setup: the device table's rows equal the
+    oracle's (sampled) and the device MSM over it equals the plain MSM."""
+    n, wbits = 4096, 10
+    m = 1 << (wbits - 1)
+    pts = hip.generator_multiples(hip.synth_srs_scalars(n, 0x44))
+    sc = hip.synth_scalars(n, 0x45)
+    d_pts = hip.DeviceBuffer(pts)
+    d_tab = hip.DeviceAlloc(n * m * 96)
+    d_sc = hip.DeviceBuffer(sc)
+    try:
+        hip.wbits_precompute_device(d_tab.ptr, wbits, d_pts.ptr, n)
+        got = hip.mult_wbits_device(d_tab.ptr, wbits, n, d_sc.ptr, 255)
+        plain = hip.msm_device(d_pts.ptr, d_sc.ptr, n, 255)
+        tab = d_tab.read().view(np.uint64).reshape(n * m, 12)
+    finally:
+        d_pts.close()
+        d_tab.close()
+        d_sc.close()
+    assert hip.compress(got) == hip.compress(plain)
+    for i in (0, 1, 2047, 4095):
+        assert np.array_equal(tab[i * m:(i + 1) * m], orc.wbits_precompute(pts[i:i + 1], wbits))
+    assert hip.compress(hip.mult_wbits(tab, wbits, sc, 255)) == hip.compress(plain)
+
+
+def test_wbits_device_2pow16_property(hip):
+    """At 2^16 points with an 8-bit table the device MSM equals the plain MSM
+    (a size-independent check against the path pinned above)."""
+    n, wbits = 1 << 16, 8
+    pts = hip.synth_points(n, 0x61)
+    sc = hip.synth_scalars(n, 0x62)
+    d_pts = hip.DeviceBuffer(pts)
+    d_tab = hip.DeviceAlloc(n * (1 << (wbits - 1)) * 96)
+    d_sc = hip.DeviceBuffer(sc)
+    try:
+        hip.wbits_precompute_device(d_tab.ptr, wbits, d_pts.ptr, n)
+        got = hip.mult_wbits_device(d_tab.ptr, wbits, n, d_sc.ptr, 255)
+        plain = hip.msm_device(d_pts.ptr, d_sc.ptr, n, 255)
+    finally:
+        d_pts.close()
+        d_tab.close()
+        d_sc.close()
+    assert hip.compress(got) == hip.compress(plain)

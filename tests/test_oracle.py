@@ -355,3 +355,65 @@ def test_batch_to_affine_against_textbook(orc):
     for j, l in zip(orc.p1s_from_affine_scaled(aff, lam), lam):
         if j[12:].any():
             assert np.array_equal(j[12:], l)
+
+
+def test_wbits_table_rows_against_textbook(orc):
+    """Rows of point_precompute_row (multi_scalar.cpp:268-286) hold
+    [1]P ... [2^(wbits-1)]P in affine; an all-zero point gives all-zero rows."""
+    pts = orc.gen_points(4, 3, threads=1)
+    pts[2] = 0
+    for wb in (2, 3, 4, 6):
+        tab = orc.wbits_precompute(pts, wb)
+        m = 1 << (wb - 1)
+        for i in range(4):
+            P = pyref.limbs_to_affine(list(pts[i])) if pts[i].any() else None
+            for j in range(m):
+                e = tab[i * m + j]
+                if P is None:
+                    assert not e.any()
+                else:
+                    assert pyref.limbs_to_affine(list(e)) == pyref.mul(j + 1, P)
+
+
+def test_wbits_msm_known_answers_and_textbook(orc):
+    """C_api_tests.cpp:1606-1690: 2*[3]G + 3*[5]G + 5*[7]G at wbits 4 and
+    the five-point case at wbits 10, nbits 8; then textbook sums over nbits
+    and wbits, and agreement with the oracle's Pippenger."""
+    def mults(ks):
+        return np.stack([orc.to_affine(orc.p1_mult(orc.from_affine(orc.generator_affine()),
+                                                    int(k).to_bytes(32, "little"), 255)) for k in ks])
+    for ks, sc, wb, total in (([3, 5, 7], [2, 3, 5], 4, 56), ([3, 5, 7, 9, 11], [2, 3, 5, 7, 11], 10, 240)):
+        pts = mults(ks)
+        r = orc.mult_wbits(orc.wbits_precompute(pts, wb), wb, np.array(sc, np.uint8).reshape(-1, 1), len(ks), 8)
+        assert orc.compress(r) == pyref.compress(pyref.mul(total, pyref.G))
+    rnd = random.Random(17)
+    pts = orc.gen_points(6, 19, threads=1)
+    pts[4] = 0
+    for nbits in (1, 2, 5, 8, 13, 64, 255, 256):
+        nb = (nbits + 7) // 8
+        ks = [rnd.getrandbits(8 * nb) for _ in range(6)]
+        ks[1] = 0
+        S = np.array([list(k.to_bytes(nb, "little")) for k in ks], np.uint8)
+        for wb in (2, 3, 5, 8, 14):
+            if wb == 14 and nbits < 64:
+                continue
+            r = orc.mult_wbits(orc.wbits_precompute(pts, wb), wb, S, 6, nbits)
+            want = None
+            for k, p in zip(ks, pts):
+                if p.any():
+                    want = pyref.add(want, pyref.mul(k % (1 << nbits), pyref.limbs_to_affine(list(p))))
+            assert orc.compress(r) == pyref.compress(want), (nbits, wb)
+            assert orc.compress(r) == orc.compress(orc.msm_pippenger(pts, S, 6, nbits))
+
+
+def test_wbits_sizes(orc):
+    """C_api_tests.cpp:2871-2873 and the closed forms of
+    multi_scalar.h:244-256, 310-315."""
+    assert orc.wbits_precompute_sizeof(0, 4) == 0
+    assert orc.wbits_precompute_sizeof(15, 4) == 0
+    assert orc.wbits_precompute_sizeof(14, (1 << 64) - 1) == 0
+    assert orc.wbits_precompute_sizeof(1, 4) == 0
+    assert orc.wbits_precompute_sizeof(2, 4) == 96 * 4 * 2
+    assert orc.wbits_precompute_sizeof(14, 4096) == 96 * 4096 * 8192
+    assert orc.wbits_scratch_sizeof(0) == 0 and orc.wbits_scratch_sizeof(3) == 3 * 144
+    assert orc.wbits_scratch_sizeof(8192) == orc.wbits_scratch_sizeof(1 << 20) == 8192 * 144
