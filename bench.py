@@ -61,10 +61,11 @@ def load_traffic():
     return None
 
 
-def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None):
+def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None, wbits_case=None):
     """The oracle (C port of blst_p1s_mult_pippenger) on the host cores, and
     the checker of sampled batched results; with jac, also the oracle's batch
-    normalisation on a 2^18-point sample of it."""
+    normalisation on a 2^18-point sample of it; with wbits_case, the oracle's
+    table MSM on the same table and scalars."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     import blst_hip as bh
@@ -94,6 +95,15 @@ def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None):
         aff_rep = {"value": m / da, "unit": "points/s", "cores": 1, "kind": "port",
                    "sample": "oracle points_to_affine over the first %d Jacobian points, 1 thread, %.3f s" % (m, da),
                    "equals_input": bool(np.array_equal(a, points[:m]))}
+    wb_rep = None
+    if wbits_case is not None:
+        table, wb, ssc, gpu_wb = wbits_case
+        t = time.perf_counter()
+        rw = oracle.mult_wbits(table, wb, ssc, ssc.shape[0], 255)
+        dw = time.perf_counter() - t
+        wb_rep = {"value": ssc.shape[0] / dw, "unit": "points/s", "cores": 1, "kind": "port",
+                  "sample": "oracle msm_mult_table over the 4096-point wbits = 10 table, 1 thread, %.3f s" % dw,
+                  "bit_exact_vs_gpu": oracle.compress(rw) == gpu_wb}
     return {
         "value": n / dt,
         "unit": "points/s",
@@ -106,6 +116,7 @@ def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None):
         "compressed": cpu_bytes.hex(),
         "batched_samples_bit_exact": checked,
         "p1s_to_affine": aff_rep,
+        "p1s_mult_wbits": wb_rep,
     }
 
 
@@ -145,7 +156,46 @@ def entry_points(bh, points, scalars, r, n, reps=5):
     for _ in range(reps):
         aff = bh.p1s_to_affine(jac)
     host_s = (time.perf_counter() - t) / reps
+    # a resident 4096-point setup and its wbits = 10 table in HBM
+    sn, wb = 4096, 10
+    srs = bh.generator_multiples(bh.synth_srs_scalars(sn, SEED ^ 0x20))
+    ssc = bh.synth_scalars(sn, SEED ^ 0x21)
+    d_srs, d_ssc = bh.DeviceBuffer(srs), bh.DeviceBuffer(ssc)
+    d_tab = bh.DeviceAlloc(sn * (1 << (wb - 1)) * 96)
+    try:
+        t = time.perf_counter()
+        bh.wbits_precompute_device(d_tab.ptr, wb, d_srs.ptr, sn)
+        pre_s = time.perf_counter() - t
+        rw = bh.mult_wbits_device(d_tab.ptr, wb, sn, d_ssc.ptr, 255)
+        t = time.perf_counter()
+        for _ in range(4 * reps):
+            rw = bh.mult_wbits_device(d_tab.ptr, wb, sn, d_ssc.ptr, 255)
+        tab_s = (time.perf_counter() - t) / (4 * reps)
+        rpl = bh.msm_device(d_srs.ptr, d_ssc.ptr, sn, 255)
+        t = time.perf_counter()
+        for _ in range(4 * reps):
+            rpl = bh.msm_device(d_srs.ptr, d_ssc.ptr, sn, 255)
+        pl_s = (time.perf_counter() - t) / (4 * reps)
+        table = d_tab.read().view(np.uint64).reshape(-1, 12)
+    finally:
+        d_srs.close()
+        d_ssc.close()
+        d_tab.close()
+    rh = bh.mult_wbits(table, wb, ssc, 255)
+    t = time.perf_counter()
+    for _ in range(reps):
+        rh = bh.mult_wbits(table, wb, ssc, 255)
+    host_tab_s = (time.perf_counter() - t) / reps
+    wb_same = bh.compress(rw) == bh.compress(rpl) == bh.compress(rh)
     rep = {
+        "p1s_mult_wbits": {
+            "workload": "resident 4096-point KZG setup, wbits = 10 table (201 MB) in HBM, one MSM of uniform Fr scalars",
+            "precompute_device_ms": pre_s * 1e3,
+            "device_ms": tab_s * 1e3, "device_points_per_s": sn / tab_s,
+            "plain_pippenger_device_ms": pl_s * 1e3,
+            "host_table_ms": host_tab_s * 1e3,
+            "equals_plain": wb_same,
+        },
         "p1s_mult_pippenger_fr": {
             "workload": "the same n points and scalars as blst_fr (Montgomery form), host buffers in and out",
             "points_per_s": n / fr_s, "ms": fr_s * 1e3,
@@ -159,7 +209,7 @@ def entry_points(bh, points, scalars, r, n, reps=5):
             "equals_input": dev_same and np.array_equal(aff, points),
         },
     }
-    return rep, jac
+    return rep, jac, (table, wb, ssc, bh.compress(rw))
 
 
 def copy_calibration(nbytes, reps=3):
@@ -300,13 +350,13 @@ def main():
     out["compressed"] = gpu_bytes.hex()
     dp.close()
     ds.close()
-    jac = None
+    jac = wbits_case = None
     if not args.no_entry_points:
-        ep, jac = entry_points(bh, points, scalars, r, n)
+        ep, jac, wbits_case = entry_points(bh, points, scalars, r, n)
         out["entry_points"] = ep
-        print("fr: %.2f ms host, to_affine: %.3f ms device" % (ep["p1s_mult_pippenger_fr"]["ms"],
-                                                               ep["p1s_to_affine"]["device_ms"]),
-              file=sys.stderr, flush=True)
+        print("fr: %.2f ms host, to_affine: %.3f ms device, wbits: %.3f ms device" % (
+            ep["p1s_mult_pippenger_fr"]["ms"], ep["p1s_to_affine"]["device_ms"], ep["p1s_mult_wbits"]["device_ms"]),
+            file=sys.stderr, flush=True)
 
     import workloads
 
@@ -327,13 +377,15 @@ def main():
                                                            sv["speedup"]), file=sys.stderr, flush=True)
         out["serving"] = sv
     if not args.no_cpu:
-        cb = cpu_baseline(points, scalars, n, gpu_bytes, samples, jac)
+        cb = cpu_baseline(points, scalars, n, gpu_bytes, samples, jac, wbits_case)
         out["cpu_baseline"] = cb
         out["bit_exact"] = cb["bit_exact_vs_gpu"] and all(cb["batched_samples_bit_exact"].values())
         if "entry_points" in out:
             out["bit_exact"] = (out["bit_exact"] and out["entry_points"]["p1s_mult_pippenger_fr"]["equals_plain"]
                                 and out["entry_points"]["p1s_to_affine"]["equals_input"]
-                                and cb["p1s_to_affine"]["equals_input"])
+                                and out["entry_points"]["p1s_mult_wbits"]["equals_plain"]
+                                and cb["p1s_to_affine"]["equals_input"]
+                                and cb["p1s_mult_wbits"]["bit_exact_vs_gpu"])
     print(json.dumps(out), flush=True)
 
 
