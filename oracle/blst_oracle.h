@@ -81,6 +81,23 @@ void or_gen_points(uint64_t *out_affine, size_t n, uint64_t seed, int threads);
 void or_gen_srs(uint64_t *out_affine, size_t n, uint64_t seed, int threads);
 void or_apply_edge_cases(uint64_t *points, uint8_t *scalars32, size_t n, uint64_t seed);
 
+/* G2 over Fp2 (blst_oracle_g2.c): Fp2 = (c0, c1) of 6 limbs each; affine 24
+ * limbs, Jacobian 36 limbs; scalars as for G1 */
+void or_fp2_mul(uint64_t r[12], const uint64_t a[12], const uint64_t b[12]);
+void or_p2_generator(uint64_t out_affine[24]);
+void or_p2_from_affine(uint64_t out[36], const uint64_t a[24]);
+void or_p2_add(uint64_t out[36], const uint64_t a[36], const uint64_t b[36]);
+void or_p2_double(uint64_t out[36], const uint64_t a[36]);
+void or_p2_to_affine(uint64_t out[24], const uint64_t p[36]);
+void or_p2_mult(uint64_t out[36], const uint64_t p[36], const uint8_t *scalar, size_t nbits);
+void or_p2s_to_affine(uint64_t *out_affine, const uint64_t *jacobian, size_t n);
+void or_p2_compress(uint8_t out[96], const uint64_t p[36]);
+void or_msm_p2_tile(uint64_t ret[36], const uint64_t *points, const uint8_t *scalars, size_t npoints,
+                    size_t nbits, size_t bit0, size_t window);
+void or_msm_p2_pippenger(uint64_t ret[36], const uint64_t *points, const uint8_t *scalars, size_t npoints,
+                         size_t nbits, size_t window);
+void or_gen_p2_points(uint64_t *out_affine, size_t n, uint64_t seed);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,18 @@ def lib():
             "or_fr_to_scalar": [_u8p, _u64p],
             "or_frs_to_scalars": [_u8p, _u64p, _sz],
             "or_wbits_precompute": [_u64p, _sz, _u64p, _sz],
+            "or_fp2_mul": [_u64p, _u64p, _u64p],
+            "or_p2_generator": [_u64p],
+            "or_p2_from_affine": [_u64p, _u64p],
+            "or_p2_add": [_u64p, _u64p, _u64p],
+            "or_p2_double": [_u64p, _u64p],
+            "or_p2_to_affine": [_u64p, _u64p],
+            "or_p2_mult": [_u64p, _u64p, _u8p, _sz],
+            "or_p2s_to_affine": [_u64p, _u64p, _sz],
+            "or_p2_compress": [_u8p, _u64p],
+            "or_msm_p2_tile": [_u64p, _u64p, _u8p, _sz, _sz, _sz, _sz],
+            "or_msm_p2_pippenger": [_u64p, _u64p, _u8p, _sz, _sz, _sz],
+            "or_gen_p2_points": [_u64p, _sz, ctypes.c_uint64],
             "or_mult_wbits": [_u64p, _u64p, _sz, _sz, _u8p, _sz],
         }
         for name, args in sigs.items():
@@ -303,6 +315,83 @@ def mult_wbits(table, wbits, scalars, n, nbits):
     sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
     lib().or_mult_wbits(u64p(r), u64p(tab), wbits, n, u8p(sc), nbits)
     return r
+
+
+# ---- G2 ----------------------------------------------------------------------
+
+def fp2_mul(a, b):
+    r = np.zeros(12, np.uint64)
+    lib().or_fp2_mul(u64p(r), u64p(np.ascontiguousarray(a, dtype=np.uint64)),
+                     u64p(np.ascontiguousarray(b, dtype=np.uint64)))
+    return r
+
+
+def p2_generator_affine():
+    g = np.zeros(24, np.uint64)
+    lib().or_p2_generator(u64p(g))
+    return g
+
+
+def p2_from_affine(a):
+    r = np.zeros(36, np.uint64)
+    lib().or_p2_from_affine(u64p(r), u64p(np.ascontiguousarray(a, dtype=np.uint64)))
+    return r
+
+
+def p2_add(a, b):
+    r = np.zeros(36, np.uint64)
+    lib().or_p2_add(u64p(r), u64p(np.ascontiguousarray(a)), u64p(np.ascontiguousarray(b)))
+    return r
+
+
+def p2_mult(p, scalar_bytes, nbits):
+    r = np.zeros(36, np.uint64)
+    s = np.frombuffer(bytes(scalar_bytes), dtype=np.uint8).copy()
+    lib().or_p2_mult(u64p(r), u64p(np.ascontiguousarray(p)), u8p(s), nbits)
+    return r
+
+
+def p2_to_affine(p):
+    r = np.zeros(24, np.uint64)
+    lib().or_p2_to_affine(u64p(r), u64p(np.ascontiguousarray(p)))
+    return r
+
+
+def p2s_to_affine(jac):
+    jac = np.ascontiguousarray(jac, dtype=np.uint64)
+    out = np.zeros((jac.shape[0], 24), np.uint64)
+    if jac.shape[0]:
+        lib().or_p2s_to_affine(u64p(out.reshape(-1)), u64p(jac.reshape(-1)), jac.shape[0])
+    return out
+
+
+def p2_compress(p):
+    out = np.zeros(96, np.uint8)
+    lib().or_p2_compress(u8p(out), u64p(np.ascontiguousarray(p, dtype=np.uint64)))
+    return out.tobytes()
+
+
+def msm_p2_pippenger(points, scalars, n, nbits, window=0):
+    r = np.zeros(36, np.uint64)
+    pts = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
+    lib().or_msm_p2_pippenger(u64p(r), u64p(pts), u8p(sc), n, nbits, window)
+    return r
+
+
+def msm_p2_tile(points, scalars, n, nbits, bit0, window):
+    r = np.zeros(36, np.uint64)
+    pts = np.ascontiguousarray(points, dtype=np.uint64).reshape(-1)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
+    lib().or_msm_p2_tile(u64p(r), u64p(pts), u8p(sc), n, nbits, bit0, window)
+    return r
+
+
+def gen_p2_points(n, seed):
+    out = np.zeros((n, 24), np.uint64)
+    if n:
+        lib().or_gen_p2_points(u64p(out.reshape(-1)), n, seed)
+    return out
 
 
 # ---- synthetic inputs ------------------------------------------------------

@@ -13,7 +13,10 @@ Run in the build container (it reads /root/reference as data, never runs it):
   h2c_g1.json    the G1 outputs P of the hash-to-curve vector files
                  (bindings/vectors/hash_to_curve/BLS12381G1_*.json): known
                  subgroup points with their compressed encodings
-  kat.json       the compressed generator of test/Conformance_tests.cpp:15-17
+  h2c_g2.json    the G2 outputs P of the G2 hash-to-curve vector files
+                 (bindings/vectors/hash_to_curve/BLS12381G2_*.json), with
+                 compressed encodings by the textbook model of pyref2.py
+  kat.json       the compressed generators of test/Conformance_tests.cpp:15-21
                  (draft-irtf-cfrg-pairing-friendly-curves-13, B.1)
 """
 import json
@@ -23,6 +26,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
 import pyref  # noqa: E402
+import pyref2  # noqa: E402
 
 REF = "/root/reference"
 
@@ -85,6 +89,22 @@ def h2c_points():
     return out
 
 
+def h2c_points_g2():
+    out = []
+    d = os.path.join(REF, "bindings/vectors/hash_to_curve")
+    for fn in sorted(os.listdir(d)):
+        if not fn.startswith("BLS12381G2_") or not fn.endswith(".json"):
+            continue
+        js = json.load(open(os.path.join(d, fn)))
+        for v in js["vectors"]:
+            x = tuple(int(c, 16) for c in v["P"]["x"].split(","))
+            y = tuple(int(c, 16) for c in v["P"]["y"].split(","))
+            assert pyref2.on_curve((x, y))
+            out.append({"suite": fn, "msg": v["msg"], "x": ["%096x" % c for c in x], "y": ["%096x" % c for c in y],
+                        "compressed": pyref2.compress((x, y)).hex()})
+    return out
+
+
 def main():
     fz = fuzz_cases()
     with open(os.path.join(HERE, "fuzz_msm.json"), "w") as f:
@@ -92,16 +112,24 @@ def main():
     hc = h2c_points()
     with open(os.path.join(HERE, "h2c_g1.json"), "w") as f:
         json.dump(hc, f, indent=1)
+    hc2 = h2c_points_g2()
+    with open(os.path.join(HERE, "h2c_g2.json"), "w") as f:
+        json.dump(hc2, f, indent=1)
     kat = {
         "g1_generator_compressed": "97f1d3a73197d7942695638c4fa9ac0fc3688c4f9774b905a14e3a3f171bac5"
                                    "86c55e83ff97a1aeffb3af00adb22c6bb",
         "g1_infinity_compressed": "c0" + "00" * 47,
-        "source": "test/Conformance_tests.cpp:15-17 (draft-irtf-cfrg-pairing-friendly-curves-13, "
+        "g2_generator_compressed": "93e02b6052719f607dacd3a088274f65596bd0d09920b61ab5da61bbdc7f504"
+                                   "9334cf11213945d57e5ac7d055d042b7e024aa2b2f08f0a91260805272dc510"
+                                   "51c6e47ad4fa403b02b4510b647ae3d1770bac0326a805bbefd48056c8c121b"
+                                   "db8",
+        "g2_infinity_compressed": "c0" + "00" * 95,
+        "source": "test/Conformance_tests.cpp:15-21 (draft-irtf-cfrg-pairing-friendly-curves-13, "
                   "Appendix B.1); infinity: blst/scalar/curve/point.cpp:895-898",
     }
     with open(os.path.join(HERE, "kat.json"), "w") as f:
         json.dump(kat, f, indent=1)
-    print("fuzz cases %d, h2c points %d" % (len(fz), len(hc)))
+    print("fuzz cases %d, h2c points %d + %d" % (len(fz), len(hc), len(hc2)))
 
 
 if __name__ == "__main__":

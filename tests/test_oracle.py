@@ -417,3 +417,92 @@ def test_wbits_sizes(orc):
     assert orc.wbits_precompute_sizeof(14, 4096) == 96 * 4096 * 8192
     assert orc.wbits_scratch_sizeof(0) == 0 and orc.wbits_scratch_sizeof(3) == 3 * 144
     assert orc.wbits_scratch_sizeof(8192) == orc.wbits_scratch_sizeof(1 << 20) == 8192 * 144
+
+
+# ---------------------------------------------------------------- G2
+
+import pyref2  # noqa: E402
+
+
+def g2_limbs(pt):
+    return np.array(pyref2.affine_to_limbs(pt), dtype=np.uint64)
+
+
+def test_fp2_mul_against_bigint(orc):
+    rnd = random.Random(71)
+    for _ in range(50):
+        a = (rnd.randrange(P), rnd.randrange(P))
+        b = (rnd.randrange(P), rnd.randrange(P))
+        la = np.array(pyref.fp_limbs(pyref.to_mont(a[0])) + pyref.fp_limbs(pyref.to_mont(a[1])), np.uint64)
+        lb = np.array(pyref.fp_limbs(pyref.to_mont(b[0])) + pyref.fp_limbs(pyref.to_mont(b[1])), np.uint64)
+        r = orc.fp2_mul(la, lb)
+        got = (pyref.from_mont(pyref.limbs_value(r[:6])), pyref.from_mont(pyref.limbs_value(r[6:])))
+        assert got == pyref2.f2mul(a, b)
+
+
+def test_g2_generator_and_hash_to_curve_points(orc):
+    """The G2 generator (point.cpp:114-138) against the IETF encoding of
+    Conformance_tests.cpp:18-21, and the 10 G2 hash-to-curve outputs."""
+    kat = json.load(open(os.path.join(GOLD, "kat.json")))
+    g = orc.p2_generator_affine()
+    assert pyref2.limbs_to_affine(list(g)) == pyref2.G2
+    assert orc.p2_compress(orc.p2_from_affine(g)).hex() == kat["g2_generator_compressed"]
+    assert orc.p2_compress(np.zeros(36, np.uint64)).hex() == kat["g2_infinity_compressed"]
+    for v in json.load(open(os.path.join(GOLD, "h2c_g2.json"))):
+        pt = (tuple(int(c, 16) for c in v["x"]), tuple(int(c, 16) for c in v["y"]))
+        assert pyref2.on_curve(pt)
+        j = orc.p2_from_affine(g2_limbs(pt))
+        assert orc.p2_compress(j).hex() == v["compressed"]
+        assert orc.p2_compress(orc.p2_mult(j, pyref.R_ORDER.to_bytes(32, "little"), 256)) == \
+            bytes([0xC0]) + bytes(95)      # in the prime-order subgroup
+
+
+def test_g2_group_law_and_normalisation(orc):
+    rnd = random.Random(72)
+    G = orc.p2_from_affine(orc.p2_generator_affine())
+    ks = [rnd.getrandbits(255) for _ in range(4)] + [1, 2, pyref.R_ORDER - 1]
+    jac = np.stack([orc.p2_mult(G, k.to_bytes(32, "little"), 255) for k in ks] + [np.zeros(36, np.uint64)])
+    want = [pyref2.mul(k, pyref2.G2) for k in ks] + [None]
+    for j, w in zip(jac, want):
+        assert pyref2.jacobian_limbs_to_affine(list(j)) == w
+        assert orc.p2_compress(j) == pyref2.compress(w)
+    aff = orc.p2s_to_affine(jac)
+    assert [pyref2.limbs_to_affine(list(a)) for a in aff] == want
+    s = orc.p2_add(jac[0], jac[1])
+    assert orc.p2_compress(s) == pyref2.compress(pyref2.add(want[0], want[1]))
+    assert orc.p2_compress(orc.p2_add(jac[0], jac[0])) == pyref2.compress(pyref2.add(want[0], want[0]))
+    neg = orc.p2_mult(jac[0], (pyref.R_ORDER - 1).to_bytes(32, "little"), 255)
+    assert orc.p2_compress(orc.p2_add(jac[0], neg)) == bytes([0xC0]) + bytes(95)
+
+
+@pytest.mark.parametrize("n,nbits", [(1, 255), (2, 8), (5, 1), (7, 64), (9, 255), (12, 256)])
+def test_g2_msm_against_textbook(orc, n, nbits):
+    rnd = random.Random(n * 31 + nbits)
+    pts = orc.gen_p2_points(n, 100 + n)
+    if n > 3:
+        pts[2] = 0
+        pts[3] = orc.p2_to_affine(orc.p2_mult(orc.p2_from_affine(pts[1]), (pyref.R_ORDER - 1).to_bytes(32, "little"),
+                                              255))
+    nb = (nbits + 7) // 8
+    ks = [rnd.getrandbits(8 * nb) for _ in range(n)]
+    if n > 4:
+        ks[4] = 0
+        ks[3] = ks[1] % (1 << nbits)   # P and -P with equal scalars cancel
+    S = np.array([list(k.to_bytes(nb, "little")) for k in ks], np.uint8)
+    want = None
+    for k, p in zip(ks, pts):
+        if p.any():
+            want = pyref2.add(want, pyref2.mul(k % (1 << nbits), pyref2.limbs_to_affine(list(p))))
+    for window in (0, 1, 2, 3, 5, 8):
+        assert orc.p2_compress(orc.msm_p2_pippenger(pts, S, n, nbits, window)) == pyref2.compress(want)
+    # a single window's tile: the digits of bits [bit0 - 1, bit0 + 4)
+    if nbits >= 9:
+        t = orc.msm_p2_tile(pts, S, n, nbits, 4, 4)
+        wt = None
+        for k, p in zip(ks, pts):
+            kk = k % (1 << nbits)
+            d = ((kk >> 3) & 0x1F)
+            dig = ((d + 1) >> 1) - (16 if d >> 4 else 0)
+            if p.any() and dig:
+                wt = pyref2.add(wt, pyref2.mul(dig % pyref.R_ORDER, pyref2.limbs_to_affine(list(p))))
+        assert orc.p2_compress(t) == pyref2.compress(wt)
