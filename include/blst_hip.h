@@ -36,6 +36,9 @@ typedef struct { blst_limb_t l[384 / 8 / sizeof(blst_limb_t)]; } blst_fp;
 typedef struct { blst_fp x, y, z; } blst_p1;
 typedef struct { blst_fp x, y; } blst_p1_affine;
 typedef struct { blst_limb_t l[256 / 8 / sizeof(blst_limb_t)]; } blst_fr;
+typedef struct { blst_fp fp[2]; } blst_fp2;
+typedef struct { blst_fp2 x, y, z; } blst_p2;
+typedef struct { blst_fp2 x, y; } blst_p2_affine;
 #endif
 
 /* ------------------------------------------------------------------ drop-in */
@@ -140,6 +143,22 @@ size_t blst_p1s_mult_wbits_scratch_sizeof(size_t npoints);
 void blst_p1s_mult_wbits(blst_p1 *ret, const blst_p1_affine table[], size_t wbits, size_t npoints,
                          const blst_byte *const scalars[], size_t nbits, blst_limb_t *scratch);
 
+/* Replace the G2 Pippenger entry points (bindings/blst_scalar.h:734-740, 768;
+ * blst/c_api.cpp:1890-1975, 2053-2080; multi_scalar.cpp:935-943): the same
+ * contracts as the G1 functions above over blst_p2 / blst_p2_affine (Fp2
+ * coordinates), with 384 * 2^(window-1) scratch bytes (the size of the
+ * reference's Pointxyzz<Fp2> buckets). */
+size_t blst_p2s_mult_pippenger_scratch_sizeof(size_t npoints);
+size_t blst_p2s_mult_pippenger_window_scratch_sizeof(size_t window);
+void blst_p2s_mult_pippenger(blst_p2 *ret, const blst_p2_affine *const points[], size_t npoints,
+                             const blst_byte *const scalars[], size_t nbits, blst_limb_t *scratch);
+void blst_p2s_mult_pippenger_window(blst_p2 *ret, const blst_p2_affine *const points[], size_t npoints,
+                                    const blst_byte *const scalars[], size_t nbits, blst_limb_t *scratch,
+                                    size_t window);
+void blst_p2s_tile_pippenger(blst_p2 *ret, const blst_p2_affine *const points[], size_t npoints,
+                             const blst_byte *const scalars[], size_t nbits, blst_limb_t *scratch, size_t bit0,
+                             size_t window);
+
 /* ------------------------------------------------------- device extensions */
 
 /* Last error message of the calling thread ("" when the last call succeeded). */
@@ -242,6 +261,12 @@ int blst_hip_set_pipeline(size_t groups);
 /* blst_p1s_to_affine on device memory: npoints Jacobian points (144 bytes
  * each) to affine (96 bytes each), both 16-byte aligned and not overlapping;
  * returns when done, 0 or -1 (npoints below 2^31 - 1). */
+/* G2 MSM over inputs resident in device memory (npoints contiguous
+ * blst_p2_affine, 16-byte aligned, and npoints * ceil(nbits/8) scalar
+ * bytes); window 0 selects the tuned window.  Returns 0 on success. */
+int blst_hip_p2_msm_device(blst_p2 *ret, const void *device_points, const void *device_scalars, size_t npoints,
+                           size_t nbits, size_t window);
+
 int blst_hip_p1s_to_affine_device(void *device_dst, const void *device_points, size_t npoints);
 
 /* Fixed-base tables resident in device memory (the table of a resident

@@ -33,6 +33,12 @@ SIGNATURES = {
     "blst_hip_p1s_mult_wbits_precompute_device": ([_vp, _sz, _vp, _sz], ctypes.c_int),
     "blst_hip_p1s_mult_wbits_device": ([_vp, _vp, _sz, _sz, _vp, _sz], ctypes.c_int),
     "blst_hip_p1s_to_affine_device": ([_vp, _vp, _sz], ctypes.c_int),
+    "blst_p2s_mult_pippenger_scratch_sizeof": ([_sz], _sz),
+    "blst_p2s_mult_pippenger_window_scratch_sizeof": ([_sz], _sz),
+    "blst_p2s_mult_pippenger": ([_vp, _vp, _sz, _vp, _sz, _vp], None),
+    "blst_p2s_mult_pippenger_window": ([_vp, _vp, _sz, _vp, _sz, _vp, _sz], None),
+    "blst_p2s_tile_pippenger": ([_vp, _vp, _sz, _vp, _sz, _vp, _sz, _sz], None),
+    "blst_hip_p2_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_last_error": ([], ctypes.c_char_p),
     "blst_hip_available": ([], ctypes.c_int),
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
@@ -278,6 +284,51 @@ class DeviceAlloc:
 
     def close(self):
         self.buf.close()
+
+
+def p2s_mult_pippenger(points, scalars, nbits, window=0, scratch=True, mode="contiguous", k=1, bit0=None):
+    """G2 MSM through the drop-in entry points (points: n x 24 uint64);
+    bit0 selects blst_p2s_tile_pippenger.  Returns the 36-limb blst_p2."""
+    L = load()
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8)
+    n = pts.shape[0]
+    nb = sc.shape[1] if sc.ndim == 2 else 1
+    base_p, base_s = pts.ctypes.data, sc.ctypes.data
+    if mode == "contiguous":
+        pp, ss = [base_p, 0], [base_s, 0]
+    elif mode == "exact":
+        pp = [base_p + 192 * i for i in range(n)]
+        ss = [base_s + nb * i for i in range(n)]
+    elif mode == "mixed":
+        pp = [base_p + 192 * i if i < k else 0 for i in range(n)]
+        ss = [base_s + nb * i if i < k else 0 for i in range(n)]
+    else:
+        raise ValueError(mode)
+    pp = np.array(pp, np.uint64)
+    ss = np.array(ss, np.uint64)
+    ret = np.zeros(36, np.uint64)
+    sz = (L.blst_p2s_mult_pippenger_window_scratch_sizeof(window) if window
+          else L.blst_p2s_mult_pippenger_scratch_sizeof(n)) or 8
+    scr = np.zeros((sz + 7) // 8, np.uint64) if scratch else None
+    sp = _ptr(scr) if scratch else None
+    if bit0 is not None:
+        L.blst_p2s_tile_pippenger(_ptr(ret), _ptr(pp), n, _ptr(ss), nbits, sp, bit0, window)
+    elif window:
+        L.blst_p2s_mult_pippenger_window(_ptr(ret), _ptr(pp), n, _ptr(ss), nbits, sp, window)
+    else:
+        L.blst_p2s_mult_pippenger(_ptr(ret), _ptr(pp), n, _ptr(ss), nbits, sp)
+    err = last_error()
+    if err and n > 0:
+        raise RuntimeError("blst_p2s_mult_pippenger: " + err)
+    return ret
+
+
+def p2_msm_device(d_points_ptr, d_scalars_ptr, n, nbits, window=0):
+    ret = np.zeros(36, np.uint64)
+    if load().blst_hip_p2_msm_device(_ptr(ret), d_points_ptr, d_scalars_ptr, n, nbits, window) != 0:
+        raise RuntimeError("blst_hip_p2_msm_device: " + last_error())
+    return ret
 
 
 def p1s_tile_pippenger(points, scalars, nbits, bit0, window):

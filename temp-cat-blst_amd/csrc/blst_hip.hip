@@ -7,6 +7,7 @@
 // reference's control flow that callers can observe.  Everything past the
 // checks is this backend's own pipeline.
 #include "pipeline.h"
+#include "host_ec2.h"
 #include "../../include/blst_hip.h"
 
 #include <dlfcn.h>
@@ -185,8 +186,10 @@ JobSpec make_spec(const void* points, const void* scalars, size_t n, size_t nbit
 
 Layout make_layout(const std::vector<JobSpec>& specs) {
     Layout L;
+    L.g2 = !specs.empty() && specs[0].g2;
     uint64_t pts = 0, ent = 0, keys = 0, segs = 0, biggest = 0, tiles = 0;
     for (const JobSpec& s : specs) {
+        if (s.g2 != L.g2) throw std::runtime_error("a batch mixes G1 and G2 jobs");
         job_t J;
         J.scalars = static_cast<const uint8_t*>(s.scalars);
         J.points = static_cast<const uint4*>(s.points);
@@ -401,22 +404,82 @@ void ensure_workspace(Workspace& cx, const Layout& L) {
     cx.hist.ensure(((size_t)nkeys + 1) * 4);
     cx.off.ensure(((size_t)nkeys + 1) * 4);
     cx.scan_tmp.ensure(scan_tmp_words(nkeys + 1) * 4);
-    cx.buckets.ensure((size_t)nkeys * sizeof(xyzz));
+    const size_t pb = L.point_bytes();
+    cx.buckets.ensure((size_t)nkeys * pb);
     cx.slot_key.ensure((size_t)nslots * 4);
-    cx.slots.ensure((size_t)nslots * sizeof(xyzz));
+    cx.slots.ensure((size_t)nslots * pb);
     cx.heavy.ensure(((size_t)nslots + 64) * 4);
-    cx.H.ensure(L.tree_nodes * 2 * sizeof(xyzz));
-    cx.W.ensure((size_t)L.groups.size() * sizeof(xyzz));
-    cx.res.ensure((size_t)L.jobs.size() * sizeof(xyzz));
+    cx.H.ensure(L.tree_nodes * 2 * pb);
+    cx.W.ensure((size_t)L.groups.size() * pb);
+    cx.res.ensure((size_t)L.jobs.size() * pb);
 }
 
 // Device bytes a batch of this layout needs beyond its inputs.
 size_t workspace_bytes(const Layout& L) {
     const size_t nslots = 2 * (size_t)std::max<uint32_t>(div_up(L.total_entries, L.T), 1);
     const size_t sort_bytes = tiled_sort(L) ? L.count_words * 4 : (size_t)L.total_entries * 8;
-    return (size_t)L.total_entries * 8 + sort_bytes + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * sizeof(xyzz) +
-           nslots * (6 + sizeof(xyzz)) + L.tree_nodes * 2 * sizeof(xyzz) +
-           (L.groups.size() + L.jobs.size()) * sizeof(xyzz) + (1 << 20);
+    const size_t pb = L.point_bytes();
+    return (size_t)L.total_entries * 8 + sort_bytes + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * pb +
+           nslots * (6 + pb) + L.tree_nodes * 2 * pb + (L.groups.size() + L.jobs.size()) * pb + (1 << 20);
+}
+
+// The point stages of a batch (accumulate, partial merges, reduction, fold)
+// over the field of its group: F = fp for G1, fp2 for G2.
+template <class F>
+void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool host_fold, void* host_out,
+                    bool timing) {
+    using P = xyzz_t<F>;
+    const uint32_t nkeys = L.nkeys;
+    const uint32_t nent = L.total_entries;
+    const uint32_t T = L.T;
+    const uint32_t nchunks = std::max<uint32_t>(div_up(nent, T), 1);
+    const uint32_t ngroups = (uint32_t)L.groups.size();
+    const uint32_t njobs = (uint32_t)L.jobs.size();
+    hipStream_t st = cx.stream;
+    uint32_t* hist = cx.hist.as<uint32_t>();
+    uint32_t* off = cx.off.as<uint32_t>();
+    // buckets are not cleared: every bucket with entries is written below,
+    // and the reduction skips the empty ones by their count
+    if (timing) HIPCK(hipEventRecord(cx.ev[2], st));
+    launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.sorted.as<const uint2>(), off + nkeys, D.jobs,
+                        D.kbase, njobs, cx.buckets.as<P>(), cx.slot_key.as<uint32_t>(), cx.slots.as<P>(), T,
+                        nchunks);
+    HIPCK(hipGetLastError());
+    if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
+
+    // buckets spanning chunks: two partials in one lane, longer spans listed
+    // for one wave each, the longest for one workgroup each
+    uint32_t* heavy = cx.heavy.as<uint32_t>();
+    uint32_t* wave_list = heavy + 64;
+    uint32_t* group_list = wave_list + nchunks;
+    fill_words(cx, heavy, 2, 0);
+    launch_k_partial_light(dim3(div_up(nchunks, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
+                           cx.slots.as<const P>(), off, hist, T, nchunks, cx.buckets.as<P>(), heavy, wave_list,
+                           group_list);
+    launch_k_partial_wave(dim3(std::min<uint32_t>(div_up(nchunks, 16), 2048)), st, heavy, wave_list,
+                          cx.slots.as<const P>(), off, hist, T, cx.buckets.as<P>());
+    launch_k_partial_heavy(dim3(std::min<uint32_t>(div_up(nchunks, 512), 256)), st, heavy + 1, group_list,
+                           cx.slots.as<const P>(), off, hist, T, cx.buckets.as<P>());
+    HIPCK(hipGetLastError());
+    if (timing) HIPCK(hipEventRecord(cx.ev[4], st));
+
+    P* Hn = cx.H.as<P>();
+    launch_k_reduce_segments(dim3(div_up(L.total_segments, 256)), dim3(256), st, D.groups, D.sbase, ngroups,
+                             L.total_segments, cx.buckets.as<const P>(), hist, Hn);
+    const size_t npass = L.tree_blocks.size();
+    for (size_t p = 0; p < npass; p++)
+        launch_k_reduce_tree(dim3(L.tree_blocks[p]), st, D.tree_blk + p * ngroups, D.tree_tn + p * ngroups, ngroups,
+                             Hn + 2 * L.tree_in[p], Hn + 2 * L.tree_out[p], cx.W.as<P>(),
+                             p + 1 == npass ? 1u : 0u);
+    HIPCK(hipGetLastError());
+    if (host_fold) {
+        HIPCK(hipMemcpyAsync(host_out, cx.W.p, (size_t)ngroups * sizeof(P), hipMemcpyDeviceToHost, st));
+    } else {
+        launch_k_fold(st, D.jobs, njobs, cx.W.as<const P>(), cx.res.as<P>());
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(host_out, cx.res.p, (size_t)njobs * sizeof(P), hipMemcpyDeviceToHost, st));
+    }
+    if (timing) HIPCK(hipEventRecord(cx.ev[5], st));
 }
 
 // Enqueues the device pipeline of a batch on cx.stream; nothing in it waits
@@ -461,48 +524,8 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host
                          cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off, cx.sorted.as<uint2>());
     }
     HIPCK(hipGetLastError());
-    // buckets are not cleared: every bucket with entries is written below,
-    // and the reduction skips the empty ones by their count
-    if (timing) HIPCK(hipEventRecord(cx.ev[2], st));
-    launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.sorted.as<const uint2>(), off + nkeys, D.jobs,
-                        D.kbase, njobs, cx.buckets.as<xyzz>(), cx.slot_key.as<uint32_t>(), cx.slots.as<xyzz>(), T,
-                        nchunks);
-    HIPCK(hipGetLastError());
-    if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
-
-    // buckets spanning chunks: two partials in one lane, longer spans listed
-    // for one wave each, the longest for one workgroup each
-    uint32_t* heavy = cx.heavy.as<uint32_t>();
-    uint32_t* wave_list = heavy + 64;
-    uint32_t* group_list = wave_list + nchunks;
-    fill_words(cx, heavy, 2, 0);
-    launch_k_partial_light(dim3(div_up(nchunks, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
-                           cx.slots.as<const xyzz>(), off, hist, T, nchunks, cx.buckets.as<xyzz>(), heavy, wave_list,
-                           group_list);
-    launch_k_partial_wave(dim3(std::min<uint32_t>(div_up(nchunks, 16), 2048)), st, heavy, wave_list,
-                          cx.slots.as<const xyzz>(), off, hist, T, cx.buckets.as<xyzz>());
-    launch_k_partial_heavy(dim3(std::min<uint32_t>(div_up(nchunks, 512), 256)), st, heavy + 1, group_list,
-                           cx.slots.as<const xyzz>(), off, hist, T, cx.buckets.as<xyzz>());
-    HIPCK(hipGetLastError());
-    if (timing) HIPCK(hipEventRecord(cx.ev[4], st));
-
-    xyzz* Hn = cx.H.as<xyzz>();
-    launch_k_reduce_segments(dim3(div_up(L.total_segments, 256)), dim3(256), st, D.groups, D.sbase, ngroups,
-                             L.total_segments, cx.buckets.as<const xyzz>(), hist, Hn);
-    const size_t npass = L.tree_blocks.size();
-    for (size_t p = 0; p < npass; p++)
-        launch_k_reduce_tree(dim3(L.tree_blocks[p]), st, D.tree_blk + p * ngroups, D.tree_tn + p * ngroups, ngroups,
-                             Hn + 2 * L.tree_in[p], Hn + 2 * L.tree_out[p], cx.W.as<xyzz>(),
-                             p + 1 == npass ? 1u : 0u);
-    HIPCK(hipGetLastError());
-    if (host_fold) {
-        HIPCK(hipMemcpyAsync(host_out, cx.W.p, (size_t)ngroups * sizeof(xyzz), hipMemcpyDeviceToHost, st));
-    } else {
-        launch_k_fold(st, D.jobs, njobs, cx.W.as<const xyzz>(), cx.res.as<xyzz>());
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(host_out, cx.res.p, (size_t)njobs * sizeof(xyzz), hipMemcpyDeviceToHost, st));
-    }
-    if (timing) HIPCK(hipEventRecord(cx.ev[5], st));
+    if (L.g2) enqueue_points<fp2>(cx, L, D, host_fold, host_out, timing);
+    else enqueue_points<fp>(cx, L, D, host_fold, host_out, timing);
 }
 
 void record_phases(Workspace& cx) {
@@ -655,6 +678,57 @@ int host_msm(blst_p1* ret, const blst_p1_affine* const points[], size_t npoints,
         return -1;
     }
     g_phase[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    g_err.clear();
+    return 0;
+}
+
+// A G2 MSM (one pipeline over Fp2 points) on inputs already on the device;
+// the host folds the windows over Fp2.
+void device_msm_g2(Ctx& cx, const JobSpec& js, blst_p2* ret) {
+    std::vector<JobSpec> v(1, js);
+    v[0].g2 = true;
+    Layout L = make_layout(v);
+    ensure_workspace(cx.ws, L);
+    const DevLayout D = upload_layout(cx.ws, L);
+    std::vector<hostec2::xyzz12> wins(L.groups.size());
+    enqueue_batch(cx.ws, L, D, true, wins.data());
+    HIPCK(hipStreamSynchronize(cx.stream));
+    record_phases(cx.ws);
+    hostec2::xyzz12 r;
+    if (!js.tile) hostec2::fold_windows(r, wins.data(), (int)wins.size(), (int)js.c);
+    else r = wins[0];
+    hostec2::jac12 j;
+    hostec2::to_jac(j, r);
+    static_assert(sizeof(hostec2::jac12) == sizeof(blst_p2), "blst_p2 layout");
+    std::memcpy(ret, &j, sizeof(j));
+}
+
+int host_msm_g2(blst_p2* ret, const blst_p2_affine* const points[], size_t npoints,
+                const blst_byte* const scalars[], size_t nbits, size_t window, bool fold, size_t bit0) {
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        const size_t nbytes = (nbits + 7) / 8;
+        std::vector<uint8_t> sp, ss;
+        const uint8_t* hp = resolve(points, npoints, sizeof(blst_p2_affine), sp);
+        const uint8_t* hs = resolve(scalars, npoints, nbytes, ss);
+        cx.pts.ensure(npoints * sizeof(blst_p2_affine));
+        cx.scl.ensure(npoints * nbytes + 16);
+        HIPCK(hipMemcpyAsync(cx.pts.p, hp, npoints * sizeof(blst_p2_affine), hipMemcpyHostToDevice, cx.stream));
+        HIPCK(hipMemcpyAsync(cx.scl.p, hs, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
+        JobSpec js = make_spec(cx.pts.p, cx.scl.p, npoints, nbits, window, false);
+        js.bit_base = (uint32_t)bit0;
+        js.tile = !fold;
+        device_msm_g2(cx, js, ret);
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        std::memset(ret, 0, sizeof(*ret));
+        return -1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        std::memset(ret, 0, sizeof(*ret));
+        return -1;
+    }
     g_err.clear();
     return 0;
 }
@@ -1094,6 +1168,92 @@ void blst_p1s_tile_pippenger(blst_p1* ret, const blst_p1_affine* const points[],
         return;
     }
     host_msm(ret, points, npoints, scalars, nbits, window, false, bit0);
+}
+
+size_t blst_p2s_mult_pippenger_scratch_sizeof(size_t npoints) {
+    if (npoints == 0) return 0;
+    return (size_t)384 << (ref_window_size(npoints) - 1);
+}
+
+size_t blst_p2s_mult_pippenger_window_scratch_sizeof(size_t window) {
+    if (window < 1 || window > 24) return 0;
+    return (size_t)384 << (window - 1);
+}
+
+void blst_p2s_mult_pippenger(blst_p2* ret, const blst_p2_affine* const points[], size_t npoints,
+                             const blst_byte* const scalars[], size_t nbits, blst_limb_t* scratch) {
+    if (npoints == 0 || nbits == 0) { std::memset(ret, 0, sizeof(*ret)); g_err.clear(); return; }
+    if (nbits > 256 || !heads_ok((const void* const*)points, (const void* const*)scalars, npoints) ||
+        npoints >= 0x7fffffffULL) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "invalid arguments";
+        return;
+    }
+    if (npoints > 1 && scratch == nullptr) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "scratch is NULL";
+        return;
+    }
+    host_msm_g2(ret, points, npoints, scalars, nbits, 0, true, 0);
+}
+
+void blst_p2s_mult_pippenger_window(blst_p2* ret, const blst_p2_affine* const points[], size_t npoints,
+                                    const blst_byte* const scalars[], size_t nbits,
+                                    blst_limb_t* scratch, size_t window) {
+    if (npoints == 0 || nbits == 0) { std::memset(ret, 0, sizeof(*ret)); g_err.clear(); return; }
+    if (nbits > 256 || window < 1 || window > 24 || npoints >= 0x7fffffffULL ||
+        !heads_ok((const void* const*)points, (const void* const*)scalars, npoints)) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "invalid arguments";
+        return;
+    }
+    if (npoints > 1 && scratch == nullptr) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "scratch is NULL";
+        return;
+    }
+    host_msm_g2(ret, points, npoints, scalars, nbits, window, true, 0);
+}
+
+void blst_p2s_tile_pippenger(blst_p2* ret, const blst_p2_affine* const points[], size_t npoints,
+                             const blst_byte* const scalars[], size_t nbits, blst_limb_t* scratch,
+                             size_t bit0, size_t window) {
+    if (npoints == 0 || !heads_ok((const void* const*)points, (const void* const*)scalars, npoints) ||
+        scratch == nullptr || window < 1 || window > 24 || nbits == 0 || nbits > 256 || bit0 >= nbits ||
+        npoints >= 0x7fffffffULL) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "invalid arguments";
+        return;
+    }
+    host_msm_g2(ret, points, npoints, scalars, nbits, window, false, bit0);
+}
+
+int blst_hip_p2_msm_device(blst_p2* ret, const void* device_points, const void* device_scalars, size_t npoints,
+                           size_t nbits, size_t window) {
+    if (!ret) { g_err = "invalid arguments"; return -1; }
+    std::memset(ret, 0, sizeof(*ret));
+    if (npoints == 0 || nbits == 0) { g_err.clear(); return 0; }
+    if (!device_points || !device_scalars || nbits > 256 || window > 24 || npoints >= 0x7fffffffULL ||
+        ((uintptr_t)device_points % 16)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        JobSpec js = make_spec(device_points, device_scalars, npoints, nbits, window, false);
+        device_msm_g2(cx, js, ret);
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        std::memset(ret, 0, sizeof(*ret));
+        return -1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        std::memset(ret, 0, sizeof(*ret));
+        return -1;
+    }
+    g_err.clear();
+    return 0;
 }
 
 const char* blst_hip_last_error(void) { return g_err.c_str(); }

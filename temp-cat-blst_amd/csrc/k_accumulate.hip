@@ -9,11 +9,12 @@ namespace bls {
 // ends inside the chunk is final and goes to buckets[key].  Otherwise, with
 // k0 the chunk holding the bucket's first entry: the segment in chunk k0 goes
 // to slot 2*k0+1, the segment in any later chunk k to slot 2*k.
+template <class F>
 __global__ void __launch_bounds__(256)
 k_accumulate(const uint2* __restrict__ sorted,
              const uint32_t* __restrict__ total, const job_t* __restrict__ jobs,
-             const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets,
-             uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks) {
+             const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets,
+             uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks) {
     const uint32_t chunk = blockIdx.x * blockDim.x + threadIdx.x;
     if (chunk >= nchunks) return;
     slot_key[2 * chunk] = kInvalid;
@@ -28,7 +29,7 @@ k_accumulate(const uint2* __restrict__ sorted,
     const uint4* pts = jobs[j].points;
     const bool before = start > 0 && sorted[start - 1].x == cur;
     bool first = true;
-    xyzz acc;
+    xyzz_t<F> acc;
     xyzz_set_inf(acc);
     #pragma unroll 1
     for (uint32_t e = start; e < end; e++) {
@@ -51,7 +52,7 @@ k_accumulate(const uint2* __restrict__ sorted,
             }
         }
         const uint32_t v = kv.y;
-        aff a;
+        aff_t<F> a;
         load_aff(a, pts, v & 0x7fffffffu);
         // an all-zero affine point is infinity and contributes nothing
         // (Pointxyzz::dadd with an all-zero affine input, point.cpp:2036-2039)
@@ -72,8 +73,12 @@ k_accumulate(const uint2* __restrict__ sorted,
     }
 }
 
-void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks) {
-    hipLaunchKernelGGL(k_accumulate, grid, block, 0, stream, sorted, total, jobs, kbase, njobs, buckets, slot_key, slots, T, nchunks);
+template <class F>
+void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks) {
+    hipLaunchKernelGGL(k_accumulate<F>, grid, block, 0, stream, sorted, total, jobs, kbase, njobs, buckets, slot_key, slots, T, nchunks);
 }
+
+template void launch_k_accumulate<fp>(dim3, dim3, hipStream_t, const uint2*, const uint32_t*, const job_t*, const uint32_t*, uint32_t, xyzz*, uint32_t*, xyzz*, uint32_t, uint32_t);
+template void launch_k_accumulate<fp2>(dim3, dim3, hipStream_t, const uint2*, const uint32_t*, const job_t*, const uint32_t*, uint32_t, xyzz2*, uint32_t*, xyzz2*, uint32_t, uint32_t);
 
 }  // namespace bls

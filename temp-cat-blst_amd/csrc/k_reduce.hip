@@ -19,17 +19,18 @@ namespace bls {
 // Segment running sums: one lane per segment.  The running sum (acc += S,
 // tot += acc) shares one addition site and the doublings of V one doubling
 // site, which keeps the kernel small.
+template <class F>
 __global__ void __launch_bounds__(256)
 k_reduce_segments(const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase,
-                  uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets,
-                  const uint32_t* __restrict__ cnt, xyzz* __restrict__ nodes) {
+                  uint32_t ngroups, uint32_t total_segments, const xyzz_t<F>* __restrict__ buckets,
+                  const uint32_t* __restrict__ cnt, xyzz_t<F>* __restrict__ nodes) {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= total_segments) return;
     const uint32_t g = find_owner(sbase, ngroups, t);
     const group_t G = groups[g];
     const uint32_t s = t - G.sb;
     const size_t base = (size_t)G.kb + (size_t)s * G.L;
-    xyzz acc, tot, x, y;
+    xyzz_t<F> acc, tot, x, y;
     xyzz_set_inf(acc);
     xyzz_set_inf(tot);
 #pragma unroll 1
@@ -59,10 +60,11 @@ k_reduce_segments(const group_t* __restrict__ groups, const uint32_t* __restrict
 // than four merges repeats them on four quads (16 lanes) and stores one.
 // tn[g] = (first input node, input nodes, first output node); blk[g] = first
 // block of group g.  last: store the root's A to W[g] instead.
+template <class F>
 __global__ void __launch_bounds__(256)
 k_reduce_tree(const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups,
-              const xyzz* __restrict__ in, xyzz* __restrict__ out, xyzz* __restrict__ W, uint32_t last) {
-    __shared__ xyzz sA[kTreeNodes], sV[kTreeNodes];
+              const xyzz_t<F>* __restrict__ in, xyzz_t<F>* __restrict__ out, xyzz_t<F>* __restrict__ W, uint32_t last) {
+    __shared__ xyzz_t<F> sA[kTreeNodes], sV[kTreeNodes];
     const uint32_t g = find_owner(blk, ngroups, blockIdx.x);
     const uint32_t j = blockIdx.x - blk[g];
     const uint4 t = tn[g];
@@ -72,53 +74,63 @@ k_reduce_tree(const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, ui
     const bool beta = u >= 32;
     const uint32_t v = u & 31;
     if (u < n) {
-        fp a, b;
+        F a, b;
         quad_load(a, &in[2 * (size_t)(first + u)]);
         quad_load(b, &in[2 * (size_t)(first + u) + 1]);
-        reinterpret_cast<fp*>(&sA[u])[ln] = a;
-        reinterpret_cast<fp*>(&sV[u])[ln] = b;
+        reinterpret_cast<F*>(&sA[u])[ln] = a;
+        reinterpret_cast<F*>(&sV[u])[ln] = b;
     }
     __syncthreads();
     for (uint32_t m = n >> 1; m >= 1; m >>= 1) {
-        fp r;
+        F r;
         if (v < max(m, 4u)) {
             const uint32_t mm = v % m;
-            fp x, y;
+            F x, y;
             if (!beta) {
-                x = reinterpret_cast<const fp*>(&sA[2 * mm + 1])[ln];
-                y = reinterpret_cast<const fp*>(&sV[2 * mm + 1])[ln];
+                x = reinterpret_cast<const F*>(&sA[2 * mm + 1])[ln];
+                y = reinterpret_cast<const F*>(&sV[2 * mm + 1])[ln];
                 quad_add(x, x, y);
-                y = reinterpret_cast<const fp*>(&sA[2 * mm])[ln];
+                y = reinterpret_cast<const F*>(&sA[2 * mm])[ln];
                 quad_add(r, y, x);
             } else {
-                x = reinterpret_cast<const fp*>(&sV[2 * mm])[ln];
-                y = reinterpret_cast<const fp*>(&sV[2 * mm + 1])[ln];
+                x = reinterpret_cast<const F*>(&sV[2 * mm])[ln];
+                y = reinterpret_cast<const F*>(&sV[2 * mm + 1])[ln];
                 quad_add(x, x, y);
                 quad_dbl(r, x);
             }
         }
         __syncthreads();
-        if (v < m) reinterpret_cast<fp*>(beta ? &sV[v] : &sA[v])[ln] = r;
+        if (v < m) reinterpret_cast<F*>(beta ? &sV[v] : &sA[v])[ln] = r;
         __syncthreads();
     }
     if (last) {
         if (u == 0) {
-            fp c;
-            fp_canon(c, reinterpret_cast<const fp*>(&sA[0])[ln]);
+            F c;
+            fp_canon(c, reinterpret_cast<const F*>(&sA[0])[ln]);
             quad_store(&W[g], c);
         }
     } else if (u == 0) {
-        quad_store(&out[2 * (size_t)(t.z + j)], reinterpret_cast<const fp*>(&sA[0])[ln]);
-        quad_store(&out[2 * (size_t)(t.z + j) + 1], reinterpret_cast<const fp*>(&sV[0])[ln]);
+        quad_store(&out[2 * (size_t)(t.z + j)], reinterpret_cast<const F*>(&sA[0])[ln]);
+        quad_store(&out[2 * (size_t)(t.z + j) + 1], reinterpret_cast<const F*>(&sV[0])[ln]);
     }
 }
 
-void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets, const uint32_t* __restrict__ cnt, xyzz* __restrict__ nodes) {
-    hipLaunchKernelGGL(k_reduce_segments, grid, block, 0, stream, groups, sbase, ngroups, total_segments, buckets, cnt, nodes);
+template <class F>
+void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz_t<F>* __restrict__ buckets, const uint32_t* __restrict__ cnt, xyzz_t<F>* __restrict__ nodes) {
+    hipLaunchKernelGGL(k_reduce_segments<F>, grid, block, 0, stream, groups, sbase, ngroups, total_segments, buckets, cnt, nodes);
 }
 
-void launch_k_reduce_tree(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups, const xyzz* __restrict__ in, xyzz* __restrict__ out, xyzz* __restrict__ W, uint32_t last) {
-    hipLaunchKernelGGL(k_reduce_tree, grid, dim3(256), 0, stream, blk, tn, ngroups, in, out, W, last);
+template <class F>
+void launch_k_reduce_tree(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups, const xyzz_t<F>* __restrict__ in, xyzz_t<F>* __restrict__ out, xyzz_t<F>* __restrict__ W, uint32_t last) {
+    hipLaunchKernelGGL(k_reduce_tree<F>, grid, dim3(256), 0, stream, blk, tn, ngroups, in, out, W, last);
 }
+
+#define BLS_REDUCE_INSTANCES(F)                                                                                     \
+    template void launch_k_reduce_segments<F>(dim3, dim3, hipStream_t, const group_t*, const uint32_t*, uint32_t,  \
+                                              uint32_t, const xyzz_t<F>*, const uint32_t*, xyzz_t<F>*);           \
+    template void launch_k_reduce_tree<F>(dim3, hipStream_t, const uint32_t*, const uint4*, uint32_t,              \
+                                          const xyzz_t<F>*, xyzz_t<F>*, xyzz_t<F>*, uint32_t);
+BLS_REDUCE_INSTANCES(fp)
+BLS_REDUCE_INSTANCES(fp2)
 
 }  // namespace bls

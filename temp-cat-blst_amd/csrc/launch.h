@@ -17,14 +17,21 @@ void launch_k_prefix(dim3 grid, dim3 block, hipStream_t stream, const group_t* _
 void launch_k_place(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off, uint2* __restrict__ sorted);
 size_t sort_lds_limit();
 void launch_k_fr_to_scalar(hipStream_t stream, uint32_t* __restrict__ io, uint32_t n);
-void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz* __restrict__ slots, uint32_t T, uint32_t nchunks);
-void launch_k_partial_light(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ slot_key, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, uint32_t nchunks, xyzz* __restrict__ buckets, uint32_t* __restrict__ counts, uint32_t* __restrict__ wave_list, uint32_t* __restrict__ group_list);
-void launch_k_partial_heavy(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ heavy_count, const uint32_t* __restrict__ heavy, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, xyzz* __restrict__ buckets);
-void launch_k_partial_wave(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ count, const uint32_t* __restrict__ list, const xyzz* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, xyzz* __restrict__ buckets);
-void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz* __restrict__ buckets, const uint32_t* __restrict__ cnt, xyzz* __restrict__ nodes);
-void launch_k_reduce_tree(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups, const xyzz* __restrict__ in, xyzz* __restrict__ out, xyzz* __restrict__ W, uint32_t last);
+template <class F>
+void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks);
+template <class F>
+void launch_k_partial_light(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ slot_key, const xyzz_t<F>* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, uint32_t nchunks, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ counts, uint32_t* __restrict__ wave_list, uint32_t* __restrict__ group_list);
+template <class F>
+void launch_k_partial_heavy(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ heavy_count, const uint32_t* __restrict__ heavy, const xyzz_t<F>* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, xyzz_t<F>* __restrict__ buckets);
+template <class F>
+void launch_k_partial_wave(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ count, const uint32_t* __restrict__ list, const xyzz_t<F>* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, xyzz_t<F>* __restrict__ buckets);
+template <class F>
+void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz_t<F>* __restrict__ buckets, const uint32_t* __restrict__ cnt, xyzz_t<F>* __restrict__ nodes);
+template <class F>
+void launch_k_reduce_tree(dim3 grid, hipStream_t stream, const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups, const xyzz_t<F>* __restrict__ in, xyzz_t<F>* __restrict__ out, xyzz_t<F>* __restrict__ W, uint32_t last);
 void launch_k_quad_check(dim3 grid, dim3 block, hipStream_t stream, const uint4* __restrict__ pts, uint32_t n, uint32_t* __restrict__ bad);
-void launch_k_fold(hipStream_t stream, const job_t* __restrict__ jobs, uint32_t njobs, const xyzz* __restrict__ W, xyzz* __restrict__ out);
+template <class F>
+void launch_k_fold(hipStream_t stream, const job_t* __restrict__ jobs, uint32_t njobs, const xyzz_t<F>* __restrict__ W, xyzz_t<F>* __restrict__ out);
 void launch_k_fixed_base(dim3 grid, dim3 block, hipStream_t stream, const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits, const uint4* __restrict__ table, xyzz* __restrict__ out);
 void launch_k_wbits_rows(hipStream_t stream, const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint32_t m, uint32_t L, xyzz* __restrict__ out);
 void launch_k_wbits_sum(hipStream_t stream, const uint4* __restrict__ table, const uint8_t* __restrict__ scalars, uint32_t nbytes, uint32_t nbits, uint32_t vec, const uint4* __restrict__ list, uint32_t n, uint32_t wbits, uint32_t D, uint32_t C, uint32_t nch, xyzz* __restrict__ part);

@@ -69,6 +69,42 @@ __device__ __forceinline__ void load_xyzz(xyzz& p, const xyzz* __restrict__ src)
         }
 }
 
+// G2 (Fp2 coordinates): the same layouts at twice the size
+__device__ __forceinline__ void fp_from_uint4(fp& r, const uint4* v) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        r.l[4 * k] = v[k].x; r.l[4 * k + 1] = v[k].y; r.l[4 * k + 2] = v[k].z; r.l[4 * k + 3] = v[k].w;
+    }
+}
+
+__device__ __forceinline__ void load_aff(aff2& a, const uint4* __restrict__ pts, uint32_t idx) {
+    const uint4* s = pts + (size_t)idx * 12;
+    uint4 v[12];
+#pragma unroll
+    for (int k = 0; k < 12; k++) v[k] = s[k];
+    fp_from_uint4(a.x.c0, v);
+    fp_from_uint4(a.x.c1, v + 3);
+    fp_from_uint4(a.y.c0, v + 6);
+    fp_from_uint4(a.y.c1, v + 9);
+}
+
+__device__ __forceinline__ void store_xyzz(xyzz2* __restrict__ dst, const xyzz2& p) {
+    uint4* d = reinterpret_cast<uint4*>(dst);
+    const fp* f = &p.x.c0;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+#pragma unroll
+        for (int k = 0; k < 3; k++)
+            d[q * 3 + k] = make_uint4(f[q].l[4 * k], f[q].l[4 * k + 1], f[q].l[4 * k + 2], f[q].l[4 * k + 3]);
+}
+
+__device__ __forceinline__ void load_xyzz(xyzz2& p, const xyzz2* __restrict__ src) {
+    const uint4* s = reinterpret_cast<const uint4*>(src);
+    fp* f = &p.x.c0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) fp_from_uint4(f[q], s + 3 * q);
+}
+
 // Scalar i as nine 32-bit words (the ninth is zero padding), bits at and
 // above nbits cleared: the reference reads only ceil(nbits/8) bytes and
 // masks each window to nbits (WindowExtract::initialize, window_end).

@@ -112,6 +112,7 @@ struct JobSpec {
     bool tile;             // one window at bit_base, no window fold
     uint32_t win0 = 0;     // window range [win0, win0 + nwin) of the job
     uint32_t nwin = 0;     // (0: all windows)
+    bool g2 = false;       // G2 points (Fp2 coordinates, 192-byte affine)
 };
 
 // Tile shape and window of one MSM size: c window bits, T sorted entries
@@ -149,6 +150,8 @@ struct Layout {
     std::vector<uint4> tree_tn;
     size_t tree_nodes = 0;
     uint32_t T = 32;       // sorted entries per accumulate lane
+    bool g2 = false;       // every job of the batch is a G2 MSM
+    size_t point_bytes() const { return g2 ? 384 : 192; }   // one XYZZ point
 };
 
 // Device copies of the layout tables (one upload per batch).

@@ -218,3 +218,35 @@ def test_wbits_sizes_and_checks_without_gpu_work(orc):
                  (P(tab), 2, None, 2), (P(tab), 2, P(np.zeros(2, np.uint64)), 2), (P(tab), 2, P(dp.pp), 0)):
         L.blst_p1s_mult_wbits_precompute(*args)
         assert (tab == 7).all(), args
+
+
+def test_g2_scratch_sizes_and_checks(orc):
+    """G2 scratch sizes (c_api.cpp:1890-1899: 384-byte Pointxyzz<Fp2>
+    buckets) and the early returns that need no device."""
+    import blst_hip
+    L = blst_hip.load()
+    for n in [0, 1, 2, 3, 17, 4096, 1 << 20]:
+        assert L.blst_p2s_mult_pippenger_scratch_sizeof(n) == 2 * orc.scratch_sizeof(n)
+    for w in range(0, 30):
+        assert L.blst_p2s_mult_pippenger_window_scratch_sizeof(w) == 2 * orc.window_scratch_sizeof(w)
+    ret = np.full(36, 7, np.uint64)
+    P = blst_hip._ptr
+    L.blst_p2s_mult_pippenger(P(ret), None, 0, None, 255, None)
+    assert not ret.any()
+    pts = np.zeros((2, 24), np.uint64)
+    sc = np.zeros((2, 32), np.uint8)
+    pp = np.array([pts.ctypes.data, 0], np.uint64)
+    ss = np.array([sc.ctypes.data, 0], np.uint64)
+    scr = np.zeros(4096, np.uint64)
+    for args in ((P(pp), 2, P(ss), 0, P(scr)), (P(pp), 2, P(ss), 257, P(scr)), (P(pp), 2, P(ss), 255, None),
+                 (P(np.zeros(2, np.uint64)), 2, P(ss), 255, P(scr))):
+        ret[:] = 7
+        L.blst_p2s_mult_pippenger(P(ret), *args)
+        assert not ret.any(), args
+    for w in (0, 25):
+        ret[:] = 7
+        L.blst_p2s_mult_pippenger_window(P(ret), P(pp), 2, P(ss), 255, P(scr), w)
+        assert not ret.any()
+        ret[:] = 7
+        L.blst_p2s_tile_pippenger(P(ret), P(pp), 2, P(ss), 255, P(scr), 0, w)
+        assert not ret.any()

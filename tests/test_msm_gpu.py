@@ -608,3 +608,104 @@ def test_wbits_device_2pow16_property(hip):
         d_tab.close()
         d_sc.close()
     assert hip.compress(got) == hip.compress(plain)
+
+
+# ---------------------------------------------------------------- G2
+
+import pyref2  # noqa: E402
+
+
+@pytest.fixture(scope="module")
+def g2_points(orc):
+    return orc.gen_p2_points(2048, 0x6262)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 7, 33, 100, 513, 2048])
+def test_g2_msm_random(hip, orc, g2_points, n):
+    pts = np.ascontiguousarray(g2_points[:n])
+    sc = orc.gen_scalars(n, 2000 + n)
+    want = orc.p2_compress(orc.msm_p2_pippenger(pts, sc, n, 255))
+    assert orc.p2_compress(hip.p2s_mult_pippenger(pts, sc, 255)) == want
+
+
+@pytest.mark.parametrize("nbits", [1, 8, 64, 255, 256])
+@pytest.mark.parametrize("window", [0, 1, 4, 9, 13])
+def test_g2_msm_nbits_windows(hip, orc, g2_points, nbits, window):
+    n = 150
+    rnd = random.Random(nbits * 100 + window)
+    nb = (nbits + 7) // 8
+    S = _scal([rnd.getrandbits(8 * nb) for _ in range(n)], nb)
+    pts = np.ascontiguousarray(g2_points[:n])
+    want = orc.p2_compress(orc.msm_p2_pippenger(pts, S, n, nbits, window))
+    assert orc.p2_compress(hip.p2s_mult_pippenger(pts, S, nbits, window=window)) == want
+
+
+def test_g2_edge_inputs(hip, orc, g2_points):
+    """Zero scalars, infinity points, P and -P with equal scalars, repeated
+    points (the doubling path) and r - 1."""
+    n = 64
+    pts = np.ascontiguousarray(g2_points[:n]).copy()
+    rnd = random.Random(5)
+    ks = [rnd.getrandbits(255) for _ in range(n)]
+    ks[0] = 0
+    pts[1] = 0
+    neg = orc.p2_to_affine(orc.p2_mult(orc.p2_from_affine(pts[2]), (R - 1).to_bytes(32, "little"), 255))
+    pts[3] = neg
+    ks[3] = ks[2]
+    for i in range(10, 20):
+        pts[i] = pts[9]
+        ks[i] = ks[9]
+    ks[30] = R - 1
+    ks[31] = 1
+    S = _scal(ks)
+    want = orc.p2_compress(orc.msm_p2_pippenger(pts, S, n, 255))
+    assert orc.p2_compress(hip.p2s_mult_pippenger(pts, S, 255)) == want
+    # everything cancels or is zero: infinity
+    Z = _scal([0] * n)
+    assert orc.p2_compress(hip.p2s_mult_pippenger(pts, Z, 255)) == bytes([0xC0]) + bytes(95)
+
+
+def test_g2_hash_to_curve_points_textbook(hip):
+    """The 10 G2 hash-to-curve outputs as bases with small scalars, against
+    the textbook Fp2 model."""
+    vs = json.load(open(os.path.join(GOLD, "h2c_g2.json")))
+    pts = [(tuple(int(c, 16) for c in v["x"]), tuple(int(c, 16) for c in v["y"])) for v in vs]
+    ks = [3 + 7 * i for i in range(len(pts))]
+    limbs = np.array([pyref2.affine_to_limbs(p) for p in pts], np.uint64)
+    got = hip.p2s_mult_pippenger(limbs, _scal(ks, 1), 8)
+    assert pyref2.jacobian_limbs_to_affine(list(got)) == pyref2.msm(pts, ks)
+
+
+@pytest.mark.parametrize("mode", ["exact", "mixed"])
+def test_g2_descriptors_and_tile(hip, orc, g2_points, mode):
+    n = 45
+    pts = np.ascontiguousarray(g2_points[:n])
+    sc = orc.gen_scalars(n, 77)
+    want = orc.p2_compress(orc.msm_p2_pippenger(pts, sc, n, 255))
+    assert orc.p2_compress(hip.p2s_mult_pippenger(pts, sc, 255, mode=mode, k=11)) == want
+    for bit0, window in ((0, 8), (248, 8), (100, 13)):
+        t = hip.p2s_mult_pippenger(pts, sc, 255, window=window, bit0=bit0)
+        assert orc.p2_compress(t) == orc.p2_compress(orc.msm_p2_tile(pts, sc, n, 255, bit0, window))
+
+
+def test_g2_device_linearity_2pow15(hip, orc):
+    """MSM(k) + MSM(k') == MSM(k + k' mod r) over 2^15 resident G2 bases."""
+    n = 1 << 15
+    pts = orc.gen_p2_points(n, 0x7777)
+    a = hip.synth_scalars(n, 0x71)
+    b = hip.synth_scalars(n, 0x72)
+    s = np.zeros_like(a)
+    for i in range(n):
+        v = (int.from_bytes(a[i].tobytes(), "little") + int.from_bytes(b[i].tobytes(), "little")) % R
+        s[i] = np.frombuffer(v.to_bytes(32, "little"), np.uint8)
+    dp = hip.DeviceBuffer(pts)
+    da, db, ds = hip.DeviceBuffer(a), hip.DeviceBuffer(b), hip.DeviceBuffer(s)
+    try:
+        ra = hip.p2_msm_device(dp.ptr, da.ptr, n, 255)
+        rb = hip.p2_msm_device(dp.ptr, db.ptr, n, 255)
+        rs = hip.p2_msm_device(dp.ptr, ds.ptr, n, 255)
+    finally:
+        for d in (dp, da, db, ds):
+            d.close()
+    assert orc.p2_compress(orc.p2_add(ra, rb)) == orc.p2_compress(rs)
+    assert orc.p2_compress(rs) != bytes([0xC0]) + bytes(95)
