@@ -61,7 +61,7 @@ def load_traffic():
     return None
 
 
-def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None, wbits_case=None):
+def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None, wbits_case=None, g2_case=None):
     """The oracle (C port of blst_p1s_mult_pippenger) on the host cores, and
     the checker of sampled batched results; with jac, also the oracle's batch
     normalisation on a 2^18-point sample of it; with wbits_case, the oracle's
@@ -104,6 +104,15 @@ def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None, wbits_ca
         wb_rep = {"value": ssc.shape[0] / dw, "unit": "points/s", "cores": 1, "kind": "port",
                   "sample": "oracle msm_mult_table over the 4096-point wbits = 10 table, 1 thread, %.3f s" % dw,
                   "bit_exact_vs_gpu": oracle.compress(rw) == gpu_wb}
+    g2_rep = None
+    if g2_case is not None:
+        gp, gs, gr = g2_case
+        t = time.perf_counter()
+        w2 = oracle.msm_p2_pippenger(gp, gs, gp.shape[0], 255)
+        d2 = time.perf_counter() - t
+        g2_rep = {"value": gp.shape[0] / d2, "unit": "points/s", "cores": 1, "kind": "port",
+                  "sample": "oracle G2 Pippenger over the first %d bases, 1 thread, %.3f s" % (gp.shape[0], d2),
+                  "bit_exact_vs_gpu": oracle.p2_compress(w2) == oracle.p2_compress(gr)}
     return {
         "value": n / dt,
         "unit": "points/s",
@@ -117,7 +126,29 @@ def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None, wbits_ca
         "batched_samples_bit_exact": checked,
         "p1s_to_affine": aff_rep,
         "p1s_mult_wbits": wb_rep,
+        "p2s_mult_pippenger": g2_rep,
     }
+
+
+def _same_p2(a, b):
+    """Two blst_p2 Jacobian results name the same point: X1 Z2^2 = X2 Z1^2
+    and Y1 Z2^3 = Y2 Z1^3 over Fp2, checked on the device-independent limbs
+    by Python big integers (Montgomery factors cancel on both sides)."""
+    P = 0x1A0111EA397FE69A4B1BA7B6434BACD764774B84F38512BF6730D2A0F6B0F6241EABFFFEB153FFFFB9FEFFFFFFFFAAAB
+
+    def f2(v, k):
+        return (sum(int(x) << (64 * i) for i, x in enumerate(v[12 * k:12 * k + 6])),
+                sum(int(x) << (64 * i) for i, x in enumerate(v[12 * k + 6:12 * k + 12])))
+
+    def mul(x, y):
+        return ((x[0] * y[0] - x[1] * y[1]) % P, (x[0] * y[1] + x[1] * y[0]) % P)
+
+    ax, ay, az = f2(a, 0), f2(a, 1), f2(a, 2)
+    bx, by, bz = f2(b, 0), f2(b, 1), f2(b, 2)
+    if az == (0, 0) or bz == (0, 0):
+        return az == bz
+    az2, bz2 = mul(az, az), mul(bz, bz)
+    return mul(ax, bz2) == mul(bx, az2) and mul(ay, mul(bz2, bz)) == mul(by, mul(az2, az))
 
 
 def entry_points(bh, points, scalars, r, n, reps=5):
@@ -187,7 +218,29 @@ def entry_points(bh, points, scalars, r, n, reps=5):
         rh = bh.mult_wbits(table, wb, ssc, 255)
     host_tab_s = (time.perf_counter() - t) / reps
     wb_same = bh.compress(rw) == bh.compress(rpl) == bh.compress(rh)
+    # G2: 2^16 resident bases through the device entry; a 4096-point sample
+    # through the drop-in entry for the oracle's comparison
+    gn = 1 << 16
+    g2p = bh.synth_p2_points(gn, SEED ^ 0x50)
+    g2s = bh.synth_scalars(gn, SEED ^ 0x51)
+    d_g2p, d_g2s = bh.DeviceBuffer(g2p), bh.DeviceBuffer(g2s)
+    try:
+        r2 = bh.p2_msm_device(d_g2p.ptr, d_g2s.ptr, gn, 255)
+        t = time.perf_counter()
+        for _ in range(4 * reps):
+            r2 = bh.p2_msm_device(d_g2p.ptr, d_g2s.ptr, gn, 255)
+        g2_s = (time.perf_counter() - t) / (4 * reps)
+    finally:
+        d_g2p.close()
+        d_g2s.close()
+    r2h = bh.p2s_mult_pippenger(g2p, g2s, 255)
+    g2_sample = (g2p[:4096].copy(), g2s[:4096].copy(), bh.p2s_mult_pippenger(g2p[:4096], g2s[:4096], 255))
     rep = {
+        "p2s_mult_pippenger": {
+            "workload": "G2 MSM, 2^16 bases [a + i b]G2 and uniform Fr scalars resident in HBM",
+            "device_ms": g2_s * 1e3, "device_points_per_s": gn / g2_s,
+            "host_entry_equals_device": _same_p2(r2h, r2),
+        },
         "p1s_mult_wbits": {
             "workload": "resident 4096-point KZG setup, wbits = 10 table (201 MB) in HBM, one MSM of uniform Fr scalars",
             "precompute_device_ms": pre_s * 1e3,
@@ -209,7 +262,7 @@ def entry_points(bh, points, scalars, r, n, reps=5):
             "equals_input": dev_same and np.array_equal(aff, points),
         },
     }
-    return rep, jac, (table, wb, ssc, bh.compress(rw))
+    return rep, jac, (table, wb, ssc, bh.compress(rw)), g2_sample
 
 
 def copy_calibration(nbytes, reps=3):
@@ -350,13 +403,13 @@ def main():
     out["compressed"] = gpu_bytes.hex()
     dp.close()
     ds.close()
-    jac = wbits_case = None
+    jac = wbits_case = g2_case = None
     if not args.no_entry_points:
-        ep, jac, wbits_case = entry_points(bh, points, scalars, r, n)
+        ep, jac, wbits_case, g2_case = entry_points(bh, points, scalars, r, n)
         out["entry_points"] = ep
-        print("fr: %.2f ms host, to_affine: %.3f ms device, wbits: %.3f ms device" % (
-            ep["p1s_mult_pippenger_fr"]["ms"], ep["p1s_to_affine"]["device_ms"], ep["p1s_mult_wbits"]["device_ms"]),
-            file=sys.stderr, flush=True)
+        print("fr: %.2f ms host, to_affine: %.3f ms device, wbits: %.3f ms device, g2 2^16: %.3f ms device" % (
+            ep["p1s_mult_pippenger_fr"]["ms"], ep["p1s_to_affine"]["device_ms"], ep["p1s_mult_wbits"]["device_ms"],
+            ep["p2s_mult_pippenger"]["device_ms"]), file=sys.stderr, flush=True)
 
     import workloads
 
@@ -377,13 +430,15 @@ def main():
                                                            sv["speedup"]), file=sys.stderr, flush=True)
         out["serving"] = sv
     if not args.no_cpu:
-        cb = cpu_baseline(points, scalars, n, gpu_bytes, samples, jac, wbits_case)
+        cb = cpu_baseline(points, scalars, n, gpu_bytes, samples, jac, wbits_case, g2_case)
         out["cpu_baseline"] = cb
         out["bit_exact"] = cb["bit_exact_vs_gpu"] and all(cb["batched_samples_bit_exact"].values())
         if "entry_points" in out:
             out["bit_exact"] = (out["bit_exact"] and out["entry_points"]["p1s_mult_pippenger_fr"]["equals_plain"]
                                 and out["entry_points"]["p1s_to_affine"]["equals_input"]
                                 and out["entry_points"]["p1s_mult_wbits"]["equals_plain"]
+                                and out["entry_points"]["p2s_mult_pippenger"]["host_entry_equals_device"]
+                                and cb["p2s_mult_pippenger"]["bit_exact_vs_gpu"]
                                 and cb["p1s_to_affine"]["equals_input"]
                                 and cb["p1s_mult_wbits"]["bit_exact_vs_gpu"])
     print(json.dumps(out), flush=True)

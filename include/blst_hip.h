@@ -310,6 +310,8 @@ void blst_hip_synth_srs_scalars(blst_byte *out32, size_t n, uint64_t seed);
 void blst_hip_synth_edge_cases(void *points96, blst_byte *scalars32, size_t n, uint64_t seed);
 void blst_hip_synth_fr(uint64_t *out4, const blst_byte *scalars32, size_t n);
 void blst_hip_synth_jacobian(void *points144, const void *affine96, size_t n, uint64_t seed);
+/* G2 bases [a + i b] G2 (affine, 192 bytes each) for a, b drawn from the seed */
+void blst_hip_synth_p2_points(void *points192, size_t n, uint64_t seed);
 
 /* 48-byte ZCash compressed encoding of a G1 Jacobian point, the format of
  * blst_p1_compress (bindings/blst_scalar.h:609, blst/c_api.cpp:1618-1621),

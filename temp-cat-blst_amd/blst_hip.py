@@ -56,6 +56,7 @@ SIGNATURES = {
     "blst_hip_synth_edge_cases": ([_vp, _vp, _sz, ctypes.c_uint64], None),
     "blst_hip_synth_fr": ([_vp, _vp, _sz], None),
     "blst_hip_synth_jacobian": ([_vp, _vp, _sz, ctypes.c_uint64], None),
+    "blst_hip_synth_p2_points": ([_vp, _sz, ctypes.c_uint64], None),
     "blst_hip_p1_compress": ([_vp, _vp], None),
     "blst_hip_msm_batch_device": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_batch_plan_create": ([_vp, _sz], _vp),
@@ -428,6 +429,13 @@ def synth_jacobian(affine, seed):
     a = np.ascontiguousarray(affine, dtype=np.uint64)
     out = np.zeros((a.shape[0], 18), np.uint64)
     load().blst_hip_synth_jacobian(_ptr(out), _ptr(a), a.shape[0], seed)
+    return out
+
+
+def synth_p2_points(n, seed):
+    """n G2 bases (n x 24 uint64 affine) in arithmetic progression [a + i b]G2."""
+    out = np.zeros((n, 24), np.uint64)
+    load().blst_hip_synth_p2_points(_ptr(out), n, seed)
     return out
 
 

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "host_ec.h"
+#include "host_ec2.h"
 
 namespace {
 
@@ -213,6 +214,89 @@ void blst_hip_synth_jacobian(void* points144, const void* affine96, size_t n, ui
         hostec::mul(out[i].x, x, l2);
         hostec::mul(out[i].y, y, l3);
         out[i].z = l;
+    }
+}
+
+// G2 bases P_i = A + i B, A = [a] G2 and B = [b] G2 for a, b uniform in Fr
+// from the seeded stream: one addition per point over Fp2 and one batched
+// normalisation (Montgomery's trick over ZZ * ZZZ); affine out, 24 limbs each
+void blst_hip_synth_p2_points(void* points192, size_t n, uint64_t seed) {
+    using hostec2::f2;
+    using hostec2::xyzz12;
+    if (n == 0) return;
+    // G2 generator, Montgomery form (blst/scalar/curve/point.cpp:114-138)
+    static const uint64_t kG2[24] = {
+        0xf5f28fa202940a10ULL, 0xb3f5fb2687b4961aULL, 0xa1a893b53e2ae580ULL, 0x9894999d1a3caee9ULL,
+        0x6f67b7631863366bULL, 0x058191924350bcd7ULL, 0xa5a9c0759e23f606ULL, 0xaaa0c59dbccd60c3ULL,
+        0x3bb17e18e2867806ULL, 0x1b1ab6cc8541b367ULL, 0xc2b6ed0ef2158547ULL, 0x11922a097360edf3ULL,
+        0x4c730af860494c4aULL, 0x597cfa1f5e369c5aULL, 0xe7e6856caa0a635aULL, 0xbbefb5e96e0d495fULL,
+        0x07d3a975f0ef25a2ULL, 0x0083fd8e7e80dae5ULL, 0xadc0fc92df64b05dULL, 0x18aa270a2b1461dcULL,
+        0x86adac6a3be4eba0ULL, 0x79495c4ec93da33aULL, 0xe7175850a43ccaedULL, 0x0b2bc2a163de1bf2ULL};
+    xyzz12 g;
+    memcpy(&g.x, kG2, 96);
+    memcpy(&g.y, kG2 + 12, 96);
+    memset(&g.zz, 0, sizeof(g.zz));
+    g.zz.c0 = hostec::one();
+    g.zzz = g.zz;
+    Rng rng(seed);
+    auto mult = [&](xyzz12& out) {
+        uint64_t k[4];
+        uniform_fr(rng, k);
+        xyzz12 acc;
+        memset(&acc, 0, sizeof(acc));
+        for (int b = 255; b >= 0; b--) {
+            hostec2::dbl(acc, acc);
+            if ((k[b / 64] >> (b % 64)) & 1) hostec2::addp(acc, acc, g);
+        }
+        out = acc;
+    };
+    xyzz12 A, B;
+    mult(A);
+    mult(B);
+    std::vector<xyzz12> pts(n);
+    pts[0] = A;
+    for (size_t i = 1; i < n; i++) hostec2::addp(pts[i], pts[i - 1], B);
+    // t_i = ZZ_i ZZZ_i; prefix products, one inversion, walk back
+    std::vector<f2> pre(n);
+    f2 acc;
+    memset(&acc, 0, sizeof(acc));
+    acc.c0 = hostec::one();
+    for (size_t i = 0; i < n; i++) {
+        pre[i] = acc;
+        if (!hostec2::is_inf(pts[i])) {
+            f2 t;
+            hostec2::mul(t, pts[i].zz, pts[i].zzz);
+            hostec2::mul(acc, acc, t);
+        }
+    }
+    // 1 / (c0 + c1 i) = (c0 - c1 i) / (c0^2 + c1^2)
+    hostec::fp6 nrm, t0, ninv;
+    hostec::mul(nrm, acc.c0, acc.c0);
+    hostec::mul(t0, acc.c1, acc.c1);
+    hostec::add(nrm, nrm, t0);
+    hostec::inv(ninv, nrm);
+    f2 inv;
+    hostec::mul(inv.c0, acc.c0, ninv);
+    hostec::mul(t0, acc.c1, ninv);
+    hostec::fp6 zero{};
+    hostec::sub(inv.c1, zero, t0);
+    uint64_t* out = static_cast<uint64_t*>(points192);
+    for (size_t i = n; i-- > 0;) {
+        uint64_t* o = out + 24 * i;
+        if (hostec2::is_inf(pts[i])) {
+            memset(o, 0, 192);
+            continue;
+        }
+        f2 it, t, izz, izzz, x, y;
+        hostec2::mul(it, inv, pre[i]);                 // 1 / (ZZ ZZZ)
+        hostec2::mul(t, pts[i].zz, pts[i].zzz);
+        hostec2::mul(inv, inv, t);
+        hostec2::mul(izz, it, pts[i].zzz);             // 1 / ZZ
+        hostec2::mul(izzz, it, pts[i].zz);             // 1 / ZZZ
+        hostec2::mul(x, pts[i].x, izz);
+        hostec2::mul(y, pts[i].y, izzz);
+        memcpy(o, &x, 96);
+        memcpy(o + 12, &y, 96);
     }
 }
 

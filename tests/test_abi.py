@@ -250,3 +250,18 @@ def test_g2_scratch_sizes_and_checks(orc):
         ret[:] = 7
         L.blst_p2s_tile_pippenger(P(ret), P(pp), 2, P(ss), 255, P(scr), 0, w)
         assert not ret.any()
+
+
+def test_g2_point_generator(orc):
+    """Synthetic G2 bases are an arithmetic progression of subgroup points:
+    on the curve, P_{i+1} - P_i constant, [r] P = infinity."""
+    import blst_hip
+    import pyref2
+    pts = blst_hip.synth_p2_points(6, 99)
+    aff = [pyref2.limbs_to_affine(list(p)) for p in pts]
+    assert all(pyref2.on_curve(a) for a in aff)
+    d = pyref2.add(aff[1], pyref2.neg(aff[0]))
+    for i in range(1, 5):
+        assert pyref2.add(aff[i + 1], pyref2.neg(aff[i])) == d
+    j = orc.p2_from_affine(pts[3])
+    assert orc.p2_compress(orc.p2_mult(j, pyref2.R_ORDER.to_bytes(32, "little"), 256)) == bytes([0xC0]) + bytes(95)
