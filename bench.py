@@ -418,7 +418,9 @@ def main():
         kz = workloads.kzg()
         sm = kz.pop("sample")
         samples["kzg"] = [(sm["srs"], sm["coeffs"][i], sm["results"][i]) for i in range(2)]
-        print("kzg: %.2f ms batched, %.1fx" % (kz["batched_ms"], kz["speedup"]), file=sys.stderr, flush=True)
+        print("kzg: %.2f ms batched, %.1fx; comb table %.2f ms" % (
+            kz["batched_ms"], kz["speedup"], kz["comb_table"]["batched_ms"] if kz["comb_table"] else float("nan")),
+            file=sys.stderr, flush=True)
         un = workloads.uneven()
         sm = un.pop("sample")
         samples["uneven"] = [(sm["points"], sm["scalars"], sm["result"])]
@@ -433,6 +435,8 @@ def main():
         cb = cpu_baseline(points, scalars, n, gpu_bytes, samples, jac, wbits_case, g2_case)
         out["cpu_baseline"] = cb
         out["bit_exact"] = cb["bit_exact_vs_gpu"] and all(cb["batched_samples_bit_exact"].values())
+        if "batched" in out and out["batched"]["kzg"]["comb_table"]:
+            out["bit_exact"] = out["bit_exact"] and out["batched"]["kzg"]["comb_table"]["equals_plain"]
         if "entry_points" in out:
             out["bit_exact"] = (out["bit_exact"] and out["entry_points"]["p1s_mult_pippenger_fr"]["equals_plain"]
                                 and out["entry_points"]["p1s_to_affine"]["equals_input"]

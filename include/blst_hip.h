@@ -179,6 +179,7 @@ int blst_hip_msm_device(blst_p1 *ret, const void *device_points, const void *dev
 void *blst_hip_device_upload(const void *host, size_t bytes);
 int blst_hip_device_write(void *device_dst, const void *host_src, size_t bytes);
 int blst_hip_device_read(void *host_dst, const void *device_src, size_t bytes);
+void *blst_hip_device_alloc(size_t bytes);   /* uninitialised; NULL on failure */
 void blst_hip_device_free(void *device_ptr);
 
 /* One MSM of a batch; points and scalars are device pointers (points
@@ -278,6 +279,20 @@ int blst_hip_p1s_mult_wbits_precompute_device(void *device_table, size_t wbits, 
                                               size_t npoints);
 int blst_hip_p1s_mult_wbits_device(blst_p1 *ret, const void *device_table, size_t wbits, size_t npoints,
                                    const void *device_scalars, size_t nbits);
+
+/* Comb tables for many MSMs over one resident setup (the KZG shape): window
+ * w of the table holds the wbits rows of [2^(wbits w)] P_i, D = 1 +
+ * nbits / wbits windows, 96 * D * npoints * 2^(wbits-1) bytes (0 for wbits
+ * outside [2, 14], nbits outside [1, 256] or on overflow).  An MSM over it is
+ * one sum of D * npoints gathered entries, with no doublings and no window
+ * fold.  _build fills a device table from device points; _msm_batch runs
+ * njobs MSMs (device scalars, job-major, npoints * ceil(nbits/8) bytes per
+ * job) and writes njobs results.  Both return when done, 0 or -1. */
+size_t blst_hip_comb_table_bytes(size_t npoints, size_t wbits, size_t nbits);
+int blst_hip_comb_table_build(void *device_table, size_t wbits, size_t nbits, const void *device_points,
+                              size_t npoints);
+int blst_hip_comb_msm_batch(blst_p1 *rets, const void *device_table, size_t wbits, size_t npoints,
+                            const void *device_scalars, size_t njobs, size_t nbits);
 
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),

@@ -32,6 +32,9 @@ SIGNATURES = {
     "blst_p1s_mult_wbits": ([_vp, _vp, _sz, _sz, _vp, _sz, _vp], None),
     "blst_hip_p1s_mult_wbits_precompute_device": ([_vp, _sz, _vp, _sz], ctypes.c_int),
     "blst_hip_p1s_mult_wbits_device": ([_vp, _vp, _sz, _sz, _vp, _sz], ctypes.c_int),
+    "blst_hip_comb_table_bytes": ([_sz, _sz, _sz], _sz),
+    "blst_hip_comb_table_build": ([_vp, _sz, _sz, _vp, _sz], ctypes.c_int),
+    "blst_hip_comb_msm_batch": ([_vp, _vp, _sz, _sz, _vp, _sz, _sz], ctypes.c_int),
     "blst_hip_p1s_to_affine_device": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_p2s_mult_pippenger_scratch_sizeof": ([_sz], _sz),
     "blst_p2s_mult_pippenger_window_scratch_sizeof": ([_sz], _sz),
@@ -50,6 +53,7 @@ SIGNATURES = {
     "blst_hip_device_free": ([_vp], None),
     "blst_hip_device_write": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_device_read": ([_vp, _vp, _sz], ctypes.c_int),
+    "blst_hip_device_alloc": ([_sz], _vp),
     "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
     "blst_hip_synth_scalars": ([_vp, _sz, ctypes.c_uint64, ctypes.c_int], None),
     "blst_hip_synth_srs_scalars": ([_vp, _sz, ctypes.c_uint64], None),
@@ -272,19 +276,41 @@ def mult_wbits_device(d_table_ptr, wbits, n, d_scalars_ptr, nbits):
     return ret
 
 
+def comb_table_bytes(n, wbits, nbits=255):
+    return load().blst_hip_comb_table_bytes(n, wbits, nbits)
+
+
+def comb_table_build(d_table_ptr, wbits, d_points_ptr, n, nbits=255):
+    if load().blst_hip_comb_table_build(d_table_ptr, wbits, nbits, d_points_ptr, n) != 0:
+        raise RuntimeError("blst_hip_comb_table_build: " + last_error())
+
+
+def comb_msm_batch(d_table_ptr, wbits, n, d_scalars_ptr, njobs, nbits=255):
+    rets = np.zeros((njobs, 18), np.uint64)
+    if load().blst_hip_comb_msm_batch(_ptr(rets), d_table_ptr, wbits, n, d_scalars_ptr, njobs, nbits) != 0:
+        raise RuntimeError("blst_hip_comb_msm_batch: " + last_error())
+    return rets
+
+
 class DeviceAlloc:
-    """Uninitialised-looking device memory of a given byte size (zeroed)."""
+    """Uninitialised device memory of a given byte size (freed on close)."""
 
     def __init__(self, nbytes):
-        self.buf = DeviceBuffer(np.zeros(nbytes, np.uint8))
-        self.ptr = self.buf.ptr
+        self.ptr = load().blst_hip_device_alloc(nbytes)
+        if not self.ptr:
+            raise RuntimeError("blst_hip_device_alloc: " + last_error())
         self.nbytes = nbytes
 
     def read(self):
-        return self.buf.read()
+        out = np.zeros(self.nbytes, np.uint8)
+        if load().blst_hip_device_read(_ptr(out), ctypes.c_void_p(self.ptr), self.nbytes):
+            raise RuntimeError("blst_hip_device_read: " + last_error())
+        return out
 
     def close(self):
-        self.buf.close()
+        if self.ptr:
+            load().blst_hip_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = None
 
 
 def p2s_mult_pippenger(points, scalars, nbits, window=0, scratch=True, mode="contiguous", k=1, bit0=None):

@@ -44,6 +44,7 @@ struct Ctx {
     DevBuf gtable, gen_tmp, gen_out;     // comb table of G, fixed-base scratch
     DevBuf jac_in, aff_out;              // batch normalisation staging
     DevBuf wb_pts, wb_rows, wb_table, wb_list, wb_scl, wb_part, wb_W;   // fixed-base tables
+    DevBuf comb_q, comb_x;               // comb-table build: shifted bases
     void* wb_host = nullptr;             // page-locked gather staging
     size_t wb_host_cap = 0;
     bool gtable_ready = false;
@@ -948,6 +949,19 @@ void host_mult_wbits(Ctx& cx, blst_p1* ret, const blst_p1_affine* table, size_t 
     std::memcpy(ret, &j, sizeof(j));
 }
 
+// ---------------------------------------------------------------- comb tables
+
+// bytes of a comb table: D windows of n rows of 2^(wbits-1) affine entries;
+// 0 for invalid parameters or on overflow
+size_t comb_table_bytes(size_t npoints, size_t wbits, size_t nbits) {
+    if (!valid_wbits(wbits) || nbits == 0 || nbits > 256 || npoints == 0) return 0;
+    const size_t D = 1 + nbits / wbits;
+    size_t b;
+    if (__builtin_mul_overflow(npoints, D * sizeof(blst_p1_affine), &b)) return 0;
+    if (b > (SIZE_MAX >> (wbits - 1))) return 0;
+    return b << (wbits - 1);
+}
+
 }  // namespace host
 }  // namespace bls
 
@@ -1476,6 +1490,14 @@ void blst_hip_host_free(void* host_ptr) {
     if (host_ptr) (void)hipHostFree(host_ptr);
 }
 
+void* blst_hip_device_alloc(size_t bytes) {
+    void* d = nullptr;
+    if (bytes == 0) { g_err = "invalid arguments"; return nullptr; }
+    if (hipMalloc(&d, bytes) != hipSuccess) { g_err = "hipMalloc failed"; return nullptr; }
+    g_err.clear();
+    return d;
+}
+
 int blst_hip_device_read(void* host_dst, const void* device_src, size_t bytes) {
     if (!host_dst || !device_src) { g_err = "invalid arguments"; return -1; }
     if (hipMemcpy(host_dst, device_src, bytes, hipMemcpyDeviceToHost) != hipSuccess) {
@@ -1683,6 +1705,103 @@ int blst_hip_p1s_mult_wbits_device(blst_p1* ret, const void* device_table, size_
     } catch (const HipFail& f) {
         g_err = f.msg;
         std::memset(ret, 0, sizeof(*ret));
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+size_t blst_hip_comb_table_bytes(size_t npoints, size_t wbits, size_t nbits) {
+    return comb_table_bytes(npoints, wbits, nbits);
+}
+
+int blst_hip_comb_table_build(void* device_table, size_t wbits, size_t nbits, const void* device_points,
+                              size_t npoints) {
+    if (npoints == 0) { g_err.clear(); return 0; }
+    if (!device_table || !device_points || comb_table_bytes(npoints, wbits, nbits) == 0 ||
+        npoints >= 0x7fffffffULL || ((uintptr_t)device_table % 16) || ((uintptr_t)device_points % 16)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        const size_t D = 1 + nbits / wbits, m = (size_t)1 << (wbits - 1);
+        cx.comb_q.ensure(npoints * sizeof(blst_p1_affine));
+        cx.comb_x.ensure(npoints * sizeof(xyzz));
+        HIPCK(hipMemcpyAsync(cx.comb_q.p, device_points, npoints * sizeof(blst_p1_affine), hipMemcpyDeviceToDevice,
+                             cx.stream));
+        const size_t R = std::max<size_t>(1, kWbitsRowEntries / m);
+        for (size_t w = 0; w < D; w++) {
+            uint8_t* slice = static_cast<uint8_t*>(device_table) + w * npoints * m * sizeof(blst_p1_affine);
+            for (size_t row0 = 0; row0 < npoints; row0 += R) {
+                const size_t nr = std::min(R, npoints - row0);
+                enqueue_wbits_table(cx, cx.comb_q.as<uint8_t>() + row0 * sizeof(blst_p1_affine), nr, wbits,
+                                    slice + row0 * m * sizeof(blst_p1_affine));
+            }
+            if (w + 1 < D) {
+                // the next window's bases: [2^wbits] Q
+                launch_k_shift_points(cx.stream, cx.comb_q.as<const uint4>(), (uint32_t)npoints, (uint32_t)wbits,
+                                      cx.comb_x.as<xyzz>());
+                const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, npoints / kAffineLanes));
+                launch_k_to_affine(dim3(div_up(div_up(npoints, Q), 256)), dim3(256), cx.stream,
+                                   cx.comb_x.as<const xyzz>(), cx.comb_q.as<aff>(), (uint32_t)npoints, Q);
+                HIPCK(hipGetLastError());
+            }
+        }
+        HIPCK(hipStreamSynchronize(cx.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_comb_msm_batch(blst_p1* rets, const void* device_table, size_t wbits, size_t npoints,
+                            const void* device_scalars, size_t njobs, size_t nbits) {
+    if (njobs == 0) { g_err.clear(); return 0; }
+    if (!rets) { g_err = "invalid arguments"; return -1; }
+    std::memset(rets, 0, njobs * sizeof(blst_p1));
+    if (npoints == 0 || nbits == 0) { g_err.clear(); return 0; }
+    if (!device_table || !device_scalars || comb_table_bytes(npoints, wbits, nbits) == 0 ||
+        npoints >= 0x7fffffffULL || njobs >= 0x7fffffffULL || ((uintptr_t)device_table % 16)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        const size_t D = 1 + nbits / wbits, nbytes = (nbits + 7) / 8;
+        const size_t terms = D * npoints;
+        if (terms >= 0xffffffffULL) throw std::runtime_error("too many table terms");
+        const size_t C = std::min(terms, std::max<size_t>(4, (terms * njobs + 4 * kWbitsSumLanes - 1) /
+                                                                 (4 * kWbitsSumLanes)));
+        const size_t nch = (terms + C - 1) / C;
+        if (nch * njobs >= 0xffffffffULL) throw std::runtime_error("batch too large");
+        cx.wb_part.ensure(njobs * nch * sizeof(xyzz));
+        cx.wb_W.ensure(njobs * sizeof(xyzz));
+        const uint32_t vec = (nbytes == 32 && ((uintptr_t)device_scalars % 16) == 0) ? 1u : 0u;
+        launch_k_comb_sum(cx.stream, static_cast<const uint4*>(device_table),
+                          static_cast<const uint8_t*>(device_scalars), (uint32_t)nbytes, (uint32_t)nbits, vec,
+                          (uint32_t)npoints, (uint32_t)wbits, (uint32_t)D, (uint32_t)njobs, (uint32_t)C,
+                          (uint32_t)nch, cx.wb_part.as<xyzz>());
+        launch_k_wbits_tree(cx.stream, cx.wb_part.as<const xyzz>(), (uint32_t)nch, (uint32_t)njobs,
+                            cx.wb_W.as<xyzz>());
+        HIPCK(hipGetLastError());
+        std::vector<hostec::xyzz6> res(njobs);
+        HIPCK(hipMemcpyAsync(res.data(), cx.wb_W.p, njobs * sizeof(xyzz), hipMemcpyDeviceToHost, cx.stream));
+        HIPCK(hipStreamSynchronize(cx.stream));
+        for (size_t k = 0; k < njobs; k++) {
+            hostec::jac6 j;
+            hostec::to_jac(j, res[k]);
+            std::memcpy(&rets[k], &j, sizeof(j));
+        }
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
         return -1;
     }
     g_err.clear();

@@ -138,6 +138,72 @@ k_wbits_tree(const xyzz* __restrict__ part, uint32_t nch, xyzz* __restrict__ W) 
     }
 }
 
+// Comb tables of a resident setup: window w of the table holds the rows of
+// Q_i = [2^(wbits w)] P_i, so sum_i k_i P_i = sum_(w, i) sign(d) row_(w, i)[|d| - 1]
+// with no doublings between windows.  k_shift_points computes the next
+// window's bases: out_i = [2^s] in_i (XYZZ, then k_to_affine).
+__global__ void __launch_bounds__(256)
+k_shift_points(const uint4* __restrict__ in, uint32_t n, uint32_t s, xyzz* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    aff a;
+    load_aff_at(a, in, i);
+    xyzz acc;
+    xyzz_set_inf(acc);
+    if (!aff_is_inf(a)) {
+        xyzz_from_aff(acc, a);
+#pragma unroll 1
+        for (uint32_t k = 0; k < s; k++) xyzz_dbl_inl(acc, acc);
+    }
+    store_xyzz(&out[i], acc);
+}
+
+// Batched sums over a comb table: lane (job, c) adds the terms t in
+// [c C, min((c + 1) C, D n)) of its job, term t = w n + i being the entry of
+// point i's Booth digit in window w, into part[job * nch + c].  Scalars are
+// job-major (job j's n scalars at scalars + j n nbytes).
+__global__ void __launch_bounds__(256)
+k_comb_sum(const uint4* __restrict__ table, const uint8_t* __restrict__ scalars, uint32_t nbytes, uint32_t nbits,
+           uint32_t vec, uint32_t n, uint32_t wbits, uint32_t D, uint32_t njobs, uint32_t C, uint32_t nch,
+           xyzz* __restrict__ part) {
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= njobs * nch) return;
+    const uint32_t job = g / nch, c = g % nch;
+    const uint32_t terms = D * n;
+    const uint32_t t0 = c * C, t1 = min(t0 + C, terms);
+    const size_t m = (size_t)1 << (wbits - 1);
+    const uint8_t* sc = scalars + (size_t)job * n * nbytes;
+    xyzz acc;
+    xyzz_set_inf(acc);
+    uint32_t w = t0 / n, i = t0 % n;
+    for (uint32_t t = t0; t < t1; t++) {
+        const int32_t d = booth_digit_at(sc + (size_t)i * nbytes, vec != 0, w * wbits, wbits, nbits);
+        if (d != 0) {
+            aff a;
+            load_aff_at(a, table, ((size_t)w * n + i) * m + (size_t)(d < 0 ? -d : d) - 1);
+            if (!aff_is_inf(a)) {
+                if (d < 0) fp_neg(a.y, a.y);
+                xyzz_madd(acc, a);
+            }
+        }
+        if (++i == n) { i = 0; w++; }
+    }
+    store_xyzz(&part[g], acc);
+}
+
+void launch_k_shift_points(hipStream_t stream, const uint4* __restrict__ in, uint32_t n, uint32_t s,
+                           xyzz* __restrict__ out) {
+    hipLaunchKernelGGL(k_shift_points, dim3((n + 255) / 256), dim3(256), 0, stream, in, n, s, out);
+}
+
+void launch_k_comb_sum(hipStream_t stream, const uint4* __restrict__ table, const uint8_t* __restrict__ scalars,
+                       uint32_t nbytes, uint32_t nbits, uint32_t vec, uint32_t n, uint32_t wbits, uint32_t D,
+                       uint32_t njobs, uint32_t C, uint32_t nch, xyzz* __restrict__ part) {
+    const uint32_t lanes = njobs * nch;
+    hipLaunchKernelGGL(k_comb_sum, dim3((lanes + 255) / 256), dim3(256), 0, stream, table, scalars, nbytes, nbits,
+                       vec, n, wbits, D, njobs, C, nch, part);
+}
+
 void launch_k_wbits_rows(hipStream_t stream, const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint32_t m,
                          uint32_t L, xyzz* __restrict__ out) {
     const uint32_t lanes = nrows * L;

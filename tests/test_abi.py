@@ -265,3 +265,13 @@ def test_g2_point_generator(orc):
         assert pyref2.add(aff[i + 1], pyref2.neg(aff[i])) == d
     j = orc.p2_from_affine(pts[3])
     assert orc.p2_compress(orc.p2_mult(j, pyref2.R_ORDER.to_bytes(32, "little"), 256)) == bytes([0xC0]) + bytes(95)
+
+
+def test_comb_table_bytes():
+    import blst_hip
+    L = blst_hip.load()
+    assert L.blst_hip_comb_table_bytes(4096, 10, 255) == 26 * 4096 * 512 * 96
+    assert L.blst_hip_comb_table_bytes(4096, 14, 255) == 19 * 4096 * 8192 * 96
+    assert L.blst_hip_comb_table_bytes(7, 2, 1) == 1 * 7 * 2 * 96
+    for args in ((0, 8, 255), (5, 1, 255), (5, 15, 255), (5, 8, 0), (5, 8, 257), ((1 << 62), 14, 255)):
+        assert L.blst_hip_comb_table_bytes(*args) == 0, args

@@ -709,3 +709,32 @@ def test_g2_device_linearity_2pow15(hip, orc):
             d.close()
     assert orc.p2_compress(orc.p2_add(ra, rb)) == orc.p2_compress(rs)
     assert orc.p2_compress(rs) != bytes([0xC0]) + bytes(95)
+
+
+# ---------------------------------------------------------------- comb tables
+
+@pytest.mark.parametrize("wbits,nbits,n,njobs", [(8, 255, 512, 5), (3, 64, 100, 3), (11, 256, 64, 2), (2, 1, 9, 2)])
+def test_comb_batch_matches_plain(hip, orc, wbits, nbits, n, njobs):
+    """Batched MSMs over a comb table equal the plain MSM of each job (and the
+    oracle for the first)."""
+    pts = hip.generator_multiples(hip.synth_srs_scalars(n, 0x90 + n))
+    pts[min(3, n - 1)] = 0
+    nb = (nbits + 7) // 8
+    rnd = random.Random(wbits * 7 + nbits)
+    S = _scal([rnd.getrandbits(8 * nb) for _ in range(n * njobs)], nb).reshape(njobs, n, nb)
+    S[0, 0] = 0
+    size = hip.comb_table_bytes(n, wbits, nbits)
+    d_pts = hip.DeviceBuffer(pts)
+    d_tab = hip.DeviceAlloc(size)
+    d_sc = hip.DeviceBuffer(S)
+    try:
+        hip.comb_table_build(d_tab.ptr, wbits, d_pts.ptr, n, nbits)
+        got = hip.comb_msm_batch(d_tab.ptr, wbits, n, d_sc.ptr, njobs, nbits)
+        plain = [hip.msm_device(d_pts.ptr, d_sc.ptr + k * n * nb, n, nbits) for k in range(njobs)]
+    finally:
+        d_pts.close()
+        d_tab.close()
+        d_sc.close()
+    for k in range(njobs):
+        assert hip.compress(got[k]) == hip.compress(plain[k]), k
+    assert hip.compress(got[0]) == orc.compress(orc.msm_pippenger(pts, S[0], n, nbits))
