@@ -54,6 +54,36 @@ k_reduce_segments(const group_t* __restrict__ groups, const uint32_t* __restrict
     store_xyzz(&nodes[2 * (size_t)t + 1], acc);
 }
 
+// The same segment sums with one quad per segment (four-lane additions):
+// the G2 form, whose one-lane version would hold four Fp2 points per lane.
+template <class F>
+__global__ void __launch_bounds__(256)
+k_reduce_segments_quad(const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase,
+                       uint32_t ngroups, uint32_t total_segments, const xyzz_t<F>* __restrict__ buckets,
+                       const uint32_t* __restrict__ cnt, xyzz_t<F>* __restrict__ nodes) {
+    const uint32_t t = (blockIdx.x * blockDim.x + threadIdx.x) >> 2;   // uniform per quad
+    if (t >= total_segments) return;
+    const uint32_t g = find_owner(sbase, ngroups, t);
+    const group_t G = groups[g];
+    const uint32_t s = t - G.sb;
+    const size_t base = (size_t)G.kb + (size_t)s * G.L;
+    F acc, tot, y;
+    fp_zero(acc);
+    fp_zero(tot);
+#pragma unroll 1
+    for (uint32_t b = G.L; b-- > 0;) {
+        const size_t key = base + b;
+        if (cnt[key]) quad_load(y, &buckets[key]);   // empty buckets hold no data
+        else fp_zero(y);
+        quad_add(acc, acc, y);
+        quad_add(tot, tot, acc);
+    }
+    quad_store(&nodes[2 * (size_t)t], tot);
+#pragma unroll 1
+    for (uint32_t l = G.L; l > 1; l >>= 1) quad_dbl(acc, acc);
+    quad_store(&nodes[2 * (size_t)t + 1], acc);
+}
+
 // One workgroup merges n <= 64 consecutive nodes of a group into one.  Quads
 // 0-31 (waves 0-1) compute the A of the merges and quads 32-63 (waves 2-3)
 // the V, so every wave follows one instruction stream; a level with fewer
@@ -115,9 +145,16 @@ k_reduce_tree(const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, ui
     }
 }
 
+// G1: one lane per segment (grid and block from the caller); G2: one quad
+// per segment
 template <class F>
 void launch_k_reduce_segments(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const uint32_t* __restrict__ sbase, uint32_t ngroups, uint32_t total_segments, const xyzz_t<F>* __restrict__ buckets, const uint32_t* __restrict__ cnt, xyzz_t<F>* __restrict__ nodes) {
-    hipLaunchKernelGGL(k_reduce_segments<F>, grid, block, 0, stream, groups, sbase, ngroups, total_segments, buckets, cnt, nodes);
+    if constexpr (sizeof(F) == sizeof(fp2)) {
+        const uint32_t threads = 4 * total_segments;
+        hipLaunchKernelGGL(k_reduce_segments_quad<F>, dim3((threads + 255) / 256), dim3(256), 0, stream, groups, sbase, ngroups, total_segments, buckets, cnt, nodes);
+    } else {
+        hipLaunchKernelGGL(k_reduce_segments<F>, grid, block, 0, stream, groups, sbase, ngroups, total_segments, buckets, cnt, nodes);
+    }
 }
 
 template <class F>
