@@ -111,9 +111,8 @@ __device__ __forceinline__ void xyzz_madd(xyzz_t<F>& acc, const aff_t<F>& a) {
     fp_sub(t, t, Q);
     fp_sub(acc.x, t, Q);     // X3 = R^2 - PPP - 2Q
     fp_sub(Q, Q, acc.x);
-    fp_mul(Q, Q, R);
-    fp_mul(t, acc.y, PPP);
-    fp_sub(acc.y, Q, t);     // Y3 = R(Q - X3) - Y1*PPP
+    fp_neg(t, acc.y);
+    fp_mul_sum(acc.y, Q, R, t, PPP);   // Y3 = R(Q - X3) - Y1*PPP, one reduction
     fp_mul(acc.zz, acc.zz, PP);
     fp_mul(acc.zzz, acc.zzz, PPP);
 }
@@ -150,9 +149,8 @@ __device__ __forceinline__ void xyzz_add(xyzz_t<F>& r, const xyzz_t<F>& p, const
     fp_sub(t, t, Q);
     fp_sub(o.x, t, Q);
     fp_sub(Q, Q, o.x);
-    fp_mul(Q, Q, R);
-    fp_mul(t, S1, PPP);
-    fp_sub(o.y, Q, t);
+    fp_neg(t, S1);
+    fp_mul_sum(o.y, Q, R, t, PPP);     // Y3 = R(Q - X3) - S1*PPP
     fp_mul(t, p.zz, q.zz);
     fp_mul(o.zz, t, PP);
     fp_mul(t, p.zzz, q.zzz);

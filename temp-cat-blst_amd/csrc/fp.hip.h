@@ -339,6 +339,119 @@ __device__ __forceinline__ void fp_mul(fp& r, const fp& a, const fp& b) {
 
 __device__ __forceinline__ void fp_sqr(fp& r, const fp& a) { fp_mul(r, a, a); }
 
+// Sum of two products with one reduction: three products per column index
+// (a_i*b_j, c_i*d_j, m_i*p_j), up to 36 per column, so the column sum stays
+// below 2^70 and fits the 96-bit accumulator.
+#define BLS_MAC3(FIRST)                                         \
+    "v_mad_u64_u32 %0, vcc, %2, %3, %0\n\t" FIRST               \
+    "v_mad_u64_u32 %0, vcc, %4, %5, %0\n\t"                     \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %6, %7, %0\n\t"                     \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+#define BLS_MAC6(FIRST)                                         \
+    BLS_MAC3(FIRST) "\n\t"                                      \
+    "v_mad_u64_u32 %0, vcc, %8, %9, %0\n\t"                     \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %10, %11, %0\n\t"                   \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"                  \
+    "v_mad_u64_u32 %0, vcc, %12, %13, %0\n\t"                   \
+    "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc"
+#define BLS_ADDC_TOP "v_addc_co_u32_e32 %1, vcc, 0, %1, vcc\n\t"
+#define BLS_NEW_TOP "v_cndmask_b32_e64 %1, 0, 1, vcc\n\t"
+
+// acc += a0*b0 + c0*d0 + m0*p0
+template <bool OPENS>
+__device__ __forceinline__ void mac3(uint64_t& acc, uint32_t& top, uint32_t a0, uint32_t b0, uint32_t c0,
+                                     uint32_t d0, uint32_t m0, uint32_t p0) {
+    if constexpr (OPENS)
+        asm(BLS_MAC3(BLS_NEW_TOP)
+            : "+v"(acc), "=&v"(top)
+            : "v"(a0), "v"(b0), "v"(c0), "v"(d0), "v"(m0), "s"(p0)
+            : "vcc");
+    else
+        asm(BLS_MAC3(BLS_ADDC_TOP)
+            : "+v"(acc), "+v"(top)
+            : "v"(a0), "v"(b0), "v"(c0), "v"(d0), "v"(m0), "s"(p0)
+            : "vcc");
+}
+
+// two column indices of the above
+template <bool OPENS>
+__device__ __forceinline__ void mac6(uint64_t& acc, uint32_t& top, uint32_t a0, uint32_t b0, uint32_t c0,
+                                     uint32_t d0, uint32_t m0, uint32_t p0, uint32_t a1, uint32_t b1,
+                                     uint32_t c1, uint32_t d1, uint32_t m1, uint32_t p1) {
+    if constexpr (OPENS)
+        asm(BLS_MAC6(BLS_NEW_TOP)
+            : "+v"(acc), "=&v"(top)
+            : "v"(a0), "v"(b0), "v"(c0), "v"(d0), "v"(m0), "s"(p0), "v"(a1), "v"(b1), "v"(c1), "v"(d1),
+              "v"(m1), "s"(p1)
+            : "vcc");
+    else
+        asm(BLS_MAC6(BLS_ADDC_TOP)
+            : "+v"(acc), "+v"(top)
+            : "v"(a0), "v"(b0), "v"(c0), "v"(d0), "v"(m0), "s"(p0), "v"(a1), "v"(b1), "v"(c1), "v"(d1),
+              "v"(m1), "s"(p1)
+            : "vcc");
+}
+
+#undef BLS_MAC6
+#undef BLS_MAC3
+#undef BLS_ADDC_TOP
+#undef BLS_NEW_TOP
+
+// the column's triples a_i*b_(k-i) + c_i*d_(k-i) + m_i*p_(k-i), i in [lo, hi)
+template <int K, int LO, int HI>
+__device__ __forceinline__ void mac_triples(uint64_t& acc, uint32_t& top, const fp& a, const fp& b,
+                                            const fp& c, const fp& d, const uint32_t m[12]) {
+    constexpr bool opens = LO == 0 || (K >= 12 && LO == K - 11);
+    if constexpr (HI - LO >= 2) {
+        mac6<opens>(acc, top, a.l[LO], b.l[K - LO], c.l[LO], d.l[K - LO], m[LO], p_limb(K - LO),
+                    a.l[LO + 1], b.l[K - LO - 1], c.l[LO + 1], d.l[K - LO - 1], m[LO + 1], p_limb(K - LO - 1));
+        mac_triples<K, LO + 2, HI>(acc, top, a, b, c, d, m);
+    } else if constexpr (HI - LO == 1) {
+        mac3<opens>(acc, top, a.l[LO], b.l[K - LO], c.l[LO], d.l[K - LO], m[LO], p_limb(K - LO));
+    }
+}
+
+template <int K>
+__device__ __forceinline__ void mul_sum_low_column(uint64_t& acc, uint32_t& top, const fp& a, const fp& b,
+                                                   const fp& c, const fp& d, uint32_t m[12]) {
+    if constexpr (K == 0) top = 0;
+    mac_triples<K, 0, K>(acc, top, a, b, c, d, m);
+    mac1(acc, top, a.l[K], b.l[0]);
+    mac1(acc, top, c.l[K], d.l[0]);
+    m[K] = (uint32_t)acc * BLS_N0;
+    mac1_ps(acc, top, m[K], p_limb(0));
+    next_column(acc, top);
+    if constexpr (K + 1 < 12) mul_sum_low_column<K + 1>(acc, top, a, b, c, d, m);
+}
+
+template <int K>
+__device__ __forceinline__ void mul_sum_high_column(uint64_t& acc, uint32_t& top, const fp& a, const fp& b,
+                                                    const fp& c, const fp& d, const uint32_t m[12],
+                                                    uint32_t t[12]) {
+    mac_triples<K, K - 11, 12>(acc, top, a, b, c, d, m);
+    t[K - 12] = (uint32_t)acc;
+    next_column(acc, top);
+    if constexpr (K + 1 < 23) mul_sum_high_column<K + 1>(acc, top, a, b, c, d, m, t);
+}
+
+// r = (a*b + c*d) / R mod p: the columns of fp_mul with both products in
+// them and one Montgomery reduction.  Inputs below 2p give a*b + c*d < 8p^2,
+// so the result is below 8p^2/R + p < 1.9p: no final subtraction, as in
+// fp_mul.
+__device__ __forceinline__ void fp_mul_sum(fp& r, const fp& a, const fp& b, const fp& c, const fp& d) {
+    uint32_t m[12];
+    uint32_t t[12];
+    uint64_t acc = 0;
+    uint32_t top = 0;
+    mul_sum_low_column<0>(acc, top, a, b, c, d, m);
+    mul_sum_high_column<12>(acc, top, a, b, c, d, m, t);
+    t[11] = (uint32_t)acc;
+#pragma unroll
+    for (int i = 0; i < 12; i++) r.l[i] = t[i];
+}
+
 // a^(p-2) = a^-1 (Fermat), left-to-right over the fixed exponent bits; the
 // branch is uniform across the wave.  0 maps to 0.
 __device__ __forceinline__ void fp_inv(fp& r, const fp& a) {

@@ -42,6 +42,14 @@ __device__ __forceinline__ void fp_mul(fp2& r, const fp2& a, const fp2& b) {
     fp_sub(r.c1, s, t1);
 }
 
+// a*b + c*d as two Fp2 products and a sum (the Fp form shares one reduction)
+__device__ __forceinline__ void fp_mul_sum(fp2& r, const fp2& a, const fp2& b, const fp2& c, const fp2& d) {
+    fp2 u, v;
+    fp_mul(u, a, b);
+    fp_mul(v, c, d);
+    fp_add(r, u, v);
+}
+
 // two Fp products: c0 = (a0 + a1)(a0 - a1), c1 = 2 a0 a1
 __device__ __forceinline__ void fp_sqr(fp2& r, const fp2& a) {
     fp s, d, m;

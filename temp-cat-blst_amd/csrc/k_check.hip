@@ -1,7 +1,7 @@
 // Self-check of the four-lane point arithmetic: on valid points, every case
 // of an addition (general, equal points in different frames, opposite
 // points, infinity on either side) and of a doubling must give the same
-// coordinates as the one-lane formulas of ec.hip.h.
+// coordinates (as field elements) as the one-lane formulas of ec.hip.h.
 #include "msm_common.hip.h"
 #include "ec_quad.hip.h"
 #include "launch.h"
@@ -14,10 +14,16 @@ __device__ __forceinline__ void xyzz_coord(fp& r, const xyzz& p, uint32_t ln) {
     fp_sel(r, ln == 3, p.zzz, r);
 }
 
+// the two forms may return different representatives in [0, 2p) of one
+// field element (the one-lane Y3 shares a reduction between two products),
+// so elements are compared in canonical form
 __device__ __forceinline__ uint32_t fp_differs(const fp& a, const fp& b) {
+    fp ca, cb;
+    fp_canon(ca, a);
+    fp_canon(cb, b);
     uint32_t d = 0;
 #pragma unroll
-    for (int k = 0; k < 12; k++) d |= a.l[k] ^ b.l[k];
+    for (int k = 0; k < 12; k++) d |= ca.l[k] ^ cb.l[k];
     return d != 0;
 }
 
