@@ -72,9 +72,17 @@ k_count(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const u
     const uint32_t i0 = k * S.ch, i1 = min(i0 + S.ch, J.n);
     const uint32_t bit0 = J.bit_base + S.w * c;
     if (bit0 <= nbits) {
-        for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-            const int32_t d = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
-            if (d) atomicAdd(&bins[(uint32_t)(d < 0 ? -d : d) - 1u], 1u);
+        // kSortBatch digits per step: their scalar loads are issued together
+        for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortBatch * blockDim.x) {
+            int32_t d[kSortBatch];
+#pragma unroll
+            for (uint32_t q = 0; q < kSortBatch; q++) {
+                const uint32_t iq = i + q * blockDim.x;
+                d[q] = iq < i1 ? booth_digit_at(scalars + (size_t)iq * nbytes, vec, bit0, c, nbits) : 0;
+            }
+#pragma unroll
+            for (uint32_t q = 0; q < kSortBatch; q++)
+                if (d[q]) atomicAdd(&bins[(uint32_t)(d[q] < 0 ? -d[q] : d[q]) - 1u], 1u);
         }
     }
     __syncthreads();
@@ -126,13 +134,21 @@ k_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const u
     const uint32_t i0 = k * S.ch, i1 = min(i0 + S.ch, J.n);
     const uint32_t bit0 = J.bit_base + S.w * c;
     if (bit0 > nbits) return;
-    for (uint32_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
-        const int32_t d = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
-        if (d) {
-            const uint32_t neg = d < 0;
-            const uint32_t b = (uint32_t)(neg ? -d : d) - 1u;
-            const uint32_t pos = atomicAdd(&cursor[b], 1u);
-            sorted[pos] = make_uint2(kb + b, i | (neg << 31));
+    for (uint32_t i = i0 + threadIdx.x; i < i1; i += kSortBatch * blockDim.x) {
+        int32_t d[kSortBatch];
+#pragma unroll
+        for (uint32_t q = 0; q < kSortBatch; q++) {
+            const uint32_t iq = i + q * blockDim.x;
+            d[q] = iq < i1 ? booth_digit_at(scalars + (size_t)iq * nbytes, vec, bit0, c, nbits) : 0;
+        }
+#pragma unroll
+        for (uint32_t q = 0; q < kSortBatch; q++) {
+            if (d[q]) {
+                const uint32_t neg = d[q] < 0;
+                const uint32_t b = (uint32_t)(neg ? -d[q] : d[q]) - 1u;
+                const uint32_t pos = atomicAdd(&cursor[b], 1u);
+                sorted[pos] = make_uint2(kb + b, (i + q * blockDim.x) | (neg << 31));
+            }
         }
     }
 }

@@ -224,6 +224,8 @@ struct sgroup_t {
 };
 
 constexpr uint32_t kSortThreads = 512;
+// points per thread and step of the count and place loops
+constexpr uint32_t kSortBatch = 4;
 
 // nodes one block of the reduction tree merges into one
 constexpr uint32_t kTreeNodes = 64;
