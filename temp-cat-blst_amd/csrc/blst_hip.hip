@@ -595,10 +595,12 @@ void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
     std::vector<Layout> Ls(G);
     std::vector<DevLayout> Ds(G);
     std::vector<hostec::xyzz6*> outs(G);
+    // group 0 holds the top windows: it is queued first and finishes first,
+    // and the host folds it while the lower groups still run
     for (uint32_t g = 0; g < G; g++) {
         JobSpec s = js;
-        s.win0 = g * nwin / G;
-        s.nwin = (g + 1) * nwin / G - s.win0;
+        s.win0 = (G - 1 - g) * nwin / G;
+        s.nwin = (G - g) * nwin / G - s.win0;
         Ls[g] = make_layout(std::vector<JobSpec>(1, s));
         Workspace& w = *cx.pipe[g];
         ensure_workspace(w, Ls[g]);
@@ -611,13 +613,27 @@ void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
         HIPCK(hipStreamWaitEvent(w.stream, cx.ready, 0));
         enqueue_batch(w, Ls[g], Ds[g], true, outs[g], g == 0);
     }
-    std::vector<hostec::xyzz6> wins;
+    // Horner over the groups from the top: acc holds the groups above g at
+    // the base of group g - 1; its shift down to group g's base needs no data
+    // of group g and runs before waiting for it
+    hostec::xyzz6 acc;
+    hostec::set_inf(acc);
+    std::chrono::steady_clock::time_point t0;
     for (uint32_t g = 0; g < G; g++) {
+        const uint32_t base = (G - 1 - g) * nwin / G;
+        if (g > 0) hostec::scale_pow2(acc, acc, (int)(js.c * ((G - g) * nwin / G - base)));
         HIPCK(hipStreamSynchronize(cx.pipe[g]->stream));
-        wins.insert(wins.end(), outs[g], outs[g] + Ls[g].groups.size());
+        if (g + 1 == G) t0 = std::chrono::steady_clock::now();
+        hostec::xyzz6 part;
+        hostec::fold_windows(part, outs[g], (int)Ls[g].groups.size(), (int)js.c);
+        hostec::addp(acc, acc, part);
     }
     record_phases(*cx.pipe[0]);
-    finish_single(js, wins, ret);
+    hostec::jac6 j;
+    hostec::to_jac(j, acc);
+    std::memcpy(ret, &j, sizeof(j));
+    // the fold time left after the last group's results arrive
+    g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
 // Resolves the reference's descriptor convention (a NULL entry after the head

@@ -246,13 +246,55 @@ inline void compress(uint8_t out[48], const jac6& p) {
     out[0] |= (uint8_t)(0x80 | (big ? 0x20 : 0));
 }
 
+// Jacobian doubling for a = 0 (dbl-2009-l, 1M + 5S), for the long doubling
+// runs of the window fold; Z = 0 stays infinity
+inline void jdbl(jac6& r, const jac6& p) {
+    if (is_zero(p.z)) { r = p; return; }
+    fp6 A, B, C, D, E, F, t;
+    mul(A, p.x, p.x);
+    mul(B, p.y, p.y);
+    mul(C, B, B);
+    add(t, p.x, B);
+    mul(t, t, t);
+    sub(t, t, A);
+    sub(t, t, C);
+    add(D, t, t);            // D = 2((X + B)^2 - A - C) = 4XY^2
+    add(E, A, A);
+    add(E, E, A);            // E = 3X^2
+    mul(F, E, E);
+    jac6 o;
+    sub(o.x, F, D);
+    sub(o.x, o.x, D);        // X3 = E^2 - 2D
+    sub(t, D, o.x);
+    mul(t, E, t);
+    add(C, C, C);
+    add(C, C, C);
+    add(C, C, C);            // 8C
+    sub(o.y, t, C);          // Y3 = E(D - X3) - 8C
+    mul(o.z, p.y, p.z);
+    add(o.z, o.z, o.z);      // Z3 = 2YZ
+    r = o;
+}
+
+// 2^bits * p: the doublings in Jacobian coordinates, back to XYZZ
+// (ZZ = Z^2, ZZZ = Z^3)
+inline void scale_pow2(xyzz6& r, const xyzz6& p, int bits) {
+    if (bits == 0 || is_inf(p)) { r = p; return; }
+    jac6 j;
+    to_jac(j, p);
+    for (int k = 0; k < bits; k++) jdbl(j, j);
+    r.x = j.x;
+    r.y = j.y;
+    mul(r.zz, j.z, j.z);
+    mul(r.zzz, r.zz, j.z);
+}
+
 // sum_j 2^(c*j) w[j], Horner from the top window
 inline void fold_windows(xyzz6& out, const xyzz6* w, int nwin, int c) {
     xyzz6 acc;
     set_inf(acc);
     for (int j = nwin - 1; j >= 0; j--) {
-        if (j != nwin - 1)
-            for (int k = 0; k < c; k++) dbl(acc, acc);
+        if (j != nwin - 1) scale_pow2(acc, acc, c);
         addp(acc, acc, w[j]);
     }
     out = acc;
