@@ -298,7 +298,9 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="skip the serving workload")
     ap.add_argument("--no-entry-points", action="store_true",
                     help="skip the blst_fr MSM and batch-normalisation measurements")
-    ap.add_argument("--serve-rate", type=float, default=2800.0, help="offered request rate (req/s)")
+    ap.add_argument("--serve-rate", type=float, default=0.0,
+                    help="offered request rate (req/s); 0 = --serve-load times the measured capacity")
+    ap.add_argument("--serve-load", type=float, default=0.9, help="offered load as a fraction of capacity")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (HBM counter calibration)")
@@ -427,7 +429,7 @@ def main():
         print("uneven: %.2f ms batched, %.1fx" % (un["batched_ms"], un["speedup"]), file=sys.stderr, flush=True)
         out["batched"] = {"kzg": kz, "uneven": un}
     if not args.no_serving:
-        sv = workloads.serving(args.serve_rate, args.serve_seconds)
+        sv = workloads.serving(args.serve_rate or None, args.serve_seconds, args.serve_load)
         print("serving: %.3g pts/s, p99 %.1f ms, %.1fx" % (sv["served_points_per_s"], sv["served_latency_ms"]["p99"],
                                                            sv["speedup"]), file=sys.stderr, flush=True)
         out["serving"] = sv

@@ -220,18 +220,43 @@ def _serve(data, trace, burst):
     return secs, res, lat, stats
 
 
-def serving(rate=2800.0, duration=2.0, check_every=97):
+def serving(rate=None, duration=2.0, load=0.9, check_every=97, burst_rate=2800.0):
+    """Capacity first: after a warm-up burst, a trace of burst_rate req/s
+    submitted at once.  Then Poisson arrivals at `rate` req/s, or, without
+    one, at `load` times the measured capacity (GPUs and hosts differ by
+    several percent, and a fixed rate near one box's capacity overloads a
+    slower one); the plain path runs the same Poisson trace one call per
+    request."""
     data = _ServingData()
-    trace = serving_trace(rate, duration)
-    pts_total = sum(1 << lg for _, lg, _, _ in trace)
+    cap_trace = serving_trace(burst_rate, duration)
+    cap_pts = sum(1 << lg for _, lg, _, _ in cap_trace)
     try:
+        # a first server in the process pays one-time costs (device memory of
+        # the input pool, first launches) that a serving process pays once at
+        # start-up: a short burst absorbs them before anything is measured
+        _serve(data, cap_trace[:2000], burst=True)
+        burst_s, res_burst, _, _ = _serve(data, cap_trace, burst=True)
+        capacity = cap_pts / burst_s
+        if rate is None:
+            rate = load * capacity / (cap_pts / len(cap_trace))
+        else:
+            load = None
+        trace = serving_trace(rate, duration)
+        pts_total = sum(1 << lg for _, lg, _, _ in trace)
         served_s, res_served, lat_served, stats = _serve(data, trace, burst=False)
-        burst_s, res_burst, _, _ = _serve(data, trace, burst=True)
 
         # plain: one call per request, in arrival order, on the same trace
         d_srs = bh.DeviceBuffer(data.srs)
         d_sc = bh.DeviceBuffer(np.zeros((4096, 32), np.uint8))
         lat_plain, same = [], True
+
+        def plain(lg, fi, si):
+            sc = data.scal[lg][si]
+            if lg == 12:
+                d_sc.write(sc)
+                return bh.msm_device(d_srs.ptr, d_sc.ptr, 4096, 255)
+            return bh.p1s_mult_pippenger(data.fresh[lg][fi], sc, 255)
+
         try:
             bh.msm_device(d_srs.ptr, d_sc.ptr, 4096, 255)
             t0 = time.perf_counter()
@@ -239,16 +264,15 @@ def serving(rate=2800.0, duration=2.0, check_every=97):
                 now = time.perf_counter() - t0
                 if now < at:
                     time.sleep(at - now)
-                sc = data.scal[lg][si]
-                if lg == 12:
-                    d_sc.write(sc)
-                    r = bh.msm_device(d_srs.ptr, d_sc.ptr, 4096, 255)
-                else:
-                    r = bh.p1s_mult_pippenger(data.fresh[lg][fi], sc, 255)
+                r = plain(lg, fi, si)
                 lat_plain.append((time.perf_counter() - t0 - at) * 1e3)
                 if k % check_every == 0:
-                    same = same and bh.compress(r) == bh.compress(res_served[k]) == bh.compress(res_burst[k])
+                    same = same and bh.compress(r) == bh.compress(res_served[k])
             plain_s = time.perf_counter() - t0
+            # the burst's results, sampled, against the plain path
+            for k in range(0, len(cap_trace), check_every):
+                _, lg, fi, si = cap_trace[k]
+                same = same and bh.compress(plain(lg, fi, si)) == bh.compress(res_burst[k])
         finally:
             d_srs.close()
             d_sc.close()
@@ -258,16 +282,19 @@ def serving(rate=2800.0, duration=2.0, check_every=97):
         "workload": "Poisson arrivals at %.0f req/s for %.1f s (%d requests, %d points), mix 70/25/5 of "
                     "4096 (resident setup) / 2^16 / 2^20 (fresh bases)" % (rate, duration, len(trace), pts_total),
         "offered_rate": rate,
+        "offered_load_of_capacity": load,
         "offered_points_per_s": pts_total / duration,
         "requests": len(trace),
         "points": pts_total,
         "served_points_per_s": pts_total / served_s,
         "served_latency_ms": {"p50": _pct(lat_served, 50), "p99": _pct(lat_served, 99), "max": max(lat_served)},
-        "served_capacity_points_per_s": pts_total / burst_s,
+        "served_capacity_points_per_s": capacity,
+        "capacity_trace": "%d requests (%d points) of a %.0f req/s trace submitted at once" % (
+            len(cap_trace), cap_pts, burst_rate),
         "plain_points_per_s": pts_total / plain_s,
         "plain_latency_ms": {"p50": _pct(lat_plain, 50), "p99": _pct(lat_plain, 99), "max": max(lat_plain)},
         "speedup": plain_s / served_s,
-        "capacity_speedup": plain_s / burst_s,
+        "capacity_speedup": capacity / (pts_total / plain_s),
         "server": stats,
         "served_equals_plain_sampled": same,
     }
