@@ -246,18 +246,45 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
     if (ent >= 0x7fffffffULL || keys >= 0x7fffffffULL || pts >= 0x7fffffffULL || segs >= 0x7fffffffULL)
         throw HipFail{"batch too large for one launch"};
     if (tiles >= 0x7fffffffULL) throw HipFail{"batch too large for one launch"};
+    // the binned sort when every window has 2^12 .. 2^15 buckets (window key
+    // ranges then start at multiples of the bin size)
+    L.binned = !L.groups.empty();
+    for (const group_t& G : L.groups)
+        L.binned = L.binned && G.B >= (1u << 12) && G.B <= (kBinMaxBins << kBinBits);
+    if (L.binned) {
+        size_t cw = 0, nt = 0;
+        for (sgroup_t& S : L.sgroups) {
+            const job_t& J = L.jobs[S.job];
+            S.ch = kBinTile;
+            S.nch = std::max<uint32_t>(div_up(J.n, kBinTile), 1);
+            S.cb = (uint32_t)cw;
+            cw += (size_t)S.nch * (J.B >> kBinBits);
+            nt += S.nch;
+        }
+        if (cw >= 0x7fffffffULL || nt >= 0x7fffffffULL) throw HipFail{"batch too large for one launch"};
+        L.count_words = cw;
+        L.ntiles = (uint32_t)nt;
+        L.tiles.reserve(nt);
+        for (size_t g = 0; g < L.sgroups.size(); g++)
+            for (uint32_t k = 0; k < L.sgroups[g].nch; k++) L.tiles.push_back(make_uint2((uint32_t)g, k));
+        L.tile_begin = {0u, (uint32_t)nt};
+        L.tile_lds = {0};
+    }
     size_t cw = 0;
     for (sgroup_t& S : L.sgroups) {
+        if (L.binned) break;
         S.cb = (uint32_t)cw;
         cw += (size_t)S.nch * L.jobs[S.job].B;
     }
     if (cw >= 0xffffffffULL) throw HipFail{"batch too large for one launch"};
-    L.count_words = cw;
-    L.ntiles = (uint32_t)tiles;
+    if (!L.binned) {
+        L.count_words = cw;
+        L.ntiles = (uint32_t)tiles;
+    }
     // tile table by LDS size class: a launch reserves the LDS of its widest
     // window for every tile, so narrow windows of a mixed batch get their own
     // launches and share CUs
-    {
+    if (!L.binned) {
         const uint32_t limits[4] = {1u << 9, 1u << 12, 1u << 14, 0xffffffffu};
         L.tiles.reserve(tiles);
         L.tile_begin.push_back(0);
@@ -395,7 +422,11 @@ bool tiled_sort(const Layout& L) {
 void ensure_workspace(Workspace& cx, const Layout& L) {
     const uint32_t nkeys = L.nkeys, nent = L.total_entries;
     const uint32_t nslots = 2 * std::max<uint32_t>(div_up(nent, L.T), 1);
-    if (tiled_sort(L)) {
+    if (L.binned) {
+        cx.counts.ensure((L.count_words + 1) * 4 + 16);
+        cx.rank.ensure((L.count_words + 1) * 4 + 16);   // scanned bin counts
+        cx.ent.ensure((size_t)nent * 8 + 16);           // entries in bin order
+    } else if (tiled_sort(L)) {
         cx.counts.ensure(L.count_words * 4 + 16);
     } else {
         cx.ent.ensure((size_t)nent * 4 + 16);
@@ -404,7 +435,7 @@ void ensure_workspace(Workspace& cx, const Layout& L) {
     cx.sorted.ensure((size_t)nent * 8 + 16);
     cx.hist.ensure(((size_t)nkeys + 1) * 4);
     cx.off.ensure(((size_t)nkeys + 1) * 4);
-    cx.scan_tmp.ensure(scan_tmp_words(nkeys + 1) * 4);
+    cx.scan_tmp.ensure(std::max(scan_tmp_words(nkeys + 1), scan_tmp_words((uint32_t)L.count_words + 1)) * 4);
     const size_t pb = L.point_bytes();
     cx.buckets.ensure((size_t)nkeys * pb);
     cx.slot_key.ensure((size_t)nslots * 4);
@@ -418,7 +449,8 @@ void ensure_workspace(Workspace& cx, const Layout& L) {
 // Device bytes a batch of this layout needs beyond its inputs.
 size_t workspace_bytes(const Layout& L) {
     const size_t nslots = 2 * (size_t)std::max<uint32_t>(div_up(L.total_entries, L.T), 1);
-    const size_t sort_bytes = tiled_sort(L) ? L.count_words * 4 : (size_t)L.total_entries * 8;
+    const size_t sort_bytes = L.binned ? (L.count_words + 1) * 8 + (size_t)L.total_entries * 8
+                              : tiled_sort(L) ? L.count_words * 4 : (size_t)L.total_entries * 8;
     const size_t pb = L.point_bytes();
     return (size_t)L.total_entries * 8 + sort_bytes + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * pb +
            nslots * (6 + pb) + L.tree_nodes * 2 * pb + (L.groups.size() + L.jobs.size()) * pb + (1 << 20);
@@ -501,7 +533,19 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host
     uint32_t* off = cx.off.as<uint32_t>();
 
     if (timing) HIPCK(hipEventRecord(cx.ev[0], st));
-    if (tiled_sort(L)) {
+    if (L.binned) {
+        uint32_t* counts = cx.counts.as<uint32_t>();
+        uint32_t* pos = cx.rank.as<uint32_t>();
+        uint2* tmp = cx.ent.as<uint2>();
+        const uint32_t nc = (uint32_t)L.count_words;
+        launch_k_bin_count(dim3(L.ntiles), st, D.jobs, D.sgroups, D.tiles, counts, nc);
+        HIPCK(hipGetLastError());
+        if (timing) HIPCK(hipEventRecord(cx.ev[1], st));
+        scan_exclusive(cx, counts, pos, nc + 1, cx.scan_tmp.as<uint32_t>());
+        launch_k_bin_place(dim3(L.ntiles), st, D.jobs, D.sgroups, D.tiles, pos, tmp);
+        launch_k_bin_fine(dim3(nkeys >> kBinBits), st, D.sgroups, D.gkb, ngroups, pos, tmp, cx.sorted.as<uint2>(),
+                          hist, off, nkeys, nc);
+    } else if (tiled_sort(L)) {
         uint32_t* counts = cx.counts.as<uint32_t>();
         for (size_t c = 0; c < L.tile_lds.size(); c++)
             launch_k_count(dim3(L.tile_begin[c + 1] - L.tile_begin[c]), dim3(kSortThreads), L.tile_lds[c], st, D.jobs,

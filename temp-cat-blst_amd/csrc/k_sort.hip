@@ -153,6 +153,163 @@ k_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const u
     }
 }
 
+// ---------------------------------------------------------------- binned sort
+//
+// For windows of 2^12 .. 2^15 buckets a tile of the one-pass sort puts about
+// two entries into each bucket's run, so k_place's 8-byte stores land one or
+// two per cache line.  The binned sort groups 2^kBinBits buckets into a bin:
+//   k_bin_count  per tile (window x kBinTile points): bin histogram in LDS,
+//                stored bin-major, counts[cb + bin * nch + tile]
+//   scan         one exclusive scan of all counts: the start of every
+//                (bin, tile) run in the sorted order, bins in key order
+//   k_bin_place  per tile: entries staged in LDS in bin order, then each
+//                bin's run written contiguously into tmp
+//   k_bin_fine   per bin: its buckets counted in LDS (the per-bucket count
+//                and offset tables of the later stages) and its entries
+//                placed in bucket order inside the bin's region
+// Keys of a binned layout start every window at a multiple of 2^kBinBits,
+// so the global bin of a key is key >> kBinBits.
+
+__device__ __forceinline__ uint32_t bin_tile_digits(const job_t& J, const sgroup_t& S, uint32_t k,
+                                                    int32_t d[kBinTile / kSortThreads]) {
+    const uint32_t i0 = k * kBinTile + threadIdx.x, i1 = J.n;
+    const uint32_t bit0 = J.bit_base + S.w * J.c;
+#pragma unroll
+    for (uint32_t q = 0; q < kBinTile / kSortThreads; q++) {
+        const uint32_t i = i0 + q * kSortThreads;
+        d[q] = (i < i1 && bit0 <= J.nbits)
+                   ? booth_digit_at(J.scalars + (size_t)i * J.nbytes, J.vec, bit0, J.c, J.nbits)
+                   : 0;
+    }
+    return i0;
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+k_bin_count(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles,
+            uint32_t* __restrict__ counts, uint32_t ncounts) {
+    __shared__ uint32_t bins[kBinMaxBins];
+    const uint2 tk = tiles[blockIdx.x];
+    const sgroup_t S = sg[tk.x];
+    const job_t& J = jobs[S.job];
+    const uint32_t nbins = J.B >> kBinBits;
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) bins[b] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) counts[ncounts] = 0;   // closes the scan
+    __syncthreads();
+    int32_t d[kBinTile / kSortThreads];
+    bin_tile_digits(J, S, tk.y, d);
+#pragma unroll
+    for (uint32_t q = 0; q < kBinTile / kSortThreads; q++)
+        if (d[q]) atomicAdd(&bins[((uint32_t)(d[q] < 0 ? -d[q] : d[q]) - 1u) >> kBinBits], 1u);
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) counts[S.cb + (size_t)b * S.nch + tk.y] = bins[b];
+}
+
+// exclusive scan of v over the block (blockDim.x == kSortThreads), LDS scratch
+// of kSortThreads / 64 words; returns the exclusive prefix, *total the sum
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t* wsum, uint32_t* total) {
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (uint32_t o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[wave] = x;
+    __syncthreads();
+    uint32_t before = 0, all = 0;
+    for (uint32_t w = 0; w < kSortThreads / 64; w++) {
+        const uint32_t s = wsum[w];
+        before += w < wave ? s : 0u;
+        all += s;
+    }
+    *total = all;
+    return before + x - v;
+}
+
+__global__ void __launch_bounds__(kSortThreads)
+k_bin_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles,
+            const uint32_t* __restrict__ pos, uint2* __restrict__ tmp) {
+    __shared__ uint2 stage[kBinTile];
+    __shared__ uint32_t start[kBinMaxBins], cursor[kBinMaxBins], wsum[kSortThreads / 64];
+    const uint2 tk = tiles[blockIdx.x];
+    const sgroup_t S = sg[tk.x];
+    const job_t& J = jobs[S.job];
+    const uint32_t nbins = J.B >> kBinBits;
+    const uint32_t kb = J.kbase + S.w * J.B;
+    for (uint32_t b = threadIdx.x; b < nbins; b += blockDim.x) cursor[b] = 0;
+    __syncthreads();
+    int32_t d[kBinTile / kSortThreads];
+    const uint32_t i0 = bin_tile_digits(J, S, tk.y, d);
+#pragma unroll
+    for (uint32_t q = 0; q < kBinTile / kSortThreads; q++)
+        if (d[q]) atomicAdd(&cursor[((uint32_t)(d[q] < 0 ? -d[q] : d[q]) - 1u) >> kBinBits], 1u);
+    __syncthreads();
+    // nbins <= kSortThreads: one bin per thread
+    uint32_t m;
+    const uint32_t c = threadIdx.x < nbins ? cursor[threadIdx.x] : 0u;
+    const uint32_t s0 = block_scan(c, wsum, &m);
+    if (threadIdx.x < nbins) {
+        start[threadIdx.x] = s0;
+        cursor[threadIdx.x] = s0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < kBinTile / kSortThreads; q++) {
+        if (d[q]) {
+            const uint32_t neg = d[q] < 0;
+            const uint32_t b = (uint32_t)(neg ? -d[q] : d[q]) - 1u;
+            const uint32_t at = atomicAdd(&cursor[b >> kBinBits], 1u);
+            stage[at] = make_uint2(kb + b, (i0 + q * kSortThreads) | (neg << 31));
+        }
+    }
+    __syncthreads();
+    // bin j's entries are stage[start[j] .. start[j] + count): consecutive
+    // slots go to consecutive positions of the bin's run
+    for (uint32_t e = threadIdx.x; e < m; e += blockDim.x) {
+        const uint2 v = stage[e];
+        const uint32_t j = (v.x - kb) >> kBinBits;
+        tmp[pos[S.cb + (size_t)j * S.nch + tk.y] + (e - start[j])] = v;
+    }
+}
+
+// one workgroup per bin; gkb[g] is the first key of group g
+__global__ void __launch_bounds__(256)
+k_bin_fine(const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, uint32_t ngroups,
+           const uint32_t* __restrict__ pos, const uint2* __restrict__ tmp, uint2* __restrict__ sorted,
+           uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, uint32_t nkeys, uint32_t ncounts) {
+    constexpr uint32_t F = 1u << kBinBits;
+    __shared__ uint32_t fcount[F], cursor[F];
+    const uint32_t key0 = blockIdx.x << kBinBits;
+    const uint32_t g = find_owner(gkb, ngroups, key0);
+    const sgroup_t S = sg[g];
+    const uint32_t j = (key0 - gkb[g]) >> kBinBits;
+    const uint32_t lo = pos[S.cb + (size_t)j * S.nch], hi = pos[S.cb + (size_t)(j + 1) * S.nch];
+    if (threadIdx.x < F) fcount[threadIdx.x] = 0;
+    if (blockIdx.x == 0 && threadIdx.x == 0) off[nkeys] = pos[ncounts];   // total entries
+    __syncthreads();
+    for (uint32_t e = lo + threadIdx.x; e < hi; e += blockDim.x) atomicAdd(&fcount[tmp[e].x & (F - 1)], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        // F == 64: one wave scans the bin's bucket counts
+        const uint32_t v = fcount[threadIdx.x];
+        uint32_t x = v;
+#pragma unroll
+        for (uint32_t o = 1; o < 64; o <<= 1) {
+            const uint32_t y = __shfl_up(x, o);
+            if (threadIdx.x >= o) x += y;
+        }
+        cursor[threadIdx.x] = x - v;
+        cnt[key0 + threadIdx.x] = v;
+        off[key0 + threadIdx.x] = lo + x - v;
+    }
+    __syncthreads();
+    for (uint32_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+        const uint2 v = tmp[e];
+        sorted[lo + atomicAdd(&cursor[v.x & (F - 1)], 1u)] = v;
+    }
+}
+static_assert((1u << kBinBits) == 64, "k_bin_fine scans one bin with one wave");
+
 // Montgomery-form Fr scalars (blst_fr: x = k * 2^256 mod r, four 64-bit
 // little-endian limbs) to canonical 32-byte little-endian scalars k =
 // x * 2^-256 mod r, in place: eight rounds of word-wise Montgomery reduction
@@ -296,6 +453,18 @@ size_t sort_lds_limit() {
         return 0;
     }
     return (size_t)v;
+}
+
+void launch_k_bin_count(dim3 grid, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, uint32_t* __restrict__ counts, uint32_t ncounts) {
+    hipLaunchKernelGGL(k_bin_count, grid, dim3(kSortThreads), 0, stream, jobs, sg, tiles, counts, ncounts);
+}
+
+void launch_k_bin_place(dim3 grid, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, const uint32_t* __restrict__ pos, uint2* __restrict__ tmp) {
+    hipLaunchKernelGGL(k_bin_place, grid, dim3(kSortThreads), 0, stream, jobs, sg, tiles, pos, tmp);
+}
+
+void launch_k_bin_fine(dim3 grid, hipStream_t stream, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, uint32_t ngroups, const uint32_t* __restrict__ pos, const uint2* __restrict__ tmp, uint2* __restrict__ sorted, uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, uint32_t nkeys, uint32_t ncounts) {
+    hipLaunchKernelGGL(k_bin_fine, grid, dim3(256), 0, stream, sg, gkb, ngroups, pos, tmp, sorted, cnt, off, nkeys, ncounts);
 }
 
 void launch_k_fr_to_scalar(hipStream_t stream, uint32_t* __restrict__ io, uint32_t n) {

@@ -16,6 +16,9 @@ void launch_k_count(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const
 void launch_k_prefix(dim3 grid, dim3 block, hipStream_t stream, const group_t* __restrict__ groups, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, uint32_t ngroups, uint32_t nkeys, uint32_t* __restrict__ counts, uint32_t* __restrict__ hist);
 void launch_k_place(dim3 grid, dim3 block, size_t lds, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, const uint32_t* __restrict__ counts, const uint32_t* __restrict__ off, uint2* __restrict__ sorted);
 size_t sort_lds_limit();
+void launch_k_bin_count(dim3 grid, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, uint32_t* __restrict__ counts, uint32_t ncounts);
+void launch_k_bin_place(dim3 grid, hipStream_t stream, const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const uint2* __restrict__ tiles, const uint32_t* __restrict__ pos, uint2* __restrict__ tmp);
+void launch_k_bin_fine(dim3 grid, hipStream_t stream, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, uint32_t ngroups, const uint32_t* __restrict__ pos, const uint2* __restrict__ tmp, uint2* __restrict__ sorted, uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, uint32_t nkeys, uint32_t ncounts);
 void launch_k_fr_to_scalar(hipStream_t stream, uint32_t* __restrict__ io, uint32_t n);
 template <class F>
 void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks);
