@@ -246,11 +246,21 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
     if (ent >= 0x7fffffffULL || keys >= 0x7fffffffULL || pts >= 0x7fffffffULL || segs >= 0x7fffffffULL)
         throw HipFail{"batch too large for one launch"};
     if (tiles >= 0x7fffffffULL) throw HipFail{"batch too large for one launch"};
-    // the binned sort when every window has 2^12 .. 2^15 buckets (window key
-    // ranges then start at multiples of the bin size)
-    L.binned = !L.groups.empty();
-    for (const group_t& G : L.groups)
-        L.binned = L.binned && G.B >= (1u << 12) && G.B <= (kBinMaxBins << kBinBits);
+    // the binned sort when every window has 2^6 .. 2^15 buckets (window key
+    // ranges then start at multiples of the bin size) and at least half of
+    // the entries fall into windows of 2^12 buckets or more, whose one-pass
+    // runs would hold one or two entries per bucket and tile
+    {
+        bool fits = !L.groups.empty();
+        uint64_t wide = 0, all = 0;
+        for (const sgroup_t& S : L.sgroups) {
+            const job_t& J = L.jobs[S.job];
+            fits = fits && J.B >= (1u << kBinBits) && J.B <= (kBinMaxBins << kBinBits);
+            all += J.n;
+            wide += J.B >= (1u << 12) ? J.n : 0;
+        }
+        L.binned = fits && 2 * wide >= all;
+    }
     if (L.binned) {
         size_t cw = 0, nt = 0;
         for (sgroup_t& S : L.sgroups) {
