@@ -303,6 +303,37 @@ def test_batch_uneven_distinct_and_shared_bases(hip, orc, base_points):
         assert hip.compress(got[k]) == want[k], k
 
 
+def test_batch_binned_sort_mixed_windows(hip, orc, base_points):
+    """A batch whose windows all have 2^6 .. 2^15 buckets with most entries in
+    windows of 2^12 or more takes the binned two-pass sort: jobs of window
+    widths 7 .. 14 bits, a short scalar width, repeated and edge scalars;
+    every result equals its own oracle MSM."""
+    rnd = random.Random(0xB1)
+    r = pyref.R_ORDER
+    specs = [(1 << 15, 255, 13), (3000, 255, 9), (5000, 64, 7), (1 << 14, 255, 14), (100, 255, 7),
+             (4096, 255, 13)]
+    keep, jobs, want = [], [], []
+    for k, (n, nb, w) in enumerate(specs):
+        nbytes = (nb + 7) // 8
+        vals = [rnd.getrandbits(nb) for _ in range(n)]
+        for i in range(0, n, 97):
+            vals[i] = rnd.choice([0, 1, r - 1 if nb == 255 else 1, vals[0]])
+        sc = _scal(vals, nbytes)
+        pts = np.ascontiguousarray(np.tile(base_points, (n // 4096 + 1, 1))[:n]) if k % 2 else \
+            orc.gen_points(n, 900 + k, threads=8)
+        dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+        keep += [dp, ds]
+        jobs.append((dp.ptr, ds.ptr, n, nb, w))
+        want.append(orc.compress(orc.msm_pippenger(pts, sc, n, nb, threads=8)))
+    try:
+        got = hip.msm_batch_device(jobs)
+    finally:
+        for b in keep:
+            b.close()
+    for k in range(len(jobs)):
+        assert hip.compress(got[k]) == want[k], k
+
+
 def test_graph_plan_replays(hip, orc, base_points):
     """A captured batch replayed several times with new scalars written into
     the same device buffers between replays, with eager work in between;
