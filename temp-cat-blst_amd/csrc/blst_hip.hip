@@ -351,6 +351,25 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
     return L;
 }
 
+// bytes (a multiple of 4) between device memory and a page-locked host
+// buffer on cx.stream: a copy kernel through the host buffer's device mapping
+// when the workspace asks for kernel transfers, else an async copy
+void transfer_to(Workspace& cx, void* dst, const void* src, size_t bytes, hipMemcpyKind kind) {
+    if (cx.kernel_transfers) {
+        void* mapped = nullptr;
+        const void* host = kind == hipMemcpyHostToDevice ? src : dst;
+        if (hipHostGetDevicePointer(&mapped, const_cast<void*>(host), 0) == hipSuccess && mapped) {
+            const uint32_t* s = static_cast<const uint32_t*>(kind == hipMemcpyHostToDevice ? mapped : src);
+            uint32_t* d = static_cast<uint32_t*>(kind == hipMemcpyHostToDevice ? dst : mapped);
+            launch_k_copy_words(cx.stream, s, d, bytes / 4);
+            HIPCK(hipGetLastError());
+            return;
+        }
+        (void)hipGetLastError();
+    }
+    HIPCK(hipMemcpyAsync(dst, src, bytes, kind, cx.stream));
+}
+
 DevLayout upload_layout(Workspace& cx, const Layout& L) {
     const size_t nj = L.jobs.size(), ng = L.groups.size();
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
@@ -373,7 +392,7 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     std::memcpy(h + o_gk, L.gkb.data(), ng * 4);
     std::memcpy(h + o_trb, L.tree_blk.data(), L.tree_blk.size() * 4);
     std::memcpy(h + o_trt, L.tree_tn.data(), L.tree_tn.size() * sizeof(uint4));
-    HIPCK(hipMemcpyAsync(cx.tables.p, h, total, hipMemcpyHostToDevice, cx.stream));
+    transfer_to(cx, cx.tables.p, h, total, hipMemcpyHostToDevice);
     uint8_t* d = cx.tables.as<uint8_t>();
     DevLayout D;
     D.jobs = reinterpret_cast<const job_t*>(d + o_jobs);
@@ -516,11 +535,11 @@ void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool hos
                              p + 1 == npass ? 1u : 0u);
     HIPCK(hipGetLastError());
     if (host_fold) {
-        HIPCK(hipMemcpyAsync(host_out, cx.W.p, (size_t)ngroups * sizeof(P), hipMemcpyDeviceToHost, st));
+        transfer_to(cx, host_out, cx.W.p, (size_t)ngroups * sizeof(P), hipMemcpyDeviceToHost);
     } else {
         launch_k_fold(st, D.jobs, njobs, cx.W.as<const P>(), cx.res.as<P>());
         HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(host_out, cx.res.p, (size_t)njobs * sizeof(P), hipMemcpyDeviceToHost, st));
+        transfer_to(cx, host_out, cx.res.p, (size_t)njobs * sizeof(P), hipMemcpyDeviceToHost);
     }
     if (timing) HIPCK(hipEventRecord(cx.ev[5], st));
 }

@@ -3,6 +3,8 @@
 #include "msm_common.hip.h"
 #include "launch.h"
 
+#include <algorithm>
+
 namespace bls {
 
 __global__ void __launch_bounds__(256)
@@ -357,6 +359,15 @@ k_fill(uint32_t* __restrict__ dst, size_t words, uint32_t value) {
         dst[i] = value;
 }
 
+// dst[0..words) = src[0..words); either side may be page-locked host memory
+// mapped into the device's address space (a transfer that takes no slot in
+// the copy engines' queues, so it cannot hold up copies of other streams)
+__global__ void __launch_bounds__(256)
+k_copy_words(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = src[i];
+}
+
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
              uint32_t* __restrict__ tile_sums, uint32_t N) {
@@ -415,6 +426,11 @@ void launch_k_digits_debug(dim3 grid, dim3 block, hipStream_t stream, const uint
 
 void launch_k_fill(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restrict__ dst, size_t words, uint32_t value) {
     hipLaunchKernelGGL(k_fill, grid, block, 0, stream, dst, words, value);
+}
+
+void launch_k_copy_words(hipStream_t stream, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words) {
+    const uint32_t blocks = (uint32_t)std::min<size_t>((words + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_copy_words, dim3(blocks ? blocks : 1), dim3(256), 0, stream, src, dst, words);
 }
 
 void launch_k_scan_tiles(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t* __restrict__ tile_sums, uint32_t N) {

@@ -60,6 +60,11 @@ struct Workspace {
     void* pinned = nullptr;               // page-locked result staging
     size_t pinned_cap = 0;
     bool owns_stream = false;
+    // layout tables in and window sums out by a copy kernel through the
+    // device mapping of the page-locked buffers instead of copy-engine
+    // transfers (the serving loop's batches: an engine transfer queued behind
+    // a running batch holds up the request copies of the copy stream)
+    bool kernel_transfers = false;
     // own stream, or a stream shared with other workspaces (kernels of
     // different workspaces then run one after another)
     void init(hipStream_t shared = nullptr) {

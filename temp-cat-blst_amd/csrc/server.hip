@@ -196,6 +196,7 @@ public:
         for (auto& s : slots_) {
             s = std::make_unique<Slot>();
             s->ws.init(compute_stream_);
+            s->ws.kernel_transfers = true;
             HIPCK(hipEventCreateWithFlags(&s->done, hipEventDisableTiming));
         }
         worker_ = std::thread([this] { loop(); });
