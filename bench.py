@@ -405,16 +405,6 @@ def main():
     out["compressed"] = gpu_bytes.hex()
     dp.close()
     ds.close()
-    import workloads
-
-    # The serving workload runs before the other legs, in the process state a
-    # dedicated serving process has: run after the batched legs in the same
-    # process its sustainable Poisson rate measured 10-30% lower.
-    if not args.no_serving:
-        sv = workloads.serving(args.serve_rate or None, args.serve_seconds, args.serve_load)
-        print("serving: %.3g pts/s, p99 %.1f ms, %.1fx" % (sv["served_points_per_s"], sv["served_latency_ms"]["p99"],
-                                                           sv["speedup"]), file=sys.stderr, flush=True)
-        out["serving"] = sv
     jac = wbits_case = g2_case = None
     if not args.no_entry_points:
         ep, jac, wbits_case, g2_case = entry_points(bh, points, scalars, r, n)
@@ -422,6 +412,8 @@ def main():
         print("fr: %.2f ms host, to_affine: %.3f ms device, wbits: %.3f ms device, g2 2^16: %.3f ms device" % (
             ep["p1s_mult_pippenger_fr"]["ms"], ep["p1s_to_affine"]["device_ms"], ep["p1s_mult_wbits"]["device_ms"],
             ep["p2s_mult_pippenger"]["device_ms"]), file=sys.stderr, flush=True)
+
+    import workloads
 
     samples = {}
     if not args.no_batched:
@@ -436,6 +428,11 @@ def main():
         samples["uneven"] = [(sm["points"], sm["scalars"], sm["result"])]
         print("uneven: %.2f ms batched, %.1fx" % (un["batched_ms"], un["speedup"]), file=sys.stderr, flush=True)
         out["batched"] = {"kzg": kz, "uneven": un}
+    if not args.no_serving:
+        sv = workloads.serving(args.serve_rate or None, args.serve_seconds, args.serve_load)
+        print("serving: %.3g pts/s, p99 %.1f ms, %.1fx" % (sv["served_points_per_s"], sv["served_latency_ms"]["p99"],
+                                                           sv["speedup"]), file=sys.stderr, flush=True)
+        out["serving"] = sv
     if not args.no_cpu:
         cb = cpu_baseline(points, scalars, n, gpu_bytes, samples, jac, wbits_case, g2_case)
         out["cpu_baseline"] = cb
