@@ -227,9 +227,9 @@ constexpr uint32_t kSortThreads = 512;
 // points per thread and step of the count and place loops
 constexpr uint32_t kSortBatch = 4;
 
-// Two-pass ("binned") sort for windows of 2^12 .. 2^15 buckets: bins of
-// 2^kBinBits consecutive buckets, tiles of kBinTile points (kBinTile /
-// kSortThreads digits per thread)
+// Two-pass ("binned") sort for batches whose windows have 2^6 .. 2^15
+// buckets, mostly 2^12 or more: bins of 2^kBinBits consecutive buckets, tiles
+// of kBinTile points (kBinTile / kSortThreads digits per thread)
 constexpr uint32_t kBinBits = 6;
 constexpr uint32_t kBinTile = 8192;
 constexpr uint32_t kBinMaxBins = kSortThreads;

@@ -146,8 +146,9 @@ struct Layout {
     uint32_t total_points = 0, total_entries = 0, nkeys = 0, total_segments = 0, max_n = 0;
     uint32_t ntiles = 0, max_B = 0;
     size_t count_words = 0;
-    // binned sort (every window of 2^12 .. 2^15 buckets): sort groups hold
-    // tiles of kBinTile points and bin-major counts, one tile class
+    // binned sort (every window of 2^6 .. 2^15 buckets, at least half of the
+    // entries in windows of 2^12 or more): sort groups hold tiles of
+    // kBinTile points and bin-major counts, one tile class
     bool binned = false;
     // reduction tree: per pass, blocks and node offsets (in nodes of two
     // xyzz) of its input and output arrays; per pass and group the first
