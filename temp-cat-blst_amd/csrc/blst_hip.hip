@@ -127,14 +127,15 @@ struct Stager {
         const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
         pool = std::make_unique<CopyPool>(std::max(1u, std::min(8u, hw / 2)));
     }
+    // page-locked host or device memory (the copy engines reach it directly)
+    static bool registered(const void* p) {
+        hipPointerAttribute_t a;
+        const bool r = hipPointerGetAttributes(&a, p) == hipSuccess && a.type != hipMemoryTypeUnregistered;
+        (void)hipGetLastError();
+        return r;
+    }
     void upload(void* dst, const void* src, size_t bytes, hipStream_t stream) {
-        bool direct = bytes < kMin;
-        if (!direct) {
-            hipPointerAttribute_t a;
-            if (hipPointerGetAttributes(&a, src) == hipSuccess && a.type != hipMemoryTypeUnregistered)
-                direct = true;   // page-locked host or device memory
-            (void)hipGetLastError();
-        }
+        const bool direct = bytes < kMin || registered(src);
         if (direct) {
             HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
             return;
@@ -149,6 +150,33 @@ struct Stager {
             pool->copy(ring[slot], s + off, len);
             HIPCK(hipMemcpyAsync(d + off, ring[slot], len, hipMemcpyHostToDevice, stream));
             HIPCK(hipEventRecord(ev[slot], stream));
+        }
+    }
+    // device-to-host after the work queued on stream; returns when dst holds
+    // the bytes: a pageable dst gets the chunks from the ring, each copied out
+    // by the pool while the next chunks' DMAs run
+    void download(void* dst, const void* src, size_t bytes, hipStream_t stream) {
+        if (bytes < kMin || registered(dst)) {
+            HIPCK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+            HIPCK(hipStreamSynchronize(stream));
+            return;
+        }
+        init();
+        const uint8_t* s = static_cast<const uint8_t*>(src);
+        uint8_t* d = static_cast<uint8_t*>(dst);
+        const size_t nch = (bytes + kChunk - 1) / kChunk;
+        auto len = [&](size_t k) { return std::min(kChunk, bytes - k * kChunk); };
+        auto issue = [&](size_t k) {
+            const int slot = (int)(k % kRing);
+            HIPCK(hipMemcpyAsync(ring[slot], s + k * kChunk, len(k), hipMemcpyDeviceToHost, stream));
+            HIPCK(hipEventRecord(ev[slot], stream));
+        };
+        for (size_t k = 0; k < std::min<size_t>(nch, kRing); k++) issue(k);
+        for (size_t k = 0; k < nch; k++) {
+            const int slot = (int)(k % kRing);
+            HIPCK(hipEventSynchronize(ev[slot]));
+            pool->copy(d + k * kChunk, ring[slot], len(k));
+            if (k + kRing < nch) issue(k + kRing);
         }
     }
 };
@@ -1740,13 +1768,10 @@ void blst_p1s_to_affine(blst_p1_affine dst[], const blst_p1* const points[], siz
         cx.aff_out.ensure(chunk * sizeof(blst_p1_affine));
         for (size_t b = 0; b < npoints; b += chunk) {
             const size_t m = std::min(chunk, npoints - b);
-            HIPCK(hipMemcpyAsync(cx.jac_in.p, hp + b * sizeof(blst_p1), m * sizeof(blst_p1), hipMemcpyHostToDevice,
-                                 cx.stream));
+            cx.stager.upload(cx.jac_in.p, hp + b * sizeof(blst_p1), m * sizeof(blst_p1), cx.stream);
             enqueue_to_affine(cx, cx.jac_in.p, cx.aff_out.p, m);
-            HIPCK(hipMemcpyAsync(dst + b, cx.aff_out.p, m * sizeof(blst_p1_affine), hipMemcpyDeviceToHost,
-                                 cx.stream));
+            cx.stager.download(dst + b, cx.aff_out.p, m * sizeof(blst_p1_affine), cx.stream);
         }
-        HIPCK(hipStreamSynchronize(cx.stream));
     } catch (const HipFail& f) {
         g_err = f.msg;
         std::memset(dst, 0, npoints * sizeof(blst_p1_affine));
