@@ -1837,16 +1837,14 @@ void blst_p1s_mult_wbits_precompute(blst_p1_affine table[], size_t wbits, const 
         cx.wb_pts.ensure(npoints * sizeof(blst_p1_affine));
         // every point is on the device before any row is written back: a
         // packed input may occupy the table's tail
-        HIPCK(hipMemcpyAsync(cx.wb_pts.p, hp, npoints * sizeof(blst_p1_affine), hipMemcpyHostToDevice, cx.stream));
+        cx.stager.upload(cx.wb_pts.p, hp, npoints * sizeof(blst_p1_affine), cx.stream);
         const size_t R = std::max<size_t>(1, kWbitsRowEntries / m);
         cx.wb_table.ensure(std::min(npoints, R) * m * sizeof(blst_p1_affine));
         for (size_t row0 = 0; row0 < npoints; row0 += R) {
             const size_t nr = std::min(R, npoints - row0);
             const uint8_t* rows = static_cast<const uint8_t*>(cx.wb_pts.p) + row0 * sizeof(blst_p1_affine);
             enqueue_wbits_table(cx, rows, nr, wbits, cx.wb_table.p);
-            HIPCK(hipMemcpyAsync(table + row0 * m, cx.wb_table.p, nr * m * sizeof(blst_p1_affine),
-                                 hipMemcpyDeviceToHost, cx.stream));
-            HIPCK(hipStreamSynchronize(cx.stream));
+            cx.stager.download(table + row0 * m, cx.wb_table.p, nr * m * sizeof(blst_p1_affine), cx.stream);
         }
     } catch (const HipFail& f) {
         g_err = f.msg;
