@@ -14,6 +14,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+#include "fp_inv.hip.h"
 
 namespace bls {
 
@@ -452,19 +453,18 @@ __device__ __forceinline__ void fp_mul_sum(fp& r, const fp& a, const fp& b, cons
     for (int i = 0; i < 12; i++) r.l[i] = t[i];
 }
 
-// a^(p-2) = a^-1 (Fermat), left-to-right over the fixed exponent bits; the
-// branch is uniform across the wave.  0 maps to 0.
+// a^-1 in Montgomery form, 0 for 0: the divstep inverse of the canonical
+// word image c = aR gives c^-1 = a^-1 R^-1, and one product with R^3 mod p
+// (c^-1 R^3 / R) brings it to a^-1 R.
 __device__ __forceinline__ void fp_inv(fp& r, const fp& a) {
-    constexpr uint32_t E[12] = {BLS_P0 - 2, BLS_P1, BLS_P2, BLS_P3, BLS_P4, BLS_P5,
-                                BLS_P6, BLS_P7, BLS_P8, BLS_P9, BLS_P10, BLS_P11};
-    fp acc;
-    fp_one(acc);
-    const fp base = a;
-    for (int bit = 380; bit >= 0; bit--) {
-        fp_sqr(acc, acc);
-        if ((E[bit >> 5] >> (bit & 31)) & 1u) fp_mul(acc, acc, base);
-    }
-    r = acc;
+    constexpr uint32_t R3[12] = {0xd94ca1e0, 0xed48ac6b, 0x03a7adf8, 0x315f831e, 0x615e29dd, 0x9a53352a,
+                                 0x921e1761, 0x34c04e5e, 0x65724728, 0x2512d435, 0x91755d4d, 0x0aa63460};
+    fp c, t, r3;
+    fp_canon(c, a);
+    divstep::inverse(t.l, c.l);   // below 2p
+#pragma unroll
+    for (int i = 0; i < 12; i++) r3.l[i] = R3[i];
+    fp_mul(r, t, r3);
 }
 
 }  // namespace bls

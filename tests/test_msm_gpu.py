@@ -517,6 +517,24 @@ def test_to_affine_all_infinity_and_mixed_z(hip, orc):
     assert not np.delete(got, [5, 69999], axis=0).any()
 
 
+def test_to_affine_inversion_edge_values(hip, orc):
+    """One point per lane, so each lane inverts its point's Z word image
+    itself: Z images 1, 2, p - 1, p - 2, (p +- 1)/2, 2^k, p - 2^k and 2^k - 1
+    (the values a binary gcd inversion handles last) give back the affine
+    points."""
+    P = pyref.P
+    zs = [1, 2, 3, P - 1, P - 2, (P - 1) // 2, (P + 1) // 2]
+    zs += [1 << k for k in range(0, 381, 7)] + [P - (1 << k) for k in range(0, 381, 7)]
+    zs += [(1 << k) - 1 for k in range(2, 381, 11)]
+    n = len(zs)
+    aff = orc.gen_points(n, 0x2E, threads=8)
+    lam = np.array([[(z >> (64 * j)) & (2**64 - 1) for j in range(6)] for z in zs], dtype=np.uint64)
+    jac = orc.p1s_from_affine_scaled(aff, lam)
+    got = hip.p1s_to_affine(jac)
+    assert np.array_equal(got, orc.p1s_to_affine(jac))
+    assert np.array_equal(got, aff)
+
+
 def test_to_affine_device_2pow20_round_trip(hip, orc):
     """At 2^20 points: affine -> scaled Jacobian -> the device kernel gives
     back the same affine points (a size-independent round trip)."""
