@@ -188,6 +188,7 @@ struct Ctx {
     Workspace ws;
     std::vector<std::unique_ptr<Workspace>> pipe;   // window-group pipelines
     hipEvent_t ready = nullptr;                     // inputs written on stream
+    std::vector<hipEvent_t> range_ready;            // per point range of a split host MSM
     DevBuf pts, scl, dbg;
     DevBuf gtable, gen_tmp, gen_out;     // comb table of G, fixed-base scratch
     DevBuf jac_in, aff_out;              // batch normalisation staging
@@ -858,6 +859,94 @@ void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
     g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// An MSM over contiguous host buffers (points, and scalars of nbits bits or,
+// with fr, Montgomery-form blst_fr values converted on the device).  Its
+// inputs cross PCIe (134 MB at 2^20 points, about as long as the MSM itself),
+// so from kHostSplitMin points the points are split into kHostRanges ranges,
+// each an MSM of its own on its own pipeline stream: range k + 1 is staged and
+// copied while range k runs, and the host folds each range's windows as it
+// finishes and adds the results.  A range's own reduction costs less than the
+// copy time it hides.
+constexpr size_t kHostSplitMin = (size_t)1 << 19;
+constexpr uint32_t kHostRanges = 2;
+
+void host_buffers_msm(Ctx& cx, const uint8_t* hp, const uint8_t* hs, size_t n, size_t nbits, size_t window,
+                      bool fr, bool fold, size_t bit0, blst_p1* ret) {
+    const size_t nbytes = fr ? 32 : (nbits + 7) / 8;
+    cx.pts.ensure(n * sizeof(blst_p1_affine));
+    cx.scl.ensure(n * nbytes + 16);
+    uint8_t* dp = static_cast<uint8_t*>(cx.pts.p);
+    uint8_t* ds = static_cast<uint8_t*>(cx.scl.p);
+    const uint32_t K = (fold && n >= kHostSplitMin) ? kHostRanges : 1;
+    if (K == 1) {
+        cx.stager.upload(ds, hs, n * nbytes, cx.stream);
+        cx.stager.upload(dp, hp, n * sizeof(blst_p1_affine), cx.stream);
+        if (fr) launch_k_fr_to_scalar(cx.stream, reinterpret_cast<uint32_t*>(ds), (uint32_t)n);
+        HIPCK(hipGetLastError());
+        JobSpec js = make_spec(dp, ds, n, nbits, window, false);
+        js.bit_base = (uint32_t)bit0;
+        js.tile = !fold;
+        device_msm(cx, js, ret);
+        return;
+    }
+    while (cx.pipe.size() < K) {
+        cx.pipe.emplace_back(new Workspace());
+        cx.pipe.back()->init();
+    }
+    while (cx.range_ready.size() < K) {
+        hipEvent_t e;
+        HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        cx.range_ready.push_back(e);
+    }
+    // range bounds on multiples of 64 points keep every range's scalars
+    // 64-byte aligned whatever nbytes is
+    std::vector<size_t> lo(K + 1);
+    for (uint32_t k = 0; k < K; k++) lo[k] = (n * k / K) & ~(size_t)63;
+    lo[K] = n;
+    // layouts, buffers and tables of every range first (allocation may
+    // synchronise), then the copies and pipelines range by range
+    std::vector<JobSpec> js(K);
+    std::vector<Layout> Ls(K);
+    std::vector<DevLayout> Ds(K);
+    std::vector<hostec::xyzz6*> outs(K);
+    for (uint32_t k = 0; k < K; k++) {
+        js[k] = make_spec(dp + lo[k] * sizeof(blst_p1_affine), ds + lo[k] * nbytes, lo[k + 1] - lo[k], nbits,
+                          window, false);
+        Ls[k] = make_layout(std::vector<JobSpec>(1, js[k]));
+        Workspace& w = *cx.pipe[k];
+        ensure_workspace(w, Ls[k]);
+        Ds[k] = upload_layout(w, Ls[k]);
+        outs[k] = static_cast<hostec::xyzz6*>(w.host_buf(Ls[k].groups.size() * sizeof(hostec::xyzz6)));
+    }
+    for (uint32_t k = 0; k < K; k++) {
+        const size_t m = lo[k + 1] - lo[k];
+        cx.stager.upload(ds + lo[k] * nbytes, hs + lo[k] * nbytes, m * nbytes, cx.stream);
+        cx.stager.upload(dp + lo[k] * sizeof(blst_p1_affine), hp + lo[k] * sizeof(blst_p1_affine),
+                         m * sizeof(blst_p1_affine), cx.stream);
+        if (fr) launch_k_fr_to_scalar(cx.stream, reinterpret_cast<uint32_t*>(ds + lo[k] * nbytes), (uint32_t)m);
+        HIPCK(hipGetLastError());
+        HIPCK(hipEventRecord(cx.range_ready[k], cx.stream));
+        Workspace& w = *cx.pipe[k];
+        HIPCK(hipStreamWaitEvent(w.stream, cx.range_ready[k], 0));
+        enqueue_batch(w, Ls[k], Ds[k], true, outs[k], k == 0);
+    }
+    hostec::xyzz6 acc;
+    hostec::set_inf(acc);
+    std::chrono::steady_clock::time_point t0;
+    for (uint32_t k = 0; k < K; k++) {
+        HIPCK(hipStreamSynchronize(cx.pipe[k]->stream));
+        if (k + 1 == K) t0 = std::chrono::steady_clock::now();
+        hostec::xyzz6 part;
+        hostec::fold_windows(part, outs[k], (int)Ls[k].groups.size(), (int)js[k].c);
+        hostec::addp(acc, acc, part);
+    }
+    record_phases(*cx.pipe[0]);
+    hostec::jac6 j;
+    hostec::to_jac(j, acc);
+    std::memcpy(ret, &j, sizeof(j));
+    g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // Resolves the reference's descriptor convention (a NULL entry after the head
 // means: continue contiguously from the previous element,
 // blst/scalar/curve/multi_scalar.cpp:838-841) into one contiguous copy.
@@ -899,14 +988,7 @@ int host_msm(blst_p1* ret, const blst_p1_affine* const points[], size_t npoints,
         std::vector<uint8_t> sp, ss;
         const uint8_t* hp = resolve(points, npoints, sizeof(blst_p1_affine), sp);
         const uint8_t* hs = resolve(scalars, npoints, nbytes, ss);
-        cx.pts.ensure(npoints * sizeof(blst_p1_affine));
-        cx.scl.ensure(npoints * nbytes + 16);
-        cx.stager.upload(cx.scl.p, hs, npoints * nbytes, cx.stream);
-        cx.stager.upload(cx.pts.p, hp, npoints * sizeof(blst_p1_affine), cx.stream);
-        JobSpec js = make_spec(cx.pts.p, cx.scl.p, npoints, nbits, window, false);
-        js.bit_base = (uint32_t)bit0;
-        js.tile = !fold;
-        device_msm(cx, js, ret);
+        host_buffers_msm(cx, hp, hs, npoints, nbits, window, false, fold, bit0, ret);
     } catch (const HipFail& f) {
         g_err = f.msg;
         std::memset(ret, 0, sizeof(*ret));
@@ -1304,15 +1386,9 @@ bool fr_msm(blst_p1* ret, const blst_p1_affine points[], const void* fr_scalars,
     try {
         Ctx& cx = context();
         std::lock_guard<std::mutex> lk(cx.mu);
-        cx.pts.ensure(n * sizeof(blst_p1_affine));
-        cx.scl.ensure(n * 32 + 16);
-        cx.stager.upload(cx.scl.p, fr_scalars, n * 32, cx.stream);
-        cx.stager.upload(cx.pts.p, points, n * sizeof(blst_p1_affine), cx.stream);
-        launch_k_fr_to_scalar(cx.stream, cx.scl.as<uint32_t>(), (uint32_t)n);
-        HIPCK(hipGetLastError());
-        const JobSpec js = make_spec(cx.pts.p, cx.scl.p, n, 255, 0, false);
         blst_p1 r;
-        device_msm(cx, js, &r);
+        host_buffers_msm(cx, reinterpret_cast<const uint8_t*>(points), static_cast<const uint8_t*>(fr_scalars), n,
+                         255, 0, true, true, 0, &r);
         *ret = r;
     } catch (const HipFail& f) {
         g_err = f.msg;

@@ -249,6 +249,31 @@ def test_config1_n2pow20_with_edge_cases(hip, orc):
         ds.close()
     want = orc.msm_pippenger(pts, sc, n, 255, threads=16)
     assert hip.compress(got) == orc.compress(want)
+    # the host-buffer entry splits this size into point ranges copied while
+    # the previous range runs
+    assert hip.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == orc.compress(want)
+
+
+@pytest.mark.parametrize("n,nbits,window", [((1 << 19) + 77, 255, 0), ((1 << 19) + 1000, 13, 0),
+                                            (600001, 64, 12), ((1 << 19) + 5, 256, 16)])
+def test_host_buffers_split_ranges(hip, n, nbits, window):
+    """Host-buffer MSMs large enough to be split into point ranges (ragged
+    sizes, short scalars, a caller's window) equal the unsplit MSM over the
+    same inputs in device memory."""
+    nb = (nbits + 7) // 8
+    pts = hip.synth_points(n, 0x5A + n)
+    sc = np.ascontiguousarray(hip.synth_scalars(n, 0x5B + n)[:, :nb])
+    sc[7] = 0
+    sc[n - 3] = 0xFF
+    pts[11] = 0
+    pts[n - 1] = 0
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        want = hip.msm_device(dp.ptr, ds.ptr, n, nbits, window)
+    finally:
+        dp.close()
+        ds.close()
+    assert hip.compress(hip.p1s_mult_pippenger(pts, sc, nbits, window=window)) == hip.compress(want)
 
 
 def test_linearity_at_2pow18(hip, orc):
