@@ -520,7 +520,23 @@ void transfer_to(Workspace& cx, void* dst, const void* src, size_t bytes, hipMem
     HIPCK(hipMemcpyAsync(dst, src, bytes, kind, cx.stream));
 }
 
-DevLayout upload_layout(Workspace& cx, const Layout& L) {
+size_t layout_table_bytes(const Layout& L) {
+    const size_t nj = L.jobs.size(), ng = L.groups.size();
+    auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
+    size_t t = al(nj * sizeof(job_t));
+    t = al(t + nj * 4);
+    t = al(t + nj * 4);
+    t = al(t + nj * 4);
+    t = al(t + ng * sizeof(group_t));
+    t = al(t + ng * 4);
+    t = al(t + ng * sizeof(sgroup_t));
+    t = al(t + L.tiles.size() * sizeof(uint2));
+    t = al(t + ng * 4);
+    t = al(t + L.tree_blk.size() * 4);
+    return al(t + L.tree_tn.size() * sizeof(uint4));
+}
+
+DevLayout upload_layout(Workspace& cx, const Layout& L, size_t base) {
     const size_t nj = L.jobs.size(), ng = L.groups.size();
     auto al = [](size_t x) { return (x + 255) & ~(size_t)255; };
     const size_t o_jobs = 0, o_p = al(o_jobs + nj * sizeof(job_t)), o_e = al(o_p + nj * 4),
@@ -529,8 +545,8 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
                  o_gk = al(o_tb + L.tiles.size() * sizeof(uint2)),
                  o_trb = al(o_gk + ng * 4), o_trt = al(o_trb + L.tree_blk.size() * 4),
                  total = al(o_trt + L.tree_tn.size() * sizeof(uint4));
-    cx.tables.ensure(total);
-    uint8_t* h = cx.tables_stage(total);
+    cx.tables.ensure(base + total);
+    uint8_t* h = cx.tables_stage(base + total) + base;
     std::memcpy(h + o_jobs, L.jobs.data(), nj * sizeof(job_t));
     std::memcpy(h + o_p, L.pbase.data(), nj * 4);
     std::memcpy(h + o_e, L.ebase.data(), nj * 4);
@@ -542,8 +558,8 @@ DevLayout upload_layout(Workspace& cx, const Layout& L) {
     std::memcpy(h + o_gk, L.gkb.data(), ng * 4);
     std::memcpy(h + o_trb, L.tree_blk.data(), L.tree_blk.size() * 4);
     std::memcpy(h + o_trt, L.tree_tn.data(), L.tree_tn.size() * sizeof(uint4));
-    transfer_to(cx, cx.tables.p, h, total, hipMemcpyHostToDevice);
-    uint8_t* d = cx.tables.as<uint8_t>();
+    uint8_t* d = cx.tables.as<uint8_t>() + base;
+    transfer_to(cx, d, h, total, hipMemcpyHostToDevice);
     DevLayout D;
     D.jobs = reinterpret_cast<const job_t*>(d + o_jobs);
     D.pbase = reinterpret_cast<const uint32_t*>(d + o_p);
@@ -639,7 +655,7 @@ size_t workspace_bytes(const Layout& L) {
 // over the field of its group: F = fp for G1, fp2 for G2.
 template <class F>
 void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool host_fold, void* host_out,
-                    bool timing) {
+                    bool timing, const RangeRole& range) {
     using P = xyzz_t<F>;
     const uint32_t nkeys = L.nkeys;
     const uint32_t nent = L.total_entries;
@@ -655,7 +671,7 @@ void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool hos
     if (timing) HIPCK(hipEventRecord(cx.ev[2], st));
     launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.sorted.as<const uint2>(), off + nkeys, D.jobs,
                         D.kbase, njobs, cx.buckets.as<P>(), cx.slot_key.as<uint32_t>(), cx.slots.as<P>(), T,
-                        nchunks);
+                        nchunks, range.prev);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
 
@@ -674,10 +690,13 @@ void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool hos
                            cx.slots.as<const P>(), off, hist, T, cx.buckets.as<P>());
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[4], st));
+    if (range.cnt_sum) launch_k_add_words(st, hist, range.cnt_sum, nkeys, range.prev == nullptr);
+    if (!range.reduce) return;
+    const uint32_t* cnt = range.cnt_sum ? range.cnt_sum : hist;
 
     P* Hn = cx.H.as<P>();
     launch_k_reduce_segments(dim3(div_up(L.total_segments, 256)), dim3(256), st, D.groups, D.sbase, ngroups,
-                             L.total_segments, cx.buckets.as<const P>(), hist, Hn);
+                             L.total_segments, cx.buckets.as<const P>(), cnt, Hn);
     const size_t npass = L.tree_blocks.size();
     for (size_t p = 0; p < npass; p++)
         launch_k_reduce_tree(dim3(L.tree_blocks[p]), st, D.tree_blk + p * ngroups, D.tree_tn + p * ngroups, ngroups,
@@ -699,7 +718,7 @@ void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool hos
 // group); otherwise k_fold combines the windows of every job and one xyzz per
 // job is copied.  host_out should be pinned for the copy to be asynchronous.
 void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host_fold, void* host_out,
-                   bool timing) {
+                   bool timing, const RangeRole& range) {
     const uint32_t nkeys = L.nkeys;
     const uint32_t nent = L.total_entries;
     const uint32_t T = L.T;
@@ -748,8 +767,8 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host
                          cx.ent.as<const uint32_t>(), cx.rank.as<const uint32_t>(), off, cx.sorted.as<uint2>());
     }
     HIPCK(hipGetLastError());
-    if (L.g2) enqueue_points<fp2>(cx, L, D, host_fold, host_out, timing);
-    else enqueue_points<fp>(cx, L, D, host_fold, host_out, timing);
+    if (L.g2) enqueue_points<fp2>(cx, L, D, host_fold, host_out, timing, range);
+    else enqueue_points<fp>(cx, L, D, host_fold, host_out, timing, range);
 }
 
 void record_phases(Workspace& cx) {
@@ -797,6 +816,33 @@ uint32_t pipeline_groups(const JobSpec& js) {
     return std::max<uint32_t>(1, std::min<uint32_t>(G, nwin / 2));
 }
 
+// The window sums of G window-group pipelines (group g on cx.pipe[g], the
+// top windows in group 0, outs[g] its nsums[g] sums) folded on the host into
+// ret.  Horner over the groups from the top: acc holds the groups above g at
+// the base of group g - 1; its shift down to group g's base needs no data of
+// group g and runs before waiting for it.
+void fold_groups(Ctx& cx, uint32_t G, uint32_t nwin, uint32_t c, const std::vector<hostec::xyzz6*>& outs,
+                 const std::vector<uint32_t>& nsums, blst_p1* ret) {
+    hostec::xyzz6 acc;
+    hostec::set_inf(acc);
+    std::chrono::steady_clock::time_point t0;
+    for (uint32_t g = 0; g < G; g++) {
+        const uint32_t base = (G - 1 - g) * nwin / G;
+        if (g > 0) hostec::scale_pow2(acc, acc, (int)(c * ((G - g) * nwin / G - base)));
+        HIPCK(hipStreamSynchronize(cx.pipe[g]->stream));
+        if (g + 1 == G) t0 = std::chrono::steady_clock::now();
+        hostec::xyzz6 part;
+        hostec::fold_windows(part, outs[g], (int)nsums[g], (int)c);
+        hostec::addp(acc, acc, part);
+    }
+    record_phases(*cx.pipe[0]);
+    hostec::jac6 j;
+    hostec::to_jac(j, acc);
+    std::memcpy(ret, &j, sizeof(j));
+    // the fold time left after the last group's results arrive
+    g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
 // one MSM (or tile) over device-resident inputs written on cx.stream
 void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
     const uint32_t G = pipeline_groups(js);
@@ -836,60 +882,29 @@ void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
         HIPCK(hipStreamWaitEvent(w.stream, cx.ready, 0));
         enqueue_batch(w, Ls[g], Ds[g], true, outs[g], g == 0);
     }
-    // Horner over the groups from the top: acc holds the groups above g at
-    // the base of group g - 1; its shift down to group g's base needs no data
-    // of group g and runs before waiting for it
-    hostec::xyzz6 acc;
-    hostec::set_inf(acc);
-    std::chrono::steady_clock::time_point t0;
-    for (uint32_t g = 0; g < G; g++) {
-        const uint32_t base = (G - 1 - g) * nwin / G;
-        if (g > 0) hostec::scale_pow2(acc, acc, (int)(js.c * ((G - g) * nwin / G - base)));
-        HIPCK(hipStreamSynchronize(cx.pipe[g]->stream));
-        if (g + 1 == G) t0 = std::chrono::steady_clock::now();
-        hostec::xyzz6 part;
-        hostec::fold_windows(part, outs[g], (int)Ls[g].groups.size(), (int)js.c);
-        hostec::addp(acc, acc, part);
-    }
-    record_phases(*cx.pipe[0]);
-    hostec::jac6 j;
-    hostec::to_jac(j, acc);
-    std::memcpy(ret, &j, sizeof(j));
-    // the fold time left after the last group's results arrive
-    g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<uint32_t> nsums(G);
+    for (uint32_t g = 0; g < G; g++) nsums[g] = (uint32_t)Ls[g].groups.size();
+    fold_groups(cx, G, nwin, js.c, outs, nsums, ret);
 }
 
-// An MSM over contiguous host buffers (points, and scalars of nbits bits or,
-// with fr, Montgomery-form blst_fr values converted on the device).  Its
-// inputs cross PCIe (134 MB at 2^20 points, about as long as the MSM itself),
-// so from kHostSplitMin points the points are split into kHostRanges ranges,
-// each an MSM of its own on its own pipeline stream: range k + 1 is staged and
-// copied while range k runs, and the host folds each range's windows as it
-// finishes and adds the results.  A range's own reduction costs less than the
-// copy time it hides.
-constexpr size_t kHostSplitMin = (size_t)1 << 19;
-constexpr uint32_t kHostRanges = 2;
-
-void host_buffers_msm(Ctx& cx, const uint8_t* hp, const uint8_t* hs, size_t n, size_t nbits, size_t window,
-                      bool fr, bool fold, size_t bit0, blst_p1* ret) {
-    const size_t nbytes = fr ? 32 : (nbits + 7) / 8;
-    cx.pts.ensure(n * sizeof(blst_p1_affine));
-    cx.scl.ensure(n * nbytes + 16);
-    uint8_t* dp = static_cast<uint8_t*>(cx.pts.p);
-    uint8_t* ds = static_cast<uint8_t*>(cx.scl.p);
-    const uint32_t K = (fold && n >= kHostSplitMin) ? kHostRanges : 1;
-    if (K == 1) {
-        cx.stager.upload(ds, hs, n * nbytes, cx.stream);
-        cx.stager.upload(dp, hp, n * sizeof(blst_p1_affine), cx.stream);
-        if (fr) launch_k_fr_to_scalar(cx.stream, reinterpret_cast<uint32_t*>(ds), (uint32_t)n);
-        HIPCK(hipGetLastError());
-        JobSpec js = make_spec(dp, ds, n, nbits, window, false);
-        js.bit_base = (uint32_t)bit0;
-        js.tile = !fold;
-        device_msm(cx, js, ret);
-        return;
-    }
-    while (cx.pipe.size() < K) {
+// An MSM in K point ranges whose buckets are shared: each window-group
+// pipeline sorts and accumulates range r (the accumulation continues the
+// bucket sums of the earlier ranges, their entry counts summed alongside)
+// and reduces the buckets once, after the last range.  full gives the
+// window and segment length of the whole MSM, a range takes the tile of its
+// own size (the whole size's tile would leave the SIMDs short of waves).
+// stage(r, lo, m) runs before range r's pipelines are queued and leaves the
+// range's inputs (m points from point lo) written on cx.stream.  Range bounds
+// fall on multiples of 64 points, so every range's scalars stay 64-byte
+// aligned whatever the scalar width.
+template <class Stage>
+void ranged_msm(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stage&& stage, blst_p1* ret) {
+    const size_t n = full.n;
+    const uint8_t* dp = static_cast<const uint8_t*>(full.points);
+    const uint8_t* ds = static_cast<const uint8_t*>(full.scalars);
+    const uint32_t G = pipeline_groups(full);
+    const uint32_t nwin = 1 + full.nbits / full.c;
+    while (cx.pipe.size() < G) {
         cx.pipe.emplace_back(new Workspace());
         cx.pipe.back()->init();
     }
@@ -898,53 +913,95 @@ void host_buffers_msm(Ctx& cx, const uint8_t* hp, const uint8_t* hs, size_t n, s
         HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         cx.range_ready.push_back(e);
     }
-    // range bounds on multiples of 64 points keep every range's scalars
-    // 64-byte aligned whatever nbytes is
     std::vector<size_t> lo(K + 1);
     for (uint32_t k = 0; k < K; k++) lo[k] = (n * k / K) & ~(size_t)63;
     lo[K] = n;
-    // layouts, buffers and tables of every range first (allocation may
-    // synchronise), then the copies and pipelines range by range
-    std::vector<JobSpec> js(K);
-    std::vector<Layout> Ls(K);
-    std::vector<DevLayout> Ds(K);
-    std::vector<hostec::xyzz6*> outs(K);
-    for (uint32_t k = 0; k < K; k++) {
-        js[k] = make_spec(dp + lo[k] * sizeof(blst_p1_affine), ds + lo[k] * nbytes, lo[k + 1] - lo[k], nbits,
-                          window, false);
-        Ls[k] = make_layout(std::vector<JobSpec>(1, js[k]));
-        Workspace& w = *cx.pipe[k];
-        ensure_workspace(w, Ls[k]);
-        Ds[k] = upload_layout(w, Ls[k]);
-        outs[k] = static_cast<hostec::xyzz6*>(w.host_buf(Ls[k].groups.size() * sizeof(hostec::xyzz6)));
+    // layouts of every (group, range), buffers and tables first (allocation
+    // may synchronise); each range's tables in a region of their own
+    std::vector<Layout> Ls(G * K);
+    std::vector<DevLayout> Ds(G * K);
+    std::vector<hostec::xyzz6*> outs(G);
+    std::vector<uint32_t> nsums(G);
+    for (uint32_t g = 0; g < G; g++) {
+        Workspace& w = *cx.pipe[g];
+        size_t stride = 0;
+        for (uint32_t r = 0; r < K; r++) {
+            JobSpec s = full;
+            s.points = dp + lo[r] * sizeof(blst_p1_affine);
+            s.scalars = ds + lo[r] * nbytes;
+            s.n = (uint32_t)(lo[r + 1] - lo[r]);
+            s.T = make_spec(s.points, s.scalars, s.n, full.nbits, full.c, false).T;
+            s.win0 = (G - 1 - g) * nwin / G;
+            s.nwin = (G - g) * nwin / G - s.win0;
+            Layout& L = Ls[g * K + r];
+            L = make_layout(std::vector<JobSpec>(1, s));
+            ensure_workspace(w, L);
+            stride = std::max(stride, layout_table_bytes(L));
+        }
+        w.tables.ensure(K * stride);
+        (void)w.tables_stage(K * stride);
+        w.cnt_sum.ensure(((size_t)Ls[g * K].nkeys + 1) * 4);
+        for (uint32_t r = 0; r < K; r++) Ds[g * K + r] = upload_layout(w, Ls[g * K + r], r * stride);
+        nsums[g] = (uint32_t)Ls[g * K].groups.size();
+        outs[g] = static_cast<hostec::xyzz6*>(w.host_buf(nsums[g] * sizeof(hostec::xyzz6)));
     }
-    for (uint32_t k = 0; k < K; k++) {
-        const size_t m = lo[k + 1] - lo[k];
-        cx.stager.upload(ds + lo[k] * nbytes, hs + lo[k] * nbytes, m * nbytes, cx.stream);
-        cx.stager.upload(dp + lo[k] * sizeof(blst_p1_affine), hp + lo[k] * sizeof(blst_p1_affine),
+    for (uint32_t r = 0; r < K; r++) {
+        stage(r, lo[r], lo[r + 1] - lo[r]);
+        HIPCK(hipEventRecord(cx.range_ready[r], cx.stream));
+        for (uint32_t g = 0; g < G; g++) {
+            Workspace& w = *cx.pipe[g];
+            HIPCK(hipStreamWaitEvent(w.stream, cx.range_ready[r], 0));
+            RangeRole role;
+            role.prev = r ? w.cnt_sum.as<const uint32_t>() : nullptr;
+            role.cnt_sum = w.cnt_sum.as<uint32_t>();
+            role.reduce = r + 1 == K;
+            enqueue_batch(w, Ls[g * K + r], Ds[g * K + r], true, outs[g], g == 0 && r + 1 == K, role);
+        }
+    }
+    fold_groups(cx, G, nwin, full.c, outs, nsums, ret);
+}
+
+// An MSM over contiguous host buffers (points, and scalars of nbits bits or,
+// with fr, Montgomery-form blst_fr values converted on the device).  Its
+// inputs cross PCIe (134 MB at 2^20 points, about as long as the MSM itself),
+// so from kHostSplitMin points they are copied in ranges of about
+// kHostRangePoints points, range r + 1 staged and copied while the pipelines
+// work on range r (ranged_msm).
+constexpr size_t kHostSplitMin = (size_t)1 << 19;
+constexpr size_t kHostRangePoints = (size_t)1 << 18;   // 2 to 16 ranges
+
+uint32_t point_ranges(size_t n, size_t min_n, size_t range_points) {
+    if (n < min_n) return 1;
+    return (uint32_t)std::min<size_t>(16, std::max<size_t>(2, n / range_points));
+}
+
+constexpr size_t kDeviceSplitMin = (size_t)1 << 21;
+constexpr size_t kDeviceRangePoints = (size_t)1 << 19;
+
+void host_buffers_msm(Ctx& cx, const uint8_t* hp, const uint8_t* hs, size_t n, size_t nbits, size_t window,
+                      bool fr, bool fold, size_t bit0, blst_p1* ret) {
+    const size_t nbytes = fr ? 32 : (nbits + 7) / 8;
+    cx.pts.ensure(n * sizeof(blst_p1_affine));
+    cx.scl.ensure(n * nbytes + 16);
+    uint8_t* dp = static_cast<uint8_t*>(cx.pts.p);
+    uint8_t* ds = static_cast<uint8_t*>(cx.scl.p);
+    auto stage = [&](uint32_t, size_t lo, size_t m) {
+        cx.stager.upload(ds + lo * nbytes, hs + lo * nbytes, m * nbytes, cx.stream);
+        cx.stager.upload(dp + lo * sizeof(blst_p1_affine), hp + lo * sizeof(blst_p1_affine),
                          m * sizeof(blst_p1_affine), cx.stream);
-        if (fr) launch_k_fr_to_scalar(cx.stream, reinterpret_cast<uint32_t*>(ds + lo[k] * nbytes), (uint32_t)m);
+        if (fr) launch_k_fr_to_scalar(cx.stream, reinterpret_cast<uint32_t*>(ds + lo * nbytes), (uint32_t)m);
         HIPCK(hipGetLastError());
-        HIPCK(hipEventRecord(cx.range_ready[k], cx.stream));
-        Workspace& w = *cx.pipe[k];
-        HIPCK(hipStreamWaitEvent(w.stream, cx.range_ready[k], 0));
-        enqueue_batch(w, Ls[k], Ds[k], true, outs[k], k == 0);
+    };
+    JobSpec js = make_spec(dp, ds, n, nbits, window, false);
+    const uint32_t K = fold ? point_ranges(n, kHostSplitMin, kHostRangePoints) : 1;
+    if (K > 1) {
+        ranged_msm(cx, js, nbytes, K, stage, ret);
+        return;
     }
-    hostec::xyzz6 acc;
-    hostec::set_inf(acc);
-    std::chrono::steady_clock::time_point t0;
-    for (uint32_t k = 0; k < K; k++) {
-        HIPCK(hipStreamSynchronize(cx.pipe[k]->stream));
-        if (k + 1 == K) t0 = std::chrono::steady_clock::now();
-        hostec::xyzz6 part;
-        hostec::fold_windows(part, outs[k], (int)Ls[k].groups.size(), (int)js[k].c);
-        hostec::addp(acc, acc, part);
-    }
-    record_phases(*cx.pipe[0]);
-    hostec::jac6 j;
-    hostec::to_jac(j, acc);
-    std::memcpy(ret, &j, sizeof(j));
-    g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stage(0, 0, n);
+    js.bit_base = (uint32_t)bit0;
+    js.tile = !fold;
+    device_msm(cx, js, ret);
 }
 
 // Resolves the reference's descriptor convention (a NULL entry after the head
@@ -1617,7 +1674,12 @@ int blst_hip_msm_device_tuned(blst_p1* ret, const void* device_points, const voi
         JobSpec js = make_spec(device_points, device_scalars, npoints, nbits, window, false);
         if (chunk) js.T = (uint32_t)chunk;
         if (segment) js.L = (uint32_t)segment;
-        device_msm(cx, js, ret);
+        // large MSMs gather their points range by range (a range's points
+        // stay in the caches while its entries are accumulated); a caller's
+        // tile keeps one range
+        const uint32_t K = chunk ? 1 : point_ranges(npoints, kDeviceSplitMin, kDeviceRangePoints);
+        if (K > 1) ranged_msm(cx, js, (nbits + 7) / 8, K, [](uint32_t, size_t, size_t) {}, ret);
+        else device_msm(cx, js, ret);
     } catch (const HipFail& f) {
         g_err = f.msg;
         std::memset(ret, 0, sizeof(*ret));

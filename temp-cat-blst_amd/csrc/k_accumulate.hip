@@ -9,12 +9,16 @@ namespace bls {
 // ends inside the chunk is final and goes to buckets[key].  Otherwise, with
 // k0 the chunk holding the bucket's first entry: the segment in chunk k0 goes
 // to slot 2*k0+1, the segment in any later chunk k to slot 2*k.
+// With prev (the entry counts of earlier point ranges of the same MSM), a
+// bucket that already holds a sum continues from it: the lane holding the
+// bucket's first entry starts from buckets[key] instead of infinity.
 template <class F>
 __global__ void __launch_bounds__(256)
 k_accumulate(const uint2* __restrict__ sorted,
              const uint32_t* __restrict__ total, const job_t* __restrict__ jobs,
              const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets,
-             uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks) {
+             uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks,
+             const uint32_t* __restrict__ prev) {
     const uint32_t chunk = blockIdx.x * blockDim.x + threadIdx.x;
     if (chunk >= nchunks) return;
     slot_key[2 * chunk] = kInvalid;
@@ -30,7 +34,8 @@ k_accumulate(const uint2* __restrict__ sorted,
     const bool before = start > 0 && sorted[start - 1].x == cur;
     bool first = true;
     xyzz_t<F> acc;
-    xyzz_set_inf(acc);
+    if (prev && !before && prev[cur]) load_xyzz(acc, &buckets[cur]);
+    else xyzz_set_inf(acc);
     #pragma unroll 1
     for (uint32_t e = start; e < end; e++) {
         const uint2 kv = sorted[e];
@@ -44,7 +49,8 @@ k_accumulate(const uint2* __restrict__ sorted,
             }
             first = false;
             cur = key;
-            xyzz_set_inf(acc);
+            if (prev && prev[key]) load_xyzz(acc, &buckets[key]);
+            else xyzz_set_inf(acc);
             if (key >= next_kb) {
                 while (j + 1 < njobs && kbase[j + 1] <= key) j++;
                 next_kb = (j + 1 < njobs) ? kbase[j + 1] : 0xffffffffu;
@@ -74,11 +80,11 @@ k_accumulate(const uint2* __restrict__ sorted,
 }
 
 template <class F>
-void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks) {
-    hipLaunchKernelGGL(k_accumulate<F>, grid, block, 0, stream, sorted, total, jobs, kbase, njobs, buckets, slot_key, slots, T, nchunks);
+void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks, const uint32_t* __restrict__ prev) {
+    hipLaunchKernelGGL(k_accumulate<F>, grid, block, 0, stream, sorted, total, jobs, kbase, njobs, buckets, slot_key, slots, T, nchunks, prev);
 }
 
-template void launch_k_accumulate<fp>(dim3, dim3, hipStream_t, const uint2*, const uint32_t*, const job_t*, const uint32_t*, uint32_t, xyzz*, uint32_t*, xyzz*, uint32_t, uint32_t);
-template void launch_k_accumulate<fp2>(dim3, dim3, hipStream_t, const uint2*, const uint32_t*, const job_t*, const uint32_t*, uint32_t, xyzz2*, uint32_t*, xyzz2*, uint32_t, uint32_t);
+template void launch_k_accumulate<fp>(dim3, dim3, hipStream_t, const uint2*, const uint32_t*, const job_t*, const uint32_t*, uint32_t, xyzz*, uint32_t*, xyzz*, uint32_t, uint32_t, const uint32_t*);
+template void launch_k_accumulate<fp2>(dim3, dim3, hipStream_t, const uint2*, const uint32_t*, const job_t*, const uint32_t*, uint32_t, xyzz2*, uint32_t*, xyzz2*, uint32_t, uint32_t, const uint32_t*);
 
 }  // namespace bls

@@ -368,6 +368,14 @@ k_copy_words(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_
         dst[i] = src[i];
 }
 
+// dst[i] += src[i] (dst[i] = src[i] with first): the running entry counts of
+// the point ranges of one MSM
+__global__ void __launch_bounds__(256)
+k_add_words(const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words, uint32_t first) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < words; i += (size_t)gridDim.x * blockDim.x)
+        dst[i] = first ? src[i] : dst[i] + src[i];
+}
+
 __global__ void __launch_bounds__(kScanThreads)
 k_scan_tiles(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
              uint32_t* __restrict__ tile_sums, uint32_t N) {
@@ -431,6 +439,12 @@ void launch_k_fill(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restri
 void launch_k_copy_words(hipStream_t stream, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words) {
     const uint32_t blocks = (uint32_t)std::min<size_t>((words + 255) / 256, 1024);
     hipLaunchKernelGGL(k_copy_words, dim3(blocks ? blocks : 1), dim3(256), 0, stream, src, dst, words);
+}
+
+void launch_k_add_words(hipStream_t stream, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words,
+                        bool first) {
+    const uint32_t blocks = (uint32_t)std::min<size_t>((words + 255) / 256, 1024);
+    hipLaunchKernelGGL(k_add_words, dim3(blocks ? blocks : 1), dim3(256), 0, stream, src, dst, words, first ? 1u : 0u);
 }
 
 void launch_k_scan_tiles(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t* __restrict__ tile_sums, uint32_t N) {

@@ -10,6 +10,8 @@ void launch_k_digits(dim3 grid, dim3 block, hipStream_t stream, const job_t* __r
 void launch_k_digits_debug(dim3 grid, dim3 block, hipStream_t stream, const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits, uint32_t bit0, uint32_t c, int32_t* __restrict__ out);
 void launch_k_fill(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restrict__ dst, size_t words, uint32_t value);
 void launch_k_copy_words(hipStream_t stream, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words);
+void launch_k_add_words(hipStream_t stream, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words,
+                        bool first);
 void launch_k_scan_tiles(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ in, uint32_t* __restrict__ out, uint32_t* __restrict__ tile_sums, uint32_t N);
 void launch_k_scan_add(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restrict__ out, const uint32_t* __restrict__ tile_offs, uint32_t N);
 void launch_k_scatter(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, uint32_t njobs, uint32_t total_entries, const uint32_t* __restrict__ ent, const uint32_t* __restrict__ rank, const uint32_t* __restrict__ off, uint2* __restrict__ sorted);
@@ -22,7 +24,7 @@ void launch_k_bin_place(dim3 grid, hipStream_t stream, const job_t* __restrict__
 void launch_k_bin_fine(dim3 grid, hipStream_t stream, const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, uint32_t ngroups, const uint32_t* __restrict__ pos, const uint2* __restrict__ tmp, uint2* __restrict__ sorted, uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, uint32_t nkeys, uint32_t ncounts);
 void launch_k_fr_to_scalar(hipStream_t stream, uint32_t* __restrict__ io, uint32_t n);
 template <class F>
-void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks);
+void launch_k_accumulate(dim3 grid, dim3 block, hipStream_t stream, const uint2* __restrict__ sorted, const uint32_t* __restrict__ total, const job_t* __restrict__ jobs, const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks, const uint32_t* __restrict__ prev);
 template <class F>
 void launch_k_partial_light(dim3 grid, dim3 block, hipStream_t stream, const uint32_t* __restrict__ slot_key, const xyzz_t<F>* __restrict__ slots, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt, uint32_t T, uint32_t nchunks, xyzz_t<F>* __restrict__ buckets, uint32_t* __restrict__ counts, uint32_t* __restrict__ wave_list, uint32_t* __restrict__ group_list);
 template <class F>

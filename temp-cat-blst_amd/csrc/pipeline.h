@@ -54,6 +54,7 @@ struct DevBuf {
 struct Workspace {
     hipStream_t stream = nullptr;
     DevBuf ent, rank, counts, hist, off, sorted, buckets, slot_key, slots, heavy, H, scan_tmp, W, res, tables;
+    DevBuf cnt_sum;                       // bucket entry counts summed over point ranges
     uint8_t* tables_pinned = nullptr;     // page-locked staging of the layout tables
     size_t tables_cap = 0;
     hipEvent_t ev[8] = {};
@@ -177,12 +178,24 @@ struct DevLayout {
 uint32_t device_window(size_t n, size_t nbits);
 JobSpec make_spec(const void* points, const void* scalars, size_t n, size_t nbits, size_t window, bool batch);
 Layout make_layout(const std::vector<JobSpec>& specs);
-DevLayout upload_layout(Workspace& cx, const Layout& L);
+// bytes of a layout's device tables; upload_layout writes them at byte
+// offset base of the workspace's table buffer (several layouts in flight on
+// one workspace take separate regions, sized beforehand)
+size_t layout_table_bytes(const Layout& L);
+DevLayout upload_layout(Workspace& cx, const Layout& L, size_t base = 0);
 void ensure_workspace(Workspace& cx, const Layout& L);
 size_t workspace_bytes(const Layout& L);
 bool tiled_sort(const Layout& L);
+// One point range of an MSM whose points arrive in ranges: prev holds the
+// bucket entry counts of the earlier ranges (nullptr for the first), cnt_sum
+// receives the running counts, and only the last range reduces.
+struct RangeRole {
+    const uint32_t* prev = nullptr;
+    uint32_t* cnt_sum = nullptr;
+    bool reduce = true;
+};
 void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host_fold, void* host_out,
-                   bool timing = true);
+                   bool timing = true, const RangeRole& range = RangeRole());
 void record_phases(Workspace& cx);
 
 }  // namespace host

@@ -254,6 +254,26 @@ def test_config1_n2pow20_with_edge_cases(hip, orc):
     assert hip.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == orc.compress(want)
 
 
+def test_2pow21_point_ranges_with_edge_cases(hip, orc):
+    """At 2^21 points both the device-memory entry and the host-buffer entry
+    run the MSM in point ranges sharing one set of buckets; both equal the
+    oracle bit for bit."""
+    n = 1 << 21
+    seed = 0x2F21
+    pts = hip.synth_points(n, seed ^ 1)
+    sc = hip.synth_scalars(n, seed ^ 2)
+    hip.synth_edge_cases(pts, sc, seed ^ 3)
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        got = hip.msm_device(dp.ptr, ds.ptr, n, 255)
+    finally:
+        dp.close()
+        ds.close()
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255, threads=16))
+    assert hip.compress(got) == want
+    assert hip.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
+
+
 @pytest.mark.parametrize("n,nbits,window", [((1 << 19) + 77, 255, 0), ((1 << 19) + 1000, 13, 0),
                                             (600001, 64, 12), ((1 << 19) + 5, 256, 16)])
 def test_host_buffers_split_ranges(hip, n, nbits, window):
