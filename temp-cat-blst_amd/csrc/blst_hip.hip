@@ -198,6 +198,8 @@ struct Ctx {
     size_t wb_host_cap = 0;
     bool gtable_ready = false;
     Stager stager;                       // uploads of pageable caller buffers
+    Stager down;                         // downloads that run beside uploads
+    hipStream_t d2h = nullptr;           // their copies (created on first use)
 };
 
 std::mutex g_ctx_mu;
@@ -1187,9 +1189,9 @@ constexpr size_t kAffineLanes = 65536;
 // points per host round trip of blst_p1s_to_affine (2.4 GB in, 1.6 GB out)
 constexpr size_t kAffineChunk = (size_t)1 << 24;
 
-void enqueue_to_affine(Ctx& cx, const void* d_in, void* d_out, size_t n) {
+void enqueue_to_affine(hipStream_t stream, const void* d_in, void* d_out, size_t n) {
     const uint32_t T = (uint32_t)std::min(n, kAffineLanes);
-    launch_k_jac_to_affine(cx.stream, static_cast<const uint4*>(d_in), static_cast<aff*>(d_out), (uint32_t)n, T);
+    launch_k_jac_to_affine(stream, static_cast<const uint4*>(d_in), static_cast<aff*>(d_out), (uint32_t)n, T);
     HIPCK(hipGetLastError());
 }
 
@@ -1888,6 +1890,83 @@ void blst_hip_device_free(void* device_ptr) {
     if (device_ptr) (void)hipFree(device_ptr);
 }
 
+// Host buffers both ways: the Jacobian points go up (144 B a point) and the
+// affine points come back (96 B), each direction about as long as the other
+// and both far longer than the conversion.  From kAffinePipeMin points the
+// points move in ranges of kAffineRange: the calling thread stages range r up
+// (copies on cx.stream, the conversion on a pipeline stream behind them, so
+// the next range's copies do not wait for it), while a second thread brings
+// converted ranges down through its own page-locked ring and copy pool on a
+// stream of its own: the two directions of the link run at once.
+constexpr size_t kAffinePipeMin = (size_t)1 << 19;
+constexpr size_t kAffineRange = (size_t)1 << 18;
+
+void host_to_affine(Ctx& cx, blst_p1_affine* dst, const uint8_t* hp, size_t n) {
+    const size_t K = (n + kAffineRange - 1) / kAffineRange;
+    if (!cx.d2h) HIPCK(hipStreamCreateWithFlags(&cx.d2h, hipStreamNonBlocking));
+    if (cx.pipe.empty()) {
+        cx.pipe.emplace_back(new Workspace());
+        cx.pipe.back()->init();
+    }
+    hipStream_t conv = cx.pipe[0]->stream;
+    // range r: event 2r when its points are up, 2r + 1 when it is converted
+    while (cx.range_ready.size() < 2 * K) {
+        hipEvent_t e;
+        HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        cx.range_ready.push_back(e);
+    }
+    uint8_t* in = static_cast<uint8_t*>(cx.jac_in.p);
+    uint8_t* out = static_cast<uint8_t*>(cx.aff_out.p);
+    std::mutex mu;
+    std::condition_variable cv;
+    size_t queued = 0;          // ranges whose conversion is queued (event recorded)
+    bool failed = false;        // the calling thread stopped early
+    std::string down_err;
+    std::thread downloader([&] {
+        try {
+            for (size_t r = 0; r < K; r++) {
+                {
+                    std::unique_lock<std::mutex> lk(mu);
+                    cv.wait(lk, [&] { return queued > r || failed; });
+                    if (queued <= r) return;
+                }
+                const size_t lo = r * kAffineRange, m = std::min(kAffineRange, n - lo);
+                HIPCK(hipStreamWaitEvent(cx.d2h, cx.range_ready[2 * r + 1], 0));
+                cx.down.download(dst + lo, out + lo * sizeof(blst_p1_affine), m * sizeof(blst_p1_affine), cx.d2h);
+            }
+        } catch (const HipFail& f) {
+            down_err = f.msg;
+        } catch (const std::exception& e) {
+            down_err = e.what();
+        }
+    });
+    try {
+        for (size_t r = 0; r < K; r++) {
+            const size_t lo = r * kAffineRange, m = std::min(kAffineRange, n - lo);
+            cx.stager.upload(in + lo * sizeof(blst_p1), hp + lo * sizeof(blst_p1), m * sizeof(blst_p1), cx.stream);
+            HIPCK(hipEventRecord(cx.range_ready[2 * r], cx.stream));
+            HIPCK(hipStreamWaitEvent(conv, cx.range_ready[2 * r], 0));
+            enqueue_to_affine(conv, in + lo * sizeof(blst_p1), out + lo * sizeof(blst_p1_affine), m);
+            HIPCK(hipEventRecord(cx.range_ready[2 * r + 1], conv));
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                queued = r + 1;
+            }
+            cv.notify_one();
+        }
+    } catch (...) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            failed = true;
+        }
+        cv.notify_one();
+        downloader.join();
+        throw;
+    }
+    downloader.join();
+    if (!down_err.empty()) throw HipFail{down_err};
+}
+
 void blst_p1s_to_affine(blst_p1_affine dst[], const blst_p1* const points[], size_t npoints) {
     if (npoints == 0) { g_err.clear(); return; }
     if (dst == nullptr) { g_err = "dst is NULL"; return; }
@@ -1906,8 +1985,12 @@ void blst_p1s_to_affine(blst_p1_affine dst[], const blst_p1* const points[], siz
         cx.aff_out.ensure(chunk * sizeof(blst_p1_affine));
         for (size_t b = 0; b < npoints; b += chunk) {
             const size_t m = std::min(chunk, npoints - b);
+            if (m >= kAffinePipeMin) {
+                host_to_affine(cx, dst + b, hp + b * sizeof(blst_p1), m);
+                continue;
+            }
             cx.stager.upload(cx.jac_in.p, hp + b * sizeof(blst_p1), m * sizeof(blst_p1), cx.stream);
-            enqueue_to_affine(cx, cx.jac_in.p, cx.aff_out.p, m);
+            enqueue_to_affine(cx.stream, cx.jac_in.p, cx.aff_out.p, m);
             cx.stager.download(dst + b, cx.aff_out.p, m * sizeof(blst_p1_affine), cx.stream);
         }
     } catch (const HipFail& f) {
@@ -1932,7 +2015,7 @@ int blst_hip_p1s_to_affine_device(void* device_dst, const void* device_points, s
     try {
         Ctx& cx = context();
         std::lock_guard<std::mutex> lk(cx.mu);
-        enqueue_to_affine(cx, device_points, device_dst, npoints);
+        enqueue_to_affine(cx.stream, device_points, device_dst, npoints);
         HIPCK(hipStreamSynchronize(cx.stream));
     } catch (const HipFail& f) {
         g_err = f.msg;

@@ -537,8 +537,10 @@ def _jacobian_inputs(orc, n, seed, inf_every=0):
     return aff, orc.p1s_from_affine_scaled(aff, lam)
 
 
-@pytest.mark.parametrize("n", [1, 2, 3, 100, 1536, 1537, 65536, 65537, 200001])
+@pytest.mark.parametrize("n", [1, 2, 3, 100, 1536, 1537, 65536, 65537, 200001, (1 << 19) + 12345])
 def test_to_affine_matches_oracle(hip, orc, n):
+    """From 2^19 points the host-buffer entry moves the points in ranges with
+    uploads and downloads overlapping; every size against the oracle."""
     aff, jac = _jacobian_inputs(orc, n, 0xAF + n, inf_every=13)
     got = hip.p1s_to_affine(jac)
     assert np.array_equal(got, orc.p1s_to_affine(jac))
