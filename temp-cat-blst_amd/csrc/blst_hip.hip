@@ -2274,7 +2274,9 @@ int blst_hip_p1s_mult_generator(blst_p1_affine* out, const blst_byte* scalars, s
         std::lock_guard<std::mutex> lk(cx.mu);
         ensure_gtable(cx);
         const size_t nbytes = (nbits + 7) / 8;
-        const uint32_t Q = 64;
+        // points per lane of the conversion: enough lanes to fill the device
+        // (one inversion per lane)
+        const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, npoints / kAffineLanes));
         cx.scl.ensure(npoints * nbytes + 16);
         cx.gen_tmp.ensure(npoints * sizeof(xyzz));
         cx.gen_out.ensure(npoints * sizeof(aff));
