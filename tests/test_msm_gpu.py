@@ -296,6 +296,45 @@ def test_host_buffers_split_ranges(hip, n, nbits, window):
     assert hip.compress(hip.p1s_mult_pippenger(pts, sc, nbits, window=window)) == hip.compress(want)
 
 
+@pytest.mark.parametrize("mode,k", [("exact", 1), ("mixed", 37)])
+def test_host_buffers_split_ranges_descriptors(hip, mode, k):
+    """A split-size MSM whose points and scalars come as per-element pointer
+    arrays (the reference's descriptor forms) equals the device-memory MSM."""
+    n = (1 << 19) + 333
+    pts = hip.synth_points(n, 0x6A)
+    sc = hip.synth_scalars(n, 0x6B)
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        want = hip.msm_device(dp.ptr, ds.ptr, n, 255)
+    finally:
+        dp.close()
+        ds.close()
+    got = hip.p1s_mult_pippenger(pts, sc, 255, mode=mode, k=k)
+    assert hip.compress(got) == hip.compress(want)
+
+
+def test_fr_split_ranges_ragged(hip, orc):
+    """The blst_fr entry at a ragged split size (its scalars converted on the
+    device range by range) equals the byte-scalar entry on the oracle's
+    conversion of the same values."""
+    n = (1 << 19) + 4097
+    pts = hip.synth_points(n, 0x6C)
+    rnd = np.random.default_rng(0x6D)
+    fr = rnd.integers(0, 1 << 63, size=(n, 4), dtype=np.uint64)
+    fr[:, 3] &= np.uint64(0x0FFFFFFFFFFFFFFF)
+    fr[::1000] = 0
+    sc = orc.fr_to_scalars(fr)
+    ok, got = hip.p1s_mult_fr(pts, fr, "pippenger")
+    assert ok
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        want = hip.msm_device(dp.ptr, ds.ptr, n, 255)
+    finally:
+        dp.close()
+        ds.close()
+    assert hip.compress(got) == hip.compress(want)
+
+
 def test_linearity_at_2pow18(hip, orc):
     """MSM(k) + MSM(k') == MSM(k + k' mod r): a size-independent property."""
     n = 1 << 18
