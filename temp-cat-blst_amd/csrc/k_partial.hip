@@ -3,11 +3,13 @@
 // Chunk k leaves the bucket that starts in it and continues in slot 2k+1 and
 // the bucket that started before it in slot 2k (partial j = 0 of a bucket
 // whose first chunk is k0 is slot 2*k0+1, partial j >= 1 is slot 2*(k0+j)).
-// Most spanning buckets cross one boundary (two partials): one lane adds
-// them.  Buckets with up to kWaveSpan partials go to a list worked by one
-// wave per bucket (16 quads, four-lane additions, strides then a shuffle
-// tree); longer ones to a list worked by one workgroup per bucket (256
-// quads, strides then a tree in LDS).  Short top windows put many entries
+// Most spanning buckets cross one or two boundaries: one lane adds their up
+// to kLightSpan partials (a tile smaller than a bucket's entries, as for
+// sizes without a tuned tile, makes three-piece buckets common; a wave each
+// would cost more than the additions).  Buckets with up to kWaveSpan
+// partials go to a list worked by one wave per bucket (16 quads, four-lane
+// additions, strides then a shuffle tree); longer ones to a list worked by
+// one workgroup per bucket (256 quads, strides then a tree in LDS).  Short top windows put many entries
 // into few buckets, so a batch of MSMs can hold very many listed buckets.
 #include "msm_common.hip.h"
 #include "ec_quad.hip.h"
@@ -15,6 +17,7 @@
 
 namespace bls {
 
+constexpr uint32_t kLightSpan = 4;
 constexpr uint32_t kWaveSpan = 128;
 
 // counts[0]: buckets listed for a wave each, counts[1]: for a workgroup each
@@ -29,11 +32,14 @@ k_partial_light(const uint32_t* __restrict__ slot_key, const xyzz_t<F>* __restri
     const uint32_t key = slot_key[2 * k + 1];
     if (key == kInvalid) return;
     const uint32_t m = (off[key] + cnt[key] - 1) / T - k + 1;   // partials
-    if (m == 2) {
+    if (m <= kLightSpan) {
         xyzz_t<F> a, b;
         load_xyzz(a, &slots[2 * k + 1]);
-        load_xyzz(b, &slots[2 * k + 2]);
-        xyzz_add(a, a, b);
+#pragma unroll 1
+        for (uint32_t j = 1; j < m; j++) {
+            load_xyzz(b, &slots[2 * (k + j)]);
+            xyzz_add(a, a, b);
+        }
         store_xyzz(&buckets[key], a);
     } else if (m <= kWaveSpan) {
         wave_list[atomicAdd(&counts[0], 1u)] = key;
