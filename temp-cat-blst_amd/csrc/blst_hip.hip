@@ -1676,9 +1676,8 @@ int blst_hip_msm_device_tuned(blst_p1* ret, const void* device_points, const voi
         JobSpec js = make_spec(device_points, device_scalars, npoints, nbits, window, false);
         if (chunk) js.T = (uint32_t)chunk;
         if (segment) js.L = (uint32_t)segment;
-        // large MSMs gather their points range by range (a range's points
-        // stay in the caches while its entries are accumulated); a caller's
-        // tile keeps one range
+        // large MSMs run in point ranges (each range's sort scatters its
+        // entries within a few tens of MB); a caller's tile keeps one range
         const uint32_t K = chunk ? 1 : point_ranges(npoints, kDeviceSplitMin, kDeviceRangePoints);
         if (K > 1) ranged_msm(cx, js, (nbits + 7) / 8, K, [](uint32_t, size_t, size_t) {}, ret);
         else device_msm(cx, js, ret);
