@@ -14,6 +14,10 @@ and writes the table the library loads at start-up (tuning.txt next to
 libblst_hip.so, or the path in BLST_HIP_TUNING).
 
     python3 temp-cat-blst_amd/autotune.py [--out PATH] [--single 6-20] [--batch 6-20]
+
+With --nbits 64 or 128 it measures lone MSMs of scalars that wide (rows
+"single64" / "single128", used for 33..64-bit and 65..128-bit scalars);
+--append adds the rows to an existing table instead of replacing it.
 """
 import argparse
 import os
@@ -62,24 +66,27 @@ def _descend(measure, c0):
     return best
 
 
-def tune(single_range, batch_range, reps=3, log=print):
+def tune(single_range, batch_range, reps=3, log=print, nbits=255):
     n_max = 1 << max(list(single_range) + list(batch_range) + [21])
     points = bh.synth_points(min(n_max, BATCH_POINTS * 2), 0xA770)
     scalars = bh.synth_scalars(points.shape[0], 0xA771)
+    mode = "single" if nbits >= 200 else "single%d" % nbits
+    if nbits < 200:
+        scalars = np.ascontiguousarray(scalars[:, :(nbits + 7) // 8])
     d_p = bh.DeviceBuffer(points)
     d_s = bh.DeviceBuffer(scalars)
     rows = []
     try:
         for lg in single_range:
             n = 1 << lg
-            c0 = bh.tuning_get(0, n)[0]
+            c0 = bh.tuning_get(0, n, nbits)[0]
 
             def measure(c, T, L):
-                return _median_time(lambda: bh.msm_device(d_p.ptr, d_s.ptr, n, 255, c, T, L), reps)
+                return _median_time(lambda: bh.msm_device(d_p.ptr, d_s.ptr, n, nbits, c, T, L), reps)
 
             t, c, T, L = _descend(measure, c0)
-            rows.append(("single", lg, c, T, L, t))
-            log("single 2^%-2d c=%-2d T=%-2d L=%-2d %.3f ms" % (lg, c, T, L, t * 1e3))
+            rows.append((mode, lg, c, T, L, t))
+            log("%s 2^%-2d c=%-2d T=%-2d L=%-2d %.3f ms" % (mode, lg, c, T, L, t * 1e3))
         for lg in batch_range:
             n = 1 << lg
             k = max(1, BATCH_POINTS // n)
@@ -102,9 +109,10 @@ def tune(single_range, batch_range, reps=3, log=print):
     return rows
 
 
-def write_table(rows, path):
-    with open(path, "w") as f:
-        f.write("# mode log2n window chunk segment   (measured ms)\n")
+def write_table(rows, path, append=False):
+    with open(path, "a" if append else "w") as f:
+        if not append:
+            f.write("# mode log2n window chunk segment   (measured ms)\n")
         for mode, lg, c, T, L, t in rows:
             f.write("%s %d %d %d %d   # %.4f\n" % (mode, lg, c, T, L, t * 1e3))
 
@@ -120,12 +128,14 @@ def main():
     ap.add_argument("--single", default="6-20")
     ap.add_argument("--batch", default="6-20")
     ap.add_argument("--reps", type=int, default=3)
+    ap.add_argument("--nbits", type=int, default=255, choices=(64, 128, 255))
+    ap.add_argument("--append", action="store_true")
     args = ap.parse_args()
     if not bh.available():
         raise SystemExit("no gfx950 device visible")
-    rows = tune(_range(args.single), _range(args.batch), args.reps,
-                log=lambda m: print(m, flush=True))
-    write_table(rows, args.out)
+    batch = _range(args.batch) if args.nbits == 255 else range(0)
+    rows = tune(_range(args.single), batch, args.reps, log=lambda m: print(m, flush=True), nbits=args.nbits)
+    write_table(rows, args.out, args.append)
     print("wrote", args.out)
 
 

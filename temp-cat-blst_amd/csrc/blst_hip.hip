@@ -255,8 +255,10 @@ uint32_t device_window(size_t n, size_t nbits) {
 //
 // Two tables (a lone MSM, and a job inside a batch) indexed by floor(log2 n),
 // filled from the autotuner's file when present: BLST_HIP_TUNING, else
-// tuning.txt next to this library.  Lines: "single|batch log2n c T L".
-// Entries the file does not set fall back to the cost model.
+// tuning.txt next to this library.  Lines: "single|batch log2n c T L" for
+// full-width scalars (200 bits or more), and "single64|single128 log2n c T L"
+// for lone MSMs of 33..64-bit and 65..128-bit scalars.  Entries the file
+// does not set fall back to the cost model.
 
 struct TuneTable {
     Tune t[32];
@@ -265,12 +267,18 @@ struct TuneTable {
 
 std::mutex g_tune_mu;
 TuneTable g_tune[2];
+TuneTable g_tune_short[2];   // lone MSMs of 33..64-bit and 65..128-bit scalars
 bool g_tune_loaded = false;
+
+// the short-scalar table of a width (-1: none)
+int short_class(size_t nbits) { return nbits > 32 && nbits <= 64 ? 0 : (nbits > 64 && nbits <= 128 ? 1 : -1); }
 
 void load_tuning_locked() {
     if (g_tune_loaded) return;
     g_tune_loaded = true;
     for (auto& tb : g_tune)
+        for (bool& b : tb.set) b = false;
+    for (auto& tb : g_tune_short)
         for (bool& b : tb.set) b = false;
     std::string path;
     if (const char* env = std::getenv("BLST_HIP_TUNING")) {
@@ -291,10 +299,15 @@ void load_tuning_locked() {
     while (std::fgets(line, sizeof line, f)) {
         if (line[0] == '#') continue;
         if (std::sscanf(line, "%15s %u %u %u %u", mode, &lg, &c, &T, &L) != 5) continue;
-        const int m = std::strcmp(mode, "batch") == 0 ? 1 : (std::strcmp(mode, "single") == 0 ? 0 : -1);
+        const int m = std::strcmp(mode, "batch") == 0      ? 1
+                      : std::strcmp(mode, "single") == 0   ? 0
+                      : std::strcmp(mode, "single64") == 0  ? 2
+                      : std::strcmp(mode, "single128") == 0 ? 3
+                                                            : -1;
         if (m < 0 || lg > 31 || c < 1 || c > 24 || T < 1 || T > 4096 || L < 1 || L > 4096) continue;
-        g_tune[m].t[lg] = Tune{c, T, L};
-        g_tune[m].set[lg] = true;
+        TuneTable& tb = m < 2 ? g_tune[m] : g_tune_short[m - 2];
+        tb.t[lg] = Tune{c, T, L};
+        tb.set[lg] = true;
     }
     std::fclose(f);
 }
@@ -309,8 +322,10 @@ Tune tuned(size_t n, size_t nbits, bool batch) {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     load_tuning_locked();
     const int lg = std::min(floor_log2(std::max<size_t>(n, 1)), 31);
-    // the tables are measured for full-width scalars
+    // the main tables are measured for full-width scalars
     if (nbits >= 200 && g_tune[batch ? 1 : 0].set[lg]) return g_tune[batch ? 1 : 0].t[lg];
+    const int sc = short_class(nbits);
+    if (!batch && sc >= 0 && g_tune_short[sc].set[lg]) return g_tune_short[sc].t[lg];
     return Tune{device_window(n, nbits), 32, 8};
 }
 
