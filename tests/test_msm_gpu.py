@@ -274,6 +274,28 @@ def test_2pow21_point_ranges_with_edge_cases(hip, orc):
     assert hip.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
 
 
+@pytest.mark.parametrize("nbits,window", [(64, 0), (128, 0), (255, 13)])
+def test_device_point_ranges_ragged(hip, orc, nbits, window):
+    """Device-memory MSMs above 2^21 points run in point ranges: a ragged size
+    with short scalars (their own tuned windows) or a caller's window, with
+    zero scalars and infinity points in several ranges, against the oracle."""
+    n = (1 << 21) + 4099
+    nb = (nbits + 7) // 8
+    pts = hip.synth_points(n, 0x7A + nbits)
+    sc = np.ascontiguousarray(hip.synth_scalars(n, 0x7B + nbits)[:, :nb])
+    for i in (0, 1, n // 3, n // 2 + 5, n - 1):
+        sc[i] = 0
+        pts[i + 7 if i + 7 < n else i - 7] = 0
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        got = hip.msm_device(dp.ptr, ds.ptr, n, nbits, window)
+    finally:
+        dp.close()
+        ds.close()
+    want = orc.msm_pippenger(pts, sc, n, nbits, threads=16)
+    assert hip.compress(got) == orc.compress(want)
+
+
 @pytest.mark.parametrize("n,nbits,window", [((1 << 19) + 77, 255, 0), ((1 << 19) + 1000, 13, 0),
                                             (600001, 64, 12), ((1 << 19) + 5, 256, 16)])
 def test_host_buffers_split_ranges(hip, n, nbits, window):
