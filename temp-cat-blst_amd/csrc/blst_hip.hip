@@ -1201,6 +1201,10 @@ bool specs_from_jobs(const blst_hip_msm_job* jobs, size_t njobs, std::vector<Job
 // Lanes of the Jacobian -> affine kernel: four waves per CU, each lane
 // inverting once for every n / 65536 points.
 constexpr size_t kAffineLanes = 65536;
+// lanes of the XYZZ -> affine conversion of table rows and comb tables
+// (131,072 against 65,536: comb table of 4096 points at wbits 13 built in 190
+// against 205 ms; 262,144 no better)
+constexpr size_t kTableAffineLanes = 131072;
 // points per host round trip of blst_p1s_to_affine (2.4 GB in, 1.6 GB out)
 constexpr size_t kAffineChunk = (size_t)1 << 24;
 
@@ -1230,7 +1234,7 @@ void enqueue_wbits_table(Ctx& cx, const void* d_pts, size_t n, size_t wbits, voi
         const size_t count = nr * m;
         launch_k_wbits_rows(cx.stream, static_cast<const uint4*>(d_pts), (uint32_t)row0, (uint32_t)nr, (uint32_t)m, L,
                             cx.wb_rows.as<xyzz>());
-        const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, count / kAffineLanes));
+        const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, count / kTableAffineLanes));
         launch_k_to_affine(dim3(div_up(div_up(count, Q), 256)), dim3(256), cx.stream, cx.wb_rows.as<const xyzz>(),
                            static_cast<aff*>(d_table) + row0 * m, (uint32_t)count, Q);
         HIPCK(hipGetLastError());
@@ -2211,7 +2215,7 @@ int blst_hip_comb_table_build(void* device_table, size_t wbits, size_t nbits, co
                 // the next window's bases: [2^wbits] Q
                 launch_k_shift_points(cx.stream, cx.comb_q.as<const uint4>(), (uint32_t)npoints, (uint32_t)wbits,
                                       cx.comb_x.as<xyzz>());
-                const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, npoints / kAffineLanes));
+                const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, npoints / kTableAffineLanes));
                 launch_k_to_affine(dim3(div_up(div_up(npoints, Q), 256)), dim3(256), cx.stream,
                                    cx.comb_x.as<const xyzz>(), cx.comb_q.as<aff>(), (uint32_t)npoints, Q);
                 HIPCK(hipGetLastError());
@@ -2290,7 +2294,7 @@ int blst_hip_p1s_mult_generator(blst_p1_affine* out, const blst_byte* scalars, s
         const size_t nbytes = (nbits + 7) / 8;
         // points per lane of the conversion: enough lanes to fill the device
         // (one inversion per lane)
-        const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, npoints / kAffineLanes));
+        const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, npoints / kTableAffineLanes));
         cx.scl.ensure(npoints * nbytes + 16);
         cx.gen_tmp.ensure(npoints * sizeof(xyzz));
         cx.gen_out.ensure(npoints * sizeof(aff));
