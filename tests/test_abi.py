@@ -275,3 +275,28 @@ def test_comb_table_bytes():
     assert L.blst_hip_comb_table_bytes(7, 2, 1) == 1 * 7 * 2 * 96
     for args in ((0, 8, 255), (5, 1, 255), (5, 15, 255), (5, 8, 0), (5, 8, 257), ((1 << 62), 14, 255)):
         assert L.blst_hip_comb_table_bytes(*args) == 0, args
+
+
+def test_tuning_rows_by_scalar_width():
+    """The library's tuning lookup (host only): full-width scalars take the
+    measured single-MSM rows, 33..64-bit and 65..128-bit scalars their own
+    rows, other widths the cost model; every entry equals the committed
+    table's line for that size."""
+    import blst_hip
+    rows = {}
+    for line in open(os.path.join(ROOT, "temp-cat-blst_amd", "tuning.txt")):
+        f = line.split("#")[0].split()
+        if len(f) == 5:
+            rows[(f[0], int(f[1]))] = tuple(int(x) for x in f[2:])
+    for lg in (12, 16, 20):
+        n = 1 << lg
+        assert blst_hip.tuning_get(0, n, 255) == rows[("single", lg)]
+        assert blst_hip.tuning_get(1, n, 255) == rows[("batch", lg)]
+        assert blst_hip.tuning_get(0, n, 64) == rows[("single64", lg)]
+        assert blst_hip.tuning_get(0, n, 40) == rows[("single64", lg)]
+        assert blst_hip.tuning_get(0, n, 128) == rows[("single128", lg)]
+        assert blst_hip.tuning_get(0, n, 100) == rows[("single128", lg)]
+        # 1..32 and 129..199 bits: not tabulated, a window within the width's reach
+        for nbits in (8, 32, 160):
+            c, T, L = blst_hip.tuning_get(0, n, nbits)
+            assert 1 <= c <= 24 and T >= 1 and L >= 1
