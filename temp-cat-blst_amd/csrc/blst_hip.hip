@@ -1098,6 +1098,66 @@ void device_msm_g2(Ctx& cx, const JobSpec& js, blst_p2* ret) {
     std::memcpy(ret, &j, sizeof(j));
 }
 
+// A G2 MSM in K point ranges sharing one set of buckets, as ranged_msm does
+// for G1: one window group, on a pipeline stream of its own so that the
+// copies of the next range (queued on cx.stream by stage) run beside it.
+template <class Stage>
+void ranged_msm_g2(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stage&& stage, blst_p2* ret) {
+    const size_t n = full.n;
+    const uint8_t* dp = static_cast<const uint8_t*>(full.points);
+    const uint8_t* ds = static_cast<const uint8_t*>(full.scalars);
+    if (cx.pipe.empty()) {
+        cx.pipe.emplace_back(new Workspace());
+        cx.pipe.back()->init();
+    }
+    Workspace& w = *cx.pipe[0];
+    while (cx.range_ready.size() < K) {
+        hipEvent_t e;
+        HIPCK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        cx.range_ready.push_back(e);
+    }
+    std::vector<size_t> lo(K + 1);
+    for (uint32_t k = 0; k < K; k++) lo[k] = (n * k / K) & ~(size_t)63;
+    lo[K] = n;
+    std::vector<Layout> Ls(K);
+    std::vector<DevLayout> Ds(K);
+    size_t stride = 0;
+    for (uint32_t r = 0; r < K; r++) {
+        JobSpec s = full;
+        s.points = dp + lo[r] * sizeof(blst_p2_affine);
+        s.scalars = ds + lo[r] * nbytes;
+        s.n = (uint32_t)(lo[r + 1] - lo[r]);
+        s.T = make_spec(s.points, s.scalars, s.n, full.nbits, full.c, false).T;
+        s.g2 = true;
+        Ls[r] = make_layout(std::vector<JobSpec>(1, s));
+        ensure_workspace(w, Ls[r]);
+        stride = std::max(stride, layout_table_bytes(Ls[r]));
+    }
+    w.tables.ensure(K * stride);
+    (void)w.tables_stage(K * stride);
+    w.cnt_sum.ensure(((size_t)Ls[0].nkeys + 1) * 4);
+    for (uint32_t r = 0; r < K; r++) Ds[r] = upload_layout(w, Ls[r], r * stride);
+    const uint32_t nsum = (uint32_t)Ls[0].groups.size();
+    auto* out = static_cast<hostec2::xyzz12*>(w.host_buf(nsum * sizeof(hostec2::xyzz12)));
+    for (uint32_t r = 0; r < K; r++) {
+        stage(r, lo[r], lo[r + 1] - lo[r]);
+        HIPCK(hipEventRecord(cx.range_ready[r], cx.stream));
+        HIPCK(hipStreamWaitEvent(w.stream, cx.range_ready[r], 0));
+        RangeRole role;
+        role.prev = r ? w.cnt_sum.as<const uint32_t>() : nullptr;
+        role.cnt_sum = w.cnt_sum.as<uint32_t>();
+        role.reduce = r + 1 == K;
+        enqueue_batch(w, Ls[r], Ds[r], true, out, r + 1 == K, role);
+    }
+    HIPCK(hipStreamSynchronize(w.stream));
+    record_phases(w);
+    hostec2::xyzz12 res;
+    hostec2::fold_windows(res, out, (int)nsum, (int)full.c);
+    hostec2::jac12 j;
+    hostec2::to_jac(j, res);
+    std::memcpy(ret, &j, sizeof(j));
+}
+
 int host_msm_g2(blst_p2* ret, const blst_p2_affine* const points[], size_t npoints,
                 const blst_byte* const scalars[], size_t nbits, size_t window, bool fold, size_t bit0) {
     try {
@@ -1109,12 +1169,24 @@ int host_msm_g2(blst_p2* ret, const blst_p2_affine* const points[], size_t npoin
         const uint8_t* hs = resolve(scalars, npoints, nbytes, ss);
         cx.pts.ensure(npoints * sizeof(blst_p2_affine));
         cx.scl.ensure(npoints * nbytes + 16);
-        cx.stager.upload(cx.scl.p, hs, npoints * nbytes, cx.stream);
-        cx.stager.upload(cx.pts.p, hp, npoints * sizeof(blst_p2_affine), cx.stream);
-        JobSpec js = make_spec(cx.pts.p, cx.scl.p, npoints, nbits, window, false);
-        js.bit_base = (uint32_t)bit0;
-        js.tile = !fold;
-        device_msm_g2(cx, js, ret);
+        uint8_t* dp = static_cast<uint8_t*>(cx.pts.p);
+        uint8_t* ds = static_cast<uint8_t*>(cx.scl.p);
+        auto stage = [&](uint32_t, size_t lo, size_t m) {
+            cx.stager.upload(ds + lo * nbytes, hs + lo * nbytes, m * nbytes, cx.stream);
+            cx.stager.upload(dp + lo * sizeof(blst_p2_affine), hp + lo * sizeof(blst_p2_affine),
+                             m * sizeof(blst_p2_affine), cx.stream);
+        };
+        JobSpec js = make_spec(dp, ds, npoints, nbits, window, false);
+        // from host buffers the points cross PCIe in ranges, as for G1
+        const uint32_t K = fold ? point_ranges(npoints, kHostSplitMin, kHostRangePoints) : 1;
+        if (K > 1) {
+            ranged_msm_g2(cx, js, nbytes, K, stage, ret);
+        } else {
+            stage(0, 0, npoints);
+            js.bit_base = (uint32_t)bit0;
+            js.tile = !fold;
+            device_msm_g2(cx, js, ret);
+        }
     } catch (const HipFail& f) {
         g_err = f.msg;
         std::memset(ret, 0, sizeof(*ret));

@@ -822,6 +822,25 @@ def test_g2_edge_inputs(hip, orc, g2_points):
     assert orc.p2_compress(hip.p2s_mult_pippenger(pts, Z, 255)) == bytes([0xC0]) + bytes(95)
 
 
+@pytest.mark.parametrize("n", [(1 << 19) + 333, (1 << 20) + 64])
+def test_g2_host_buffers_in_point_ranges(hip, orc, n):
+    """From 2^19 points the G2 host-buffer entry copies the points in ranges
+    that share one set of buckets; it equals the one-range MSM over the same
+    inputs in device memory (zero scalars and infinity points included)."""
+    pts = hip.synth_p2_points(n, 0x3C + n)
+    sc = hip.synth_scalars(n, 0x3D + n)
+    for i in (0, n // 2, n - 1):
+        sc[i] = 0
+        pts[i + 3 if i + 3 < n else i - 3] = 0
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        want = hip.p2_msm_device(dp.ptr, ds.ptr, n, 255)
+    finally:
+        dp.close()
+        ds.close()
+    assert orc.p2_compress(hip.p2s_mult_pippenger(pts, sc, 255)) == orc.p2_compress(want)
+
+
 def test_g2_hash_to_curve_points_textbook(hip):
     """The 10 G2 hash-to-curve outputs as bases with small scalars, against
     the textbook Fp2 model."""
