@@ -294,40 +294,6 @@ int blst_hip_comb_table_build(void *device_table, size_t wbits, size_t nbits, co
 int blst_hip_comb_msm_batch(blst_p1 *rets, const void *device_table, size_t wbits, size_t npoints,
                             const void *device_scalars, size_t njobs, size_t nbits);
 
-/* Test hook: the signed Booth digits the device computes for window size
- * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
- * written to out[npoints] (host).  Returns 0 on success. */
-int blst_hip_booth_digits(int32_t *out, const blst_byte *scalars, size_t npoints,
-                          size_t nbits, size_t bit0, size_t window);
-
-/* Test hook: checks the four-lane point additions and doublings of the
- * reduction stages against the one-lane formulas on the affine points
- * (device memory, 96 bytes each, on the curve): for each point P and its
- * successor Q, 2P + 2Q, 2P + 2P in two frames, 2P - 2P, sums with infinity
- * and doublings.  Returns the number of mismatching coordinates, -1 on error. */
-int blst_hip_point_ops_check(const void *device_points, size_t npoints);
-
-/* [k_i]G for each scalar (generator multiples, affine, all-zero for k = 0),
- * computed on the device from a comb table of G: the synthetic bases of the
- * benchmark workloads and KZG setup points [tau^i]G.  Returns 0 on success. */
-int blst_hip_p1s_mult_generator(blst_p1_affine *out, const blst_byte *scalars, size_t npoints,
-                                size_t nbits);
-
-/* Seeded workload generators (xorshift64*): n uniform elements of Fr as
- * 32-byte little-endian scalars (low64_pct percent keep only their low 64
- * bits); the KZG exponents tau^i mod r with tau drawn from the seed; and the
- * edge-case mix (1% zero, 1% one, 1% r-1 scalars, 0.5% repeated points)
- * applied in place at seeded distinct positions; blst_fr (Montgomery form)
- * of canonical scalars below r; Jacobian representatives (x l^2, y l^3, l) of
- * affine points at seeded nonzero l (all-zero points stay all-zero). */
-void blst_hip_synth_scalars(blst_byte *out32, size_t n, uint64_t seed, int low64_pct);
-void blst_hip_synth_srs_scalars(blst_byte *out32, size_t n, uint64_t seed);
-void blst_hip_synth_edge_cases(void *points96, blst_byte *scalars32, size_t n, uint64_t seed);
-void blst_hip_synth_fr(uint64_t *out4, const blst_byte *scalars32, size_t n);
-void blst_hip_synth_jacobian(void *points144, const void *affine96, size_t n, uint64_t seed);
-/* G2 bases [a + i b] G2 (affine, 192 bytes each) for a, b drawn from the seed */
-void blst_hip_synth_p2_points(void *points192, size_t n, uint64_t seed);
-
 /* 48-byte ZCash compressed encoding of a G1 Jacobian point, the format of
  * blst_p1_compress (bindings/blst_scalar.h:609, blst/c_api.cpp:1618-1621),
  * computed on the host by this library. */

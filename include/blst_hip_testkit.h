@@ -1,0 +1,72 @@
+/*
+ * blst_hip_testkit.h -- test and workload-generation hooks of the gfx950 MSM
+ * backend, built as libblst_hip_testkit.so beside libblst_hip.so (which it
+ * links against).  None of these has a counterpart in the reference and none
+ * is part of the drop-in surface: they let tests compare device arithmetic
+ * with exact host arithmetic and let benchmarks generate seeded inputs.
+ * Errors are reported through blst_hip_last_error() of blst_hip.h.
+ */
+#ifndef BLST_HIP_TESTKIT_H
+#define BLST_HIP_TESTKIT_H
+
+#include "blst_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One device field operation over n elements (48-byte Montgomery limbs, host
+ * arrays): op 0 a*b, 1 a^2, 2 a*b + c*d with one reduction, 3 a + b, 4 a - b,
+ * 5 2a, 6 -a, 7 -a when bit 0 of b's first limb is set (else a), 8 the
+ * canonical representative of a, 9 whether a is zero modulo p (1 or 0 in the
+ * first limb), 10 a^-1 (0 for 0).  Inputs must lie in [0, 2p), the device
+ * layer's redundant range; outputs lie in [0, 2p) (op 8: [0, p)).  c and d are
+ * read only by op 2.  Returns 0 on success, -1 on invalid arguments or a
+ * device error. */
+int blst_hip_fp_ops(int op, void *out, const void *a, const void *b, const void *c, const void *d, size_t n);
+
+/* Makes every later device allocation of the product library larger than
+ * `bytes` fail as an out-of-memory device would (0 lifts the cap), so tests
+ * can drive the fail-closed paths: infinity (or NULL / -1) and a message in
+ * blst_hip_last_error(), with later calls unaffected once the cap is lifted. */
+void blst_hip_set_alloc_cap(size_t bytes);
+
+/* Test hook: the signed Booth digits the device computes for window size
+ * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
+ * written to out[npoints] (host).  Returns 0 on success. */
+int blst_hip_booth_digits(int32_t *out, const blst_byte *scalars, size_t npoints,
+                          size_t nbits, size_t bit0, size_t window);
+
+/* Test hook: checks the four-lane point additions and doublings of the
+ * reduction stages against the one-lane formulas on the affine points
+ * (device memory, 96 bytes each, on the curve): for each point P and its
+ * successor Q, 2P + 2Q, 2P + 2P in two frames, 2P - 2P, sums with infinity
+ * and doublings.  Returns the number of mismatching coordinates, -1 on error. */
+int blst_hip_point_ops_check(const void *device_points, size_t npoints);
+
+/* [k_i]G for each scalar (generator multiples, affine, all-zero for k = 0),
+ * computed on the device from a comb table of G: the synthetic bases of the
+ * benchmark workloads and KZG setup points [tau^i]G.  Returns 0 on success. */
+int blst_hip_p1s_mult_generator(blst_p1_affine *out, const blst_byte *scalars, size_t npoints,
+                                size_t nbits);
+
+/* Seeded workload generators (xorshift64*): n uniform elements of Fr as
+ * 32-byte little-endian scalars (low64_pct percent keep only their low 64
+ * bits); the KZG exponents tau^i mod r with tau drawn from the seed; and the
+ * edge-case mix (1% zero, 1% one, 1% r-1 scalars, 0.5% repeated points)
+ * applied in place at seeded distinct positions; blst_fr (Montgomery form)
+ * of canonical scalars below r; Jacobian representatives (x l^2, y l^3, l) of
+ * affine points at seeded nonzero l (all-zero points stay all-zero). */
+void blst_hip_synth_scalars(blst_byte *out32, size_t n, uint64_t seed, int low64_pct);
+void blst_hip_synth_srs_scalars(blst_byte *out32, size_t n, uint64_t seed);
+void blst_hip_synth_edge_cases(void *points96, blst_byte *scalars32, size_t n, uint64_t seed);
+void blst_hip_synth_fr(uint64_t *out4, const blst_byte *scalars32, size_t n);
+void blst_hip_synth_jacobian(void *points144, const void *affine96, size_t n, uint64_t seed);
+/* G2 bases [a + i b] G2 (affine, 192 bytes each) for a, b drawn from the seed */
+void blst_hip_synth_p2_points(void *points192, size_t n, uint64_t seed);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BLST_HIP_TESTKIT_H */

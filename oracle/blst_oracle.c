@@ -855,6 +855,48 @@ void or_msm_pippenger_mt(u64 ret[18], const u64 *points, const uint8_t *scalars,
     free(jobs);
 }
 
+/* Many independent MSMs (or_msm_pippenger each) handed out to threads, the
+ * largest first: the reference library is reentrant and a batch of MSMs is
+ * independent calls (README.md:39-40), so this is the checker for batched
+ * GPU launches.  rets: njobs x 18 limbs. */
+typedef struct {
+    u64 *rets;
+    const u64 *const *points;
+    const uint8_t *const *scalars;
+    const size_t *ns, *nbits, *order;
+    size_t njobs;
+    size_t next; /* shared job counter */
+} batch_job_t;
+
+static void *batch_worker(void *arg) {
+    batch_job_t *b = (batch_job_t *)arg;
+    for (;;) {
+        size_t k = __atomic_fetch_add(&b->next, 1, __ATOMIC_RELAXED);
+        if (k >= b->njobs) return NULL;
+        size_t j = b->order[k];
+        or_msm_pippenger(b->rets + 18 * j, b->points[j], b->scalars[j], b->ns[j], b->nbits[j], 0);
+    }
+}
+
+void or_msm_batch_mt(u64 *rets, const u64 *const *points, const uint8_t *const *scalars,
+                     const size_t *ns, const size_t *nbits, size_t njobs, int threads) {
+    if (threads < 1) threads = 1;
+    size_t *order = calloc(njobs ? njobs : 1, sizeof(size_t));
+    for (size_t j = 0; j < njobs; j++) order[j] = j;
+    /* largest first (insertion sort on a copy of the sizes is enough here) */
+    for (size_t a = 1; a < njobs; a++) {
+        size_t v = order[a], b = a;
+        while (b > 0 && ns[order[b - 1]] < ns[v]) { order[b] = order[b - 1]; b--; }
+        order[b] = v;
+    }
+    batch_job_t job = {rets, points, scalars, ns, nbits, order, njobs, 0};
+    pthread_t *th = calloc((size_t)threads, sizeof(pthread_t));
+    for (int t = 0; t < threads; t++) pthread_create(&th[t], NULL, batch_worker, &job);
+    for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+    free(th);
+    free(order);
+}
+
 /* Sum of [k_i]P_i term by term: the "expected" of C_api_tests.cpp:1488-1515
  * and adapter_msm.cpp:19-50. */
 void or_msm_naive(u64 ret[18], const u64 *points, const uint8_t *scalars, size_t npoints,

@@ -72,6 +72,9 @@ void or_msm_pippenger_mt(uint64_t ret[18], const uint64_t *points, const uint8_t
                          size_t npoints, size_t nbits, size_t window, int threads);
 void or_msm_tile(uint64_t ret[18], const uint64_t *points, const uint8_t *scalars,
                  size_t npoints, size_t nbits, size_t bit0, size_t window);
+/* njobs independent MSMs on a pool of threads; rets holds njobs x 18 limbs */
+void or_msm_batch_mt(uint64_t *rets, const uint64_t *const *points, const uint8_t *const *scalars,
+                     const size_t *ns, const size_t *nbits, size_t njobs, int threads);
 void or_msm_naive(uint64_t ret[18], const uint64_t *points, const uint8_t *scalars,
                   size_t npoints, size_t nbits);
 

@@ -51,6 +51,8 @@ def lib():
             "or_msm_pippenger": [_u64p, _u64p, _u8p, _sz, _sz, _sz],
             "or_msm_pippenger_mt": [_u64p, _u64p, _u8p, _sz, _sz, _sz, ctypes.c_int],
             "or_msm_tile": [_u64p, _u64p, _u8p, _sz, _sz, _sz, _sz],
+            "or_msm_batch_mt": [_u64p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, _sz,
+                                ctypes.c_int],
             "or_msm_naive": [_u64p, _u64p, _u8p, _sz, _sz],
             "or_gen_scalars": [_u8p, _sz, ctypes.c_uint64, ctypes.c_int],
             "or_gen_points": [_u64p, _sz, ctypes.c_uint64, ctypes.c_int],
@@ -239,6 +241,27 @@ def msm_pippenger(points, scalars, n, nbits, window=0, threads=1):
     else:
         lib().or_msm_pippenger(u64p(r), u64p(pts), u8p(sc), n, nbits, window)
     return r
+
+
+def msm_batch(jobs, threads=16):
+    """Independent MSMs on a thread pool.  jobs: (points, scalars, n, nbits)
+    with points (>= n, 12) uint64 and scalars (>= n, nbytes) uint8 arrays;
+    returns (njobs, 18) uint64 Jacobian results."""
+    nj = len(jobs)
+    keep = []
+    pp = np.zeros(max(nj, 1), np.uint64)
+    sp = np.zeros(max(nj, 1), np.uint64)
+    ns = np.zeros(max(nj, 1), np.uint64)
+    nb = np.zeros(max(nj, 1), np.uint64)
+    for k, (pts, sc, n, nbits) in enumerate(jobs):
+        P = np.ascontiguousarray(pts, dtype=np.uint64)
+        S = np.ascontiguousarray(sc, dtype=np.uint8)
+        keep += [P, S]
+        pp[k], sp[k], ns[k], nb[k] = P.ctypes.data, S.ctypes.data, n, nbits
+    out = np.zeros((max(nj, 1), 18), np.uint64)
+    lib().or_msm_batch_mt(u64p(out.reshape(-1)), ctypes.c_void_p(pp.ctypes.data), ctypes.c_void_p(sp.ctypes.data),
+                          ctypes.c_void_p(ns.ctypes.data), ctypes.c_void_p(nb.ctypes.data), nj, threads)
+    return out[:nj]
 
 
 def msm_tile(points, scalars, n, nbits, bit0, window):

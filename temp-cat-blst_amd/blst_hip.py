@@ -11,6 +11,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libblst_hip.so")
+# test and workload-generation hooks (include/blst_hip_testkit.h)
+TESTKIT_PATH = os.path.join(HERE, "libblst_hip_testkit.so")
 
 # symbol -> (argtypes, restype); the full exported surface of include/blst_hip.h
 _vp = ctypes.c_void_p
@@ -45,8 +47,6 @@ SIGNATURES = {
     "blst_hip_last_error": ([], ctypes.c_char_p),
     "blst_hip_available": ([], ctypes.c_int),
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
-    "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
-    "blst_hip_point_ops_check": ([_vp, _sz], ctypes.c_int),
     "blst_hip_set_pipeline": ([_sz], ctypes.c_int),
     "blst_hip_last_phase_ms": ([_vp], None),
     "blst_hip_device_upload": ([_vp, _sz], _vp),
@@ -54,13 +54,6 @@ SIGNATURES = {
     "blst_hip_device_write": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_device_read": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_device_alloc": ([_sz], _vp),
-    "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
-    "blst_hip_synth_scalars": ([_vp, _sz, ctypes.c_uint64, ctypes.c_int], None),
-    "blst_hip_synth_srs_scalars": ([_vp, _sz, ctypes.c_uint64], None),
-    "blst_hip_synth_edge_cases": ([_vp, _vp, _sz, ctypes.c_uint64], None),
-    "blst_hip_synth_fr": ([_vp, _vp, _sz], None),
-    "blst_hip_synth_jacobian": ([_vp, _vp, _sz, ctypes.c_uint64], None),
-    "blst_hip_synth_p2_points": ([_vp, _sz, ctypes.c_uint64], None),
     "blst_hip_p1_compress": ([_vp, _vp], None),
     "blst_hip_msm_batch_device": ([_vp, _vp, _sz], ctypes.c_int),
     "blst_hip_batch_plan_create": ([_vp, _sz], _vp),
@@ -79,22 +72,50 @@ SIGNATURES = {
     "blst_hip_tuning_get": ([ctypes.c_int, _sz, _sz, _vp], None),
 }
 
+TESTKIT_SIGNATURES = {
+    "blst_hip_fp_ops": ([ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _sz], ctypes.c_int),
+    "blst_hip_set_alloc_cap": ([_sz], None),
+    "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
+    "blst_hip_point_ops_check": ([_vp, _sz], ctypes.c_int),
+    "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
+    "blst_hip_synth_scalars": ([_vp, _sz, ctypes.c_uint64, ctypes.c_int], None),
+    "blst_hip_synth_srs_scalars": ([_vp, _sz, ctypes.c_uint64], None),
+    "blst_hip_synth_edge_cases": ([_vp, _vp, _sz, ctypes.c_uint64], None),
+    "blst_hip_synth_fr": ([_vp, _vp, _sz], None),
+    "blst_hip_synth_jacobian": ([_vp, _vp, _sz, ctypes.c_uint64], None),
+    "blst_hip_synth_p2_points": ([_vp, _sz, ctypes.c_uint64], None),
+}
+
 _lib = None
+_kit = None
+
+
+def _bind(path, sigs, what):
+    if not os.path.exists(path):
+        raise RuntimeError("%s is not built (run __graft_entry__.build())" % what)
+    L = ctypes.CDLL(path)
+    for name, (args, res) in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    return L
 
 
 def load():
     """Load the library (no GPU call is made here)."""
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError("libblst_hip.so is not built (run __graft_entry__.build())")
-        L = ctypes.CDLL(LIB_PATH)
-        for name, (args, res) in SIGNATURES.items():
-            f = getattr(L, name)
-            f.argtypes = args
-            f.restype = res
-        _lib = L
+        _lib = _bind(LIB_PATH, SIGNATURES, "libblst_hip.so")
     return _lib
+
+
+def testkit():
+    """The test and workload-generation hooks (loads the product first)."""
+    global _kit
+    if _kit is None:
+        load()
+        _kit = _bind(TESTKIT_PATH, TESTKIT_SIGNATURES, "libblst_hip_testkit.so")
+    return _kit
 
 
 def _ptr(a):
@@ -371,8 +392,34 @@ def p1s_tile_pippenger(points, scalars, nbits, bit0, window):
     return ret
 
 
+FP_OPS = {"mul": 0, "sqr": 1, "mul_sum": 2, "add": 3, "sub": 4, "dbl": 5, "neg": 6, "cneg": 7, "canon": 8,
+          "is_zero": 9, "inv": 10}
+
+
+def fp_ops(op, a, b=None, c=None, d=None):
+    """One device field operation over arrays of (n, 12) uint32 limbs (or
+    (n, 6) uint64): returns (n, 12) uint32."""
+    A = np.ascontiguousarray(a).view(np.uint32).reshape(-1, 12)
+    n = A.shape[0]
+    B = np.ascontiguousarray(b).view(np.uint32).reshape(-1, 12) if b is not None else A
+    C = np.ascontiguousarray(c).view(np.uint32).reshape(-1, 12) if c is not None else A
+    D = np.ascontiguousarray(d).view(np.uint32).reshape(-1, 12) if d is not None else A
+    for X in (B, C, D):
+        assert X.shape[0] == n
+    out = np.zeros((n, 12), np.uint32)
+    rc = testkit().blst_hip_fp_ops(FP_OPS[op], _ptr(out), _ptr(A), _ptr(B), _ptr(C), _ptr(D), n)
+    if rc:
+        raise RuntimeError("blst_hip_fp_ops: " + last_error())
+    return out
+
+
+def set_alloc_cap(nbytes):
+    """Device allocations of the product above nbytes fail (0: no cap)."""
+    testkit().blst_hip_set_alloc_cap(nbytes)
+
+
 def booth_digits(scalars, nbits, bit0, window):
-    L = load()
+    L = testkit()
     sc = np.ascontiguousarray(scalars, dtype=np.uint8)
     out = np.zeros(sc.shape[0], np.int32)
     rc = L.blst_hip_booth_digits(_ptr(out), _ptr(sc), sc.shape[0], nbits, bit0, window)
@@ -390,7 +437,7 @@ def set_pipeline(groups):
 def point_ops_check(d_points_ptr, n):
     """Mismatching coordinates between the four-lane and one-lane point
     formulas on n device-resident affine points."""
-    rc = load().blst_hip_point_ops_check(d_points_ptr, n)
+    rc = testkit().blst_hip_point_ops_check(d_points_ptr, n)
     if rc < 0:
         raise RuntimeError("blst_hip_point_ops_check: " + last_error())
     return rc
@@ -419,7 +466,7 @@ def generator_multiples(scalars, nbits=255):
     """[k_i]G on the device, returned as (n, 12) uint64 affine limbs."""
     sc = np.ascontiguousarray(scalars, dtype=np.uint8)
     out = np.zeros((sc.shape[0], 12), np.uint64)
-    rc = load().blst_hip_p1s_mult_generator(_ptr(out), _ptr(sc), sc.shape[0], nbits)
+    rc = testkit().blst_hip_p1s_mult_generator(_ptr(out), _ptr(sc), sc.shape[0], nbits)
     if rc:
         raise RuntimeError("blst_hip_p1s_mult_generator: " + last_error())
     return out
@@ -427,26 +474,26 @@ def generator_multiples(scalars, nbits=255):
 
 def synth_scalars(n, seed, low64_pct=0):
     out = np.zeros((n, 32), np.uint8)
-    load().blst_hip_synth_scalars(_ptr(out), n, seed, low64_pct)
+    testkit().blst_hip_synth_scalars(_ptr(out), n, seed, low64_pct)
     return out
 
 
 def synth_srs_scalars(n, seed):
     out = np.zeros((n, 32), np.uint8)
-    load().blst_hip_synth_srs_scalars(_ptr(out), n, seed)
+    testkit().blst_hip_synth_srs_scalars(_ptr(out), n, seed)
     return out
 
 
 def synth_edge_cases(points, scalars, seed):
     assert points.flags.c_contiguous and scalars.flags.c_contiguous
-    load().blst_hip_synth_edge_cases(_ptr(points), _ptr(scalars), points.shape[0], seed)
+    testkit().blst_hip_synth_edge_cases(_ptr(points), _ptr(scalars), points.shape[0], seed)
 
 
 def synth_fr(scalars):
     """Montgomery-form blst_fr (n x 4 uint64) of canonical scalars below r."""
     sc = np.ascontiguousarray(scalars, dtype=np.uint8)
     out = np.zeros((sc.shape[0], 4), np.uint64)
-    load().blst_hip_synth_fr(_ptr(out), _ptr(sc), sc.shape[0])
+    testkit().blst_hip_synth_fr(_ptr(out), _ptr(sc), sc.shape[0])
     return out
 
 
@@ -454,14 +501,14 @@ def synth_jacobian(affine, seed):
     """Jacobian representatives (n x 18 uint64) of affine points at seeded Z."""
     a = np.ascontiguousarray(affine, dtype=np.uint64)
     out = np.zeros((a.shape[0], 18), np.uint64)
-    load().blst_hip_synth_jacobian(_ptr(out), _ptr(a), a.shape[0], seed)
+    testkit().blst_hip_synth_jacobian(_ptr(out), _ptr(a), a.shape[0], seed)
     return out
 
 
 def synth_p2_points(n, seed):
     """n G2 bases (n x 24 uint64 affine) in arithmetic progression [a + i b]G2."""
     out = np.zeros((n, 24), np.uint64)
-    load().blst_hip_synth_p2_points(_ptr(out), n, seed)
+    testkit().blst_hip_synth_p2_points(_ptr(out), n, seed)
     return out
 
 

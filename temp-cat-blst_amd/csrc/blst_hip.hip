@@ -33,6 +33,21 @@ namespace host {
 
 thread_local std::string g_err;
 thread_local double g_phase[7];
+std::atomic<size_t> g_alloc_cap{0};
+
+// the message of the exception being handled (called inside a catch block):
+// no exception crosses the C ABI, every one becomes blst_hip_last_error()
+std::string current_error() {
+    try {
+        throw;
+    } catch (const HipFail& f) {
+        return f.msg;
+    } catch (const std::exception& e) {
+        return e.what();
+    } catch (...) {
+        return "unknown error";
+    }
+}
 
 // Parallel host memcpy, for staging pageable caller buffers into page-locked
 // memory: copy() splits the bytes over the pool's threads and the caller.
@@ -189,14 +204,12 @@ struct Ctx {
     std::vector<std::unique_ptr<Workspace>> pipe;   // window-group pipelines
     hipEvent_t ready = nullptr;                     // inputs written on stream
     std::vector<hipEvent_t> range_ready;            // per point range of a split host MSM
-    DevBuf pts, scl, dbg;
-    DevBuf gtable, gen_tmp, gen_out;     // comb table of G, fixed-base scratch
+    DevBuf pts, scl;
     DevBuf jac_in, aff_out;              // batch normalisation staging
     DevBuf wb_pts, wb_rows, wb_table, wb_list, wb_scl, wb_part, wb_W;   // fixed-base tables
     DevBuf comb_q, comb_x;               // comb-table build: shifted bases
     void* wb_host = nullptr;             // page-locked gather staging
     size_t wb_host_cap = 0;
-    bool gtable_ready = false;
     Stager stager;                       // uploads of pageable caller buffers
     Stager down;                         // downloads that run beside uploads
     hipStream_t d2h = nullptr;           // their copies (created on first use)
@@ -631,31 +644,38 @@ bool tiled_sort(const Layout& L) {
     return (size_t)L.max_B * 4 <= limit;
 }
 
-void ensure_workspace(Workspace& cx, const Layout& L) {
+// Every device buffer of a batch's workspace with the bytes the layout needs
+// (f(buffer, bytes)); the layout-table buffer is sized by upload_layout.
+template <class Fn>
+void for_each_buffer(Workspace& cx, const Layout& L, Fn&& f) {
     const uint32_t nkeys = L.nkeys, nent = L.total_entries;
     const uint32_t nslots = 2 * std::max<uint32_t>(div_up(nent, L.T), 1);
     if (L.binned) {
-        cx.counts.ensure((L.count_words + 1) * 4 + 16);
-        cx.rank.ensure((L.count_words + 1) * 4 + 16);   // scanned bin counts
-        cx.ent.ensure((size_t)nent * 8 + 16);           // entries in bin order
+        f(cx.counts, (L.count_words + 1) * 4 + 16);
+        f(cx.rank, (L.count_words + 1) * 4 + 16);   // scanned bin counts
+        f(cx.ent, (size_t)nent * 8 + 16);           // entries in bin order
     } else if (tiled_sort(L)) {
-        cx.counts.ensure(L.count_words * 4 + 16);
+        f(cx.counts, L.count_words * 4 + 16);
     } else {
-        cx.ent.ensure((size_t)nent * 4 + 16);
-        cx.rank.ensure((size_t)nent * 4 + 16);
+        f(cx.ent, (size_t)nent * 4 + 16);
+        f(cx.rank, (size_t)nent * 4 + 16);
     }
-    cx.sorted.ensure((size_t)nent * 8 + 16);
-    cx.hist.ensure(((size_t)nkeys + 1) * 4);
-    cx.off.ensure(((size_t)nkeys + 1) * 4);
-    cx.scan_tmp.ensure(std::max(scan_tmp_words(nkeys + 1), scan_tmp_words((uint32_t)L.count_words + 1)) * 4);
+    f(cx.sorted, (size_t)nent * 8 + 16);
+    f(cx.hist, ((size_t)nkeys + 1) * 4);
+    f(cx.off, ((size_t)nkeys + 1) * 4);
+    f(cx.scan_tmp, std::max(scan_tmp_words(nkeys + 1), scan_tmp_words((uint32_t)L.count_words + 1)) * 4);
     const size_t pb = L.point_bytes();
-    cx.buckets.ensure((size_t)nkeys * pb);
-    cx.slot_key.ensure((size_t)nslots * 4);
-    cx.slots.ensure((size_t)nslots * pb);
-    cx.heavy.ensure(((size_t)nslots + 64) * 4);
-    cx.H.ensure(L.tree_nodes * 2 * pb);
-    cx.W.ensure((size_t)L.groups.size() * pb);
-    cx.res.ensure((size_t)L.jobs.size() * pb);
+    f(cx.buckets, (size_t)nkeys * pb);
+    f(cx.slot_key, (size_t)nslots * 4);
+    f(cx.slots, (size_t)nslots * pb);
+    f(cx.heavy, ((size_t)nslots + 64) * 4);
+    f(cx.H, L.tree_nodes * 2 * pb);
+    f(cx.W, (size_t)L.groups.size() * pb);
+    f(cx.res, (size_t)L.jobs.size() * pb);
+}
+
+void ensure_workspace(Workspace& cx, const Layout& L) {
+    for_each_buffer(cx, L, [](DevBuf& b, size_t bytes) { b.ensure(bytes); });
 }
 
 // Device bytes a batch of this layout needs beyond its inputs.
@@ -1063,12 +1083,8 @@ int host_msm(blst_p1* ret, const blst_p1_affine* const points[], size_t npoints,
         const uint8_t* hp = resolve(points, npoints, sizeof(blst_p1_affine), sp);
         const uint8_t* hs = resolve(scalars, npoints, nbytes, ss);
         host_buffers_msm(cx, hp, hs, npoints, nbits, window, false, fold, bit0, ret);
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        std::memset(ret, 0, sizeof(*ret));
-        return -1;
-    } catch (const std::exception& e) {
-        g_err = e.what();
+    } catch (...) {
+        g_err = current_error();
         std::memset(ret, 0, sizeof(*ret));
         return -1;
     }
@@ -1187,47 +1203,13 @@ int host_msm_g2(blst_p2* ret, const blst_p2_affine* const points[], size_t npoin
             js.tile = !fold;
             device_msm_g2(cx, js, ret);
         }
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        std::memset(ret, 0, sizeof(*ret));
-        return -1;
-    } catch (const std::exception& e) {
-        g_err = e.what();
+    } catch (...) {
+        g_err = current_error();
         std::memset(ret, 0, sizeof(*ret));
         return -1;
     }
     g_err.clear();
     return 0;
-}
-
-// Comb table T[m][j-1] = [j * 2^(8m)] G, m < 32, j < 256, affine, on the
-// device; built once per context on the host (8160 additions) with one
-// batched inversion.
-void ensure_gtable(Ctx& cx) {
-    if (cx.gtable_ready) return;
-    static const uint64_t GX[6] = {0x5cb38790fd530c16ULL, 0x7817fc679976fff5ULL, 0x154f95c7143ba1c1ULL,
-                                   0xf0ae6acdf3d0e747ULL, 0xedce6ecc21dbf440ULL, 0x120177419e0bfb75ULL};
-    static const uint64_t GY[6] = {0xbaac93d50ce72271ULL, 0x8c22631a7918fd8eULL, 0xdd595f13570725ceULL,
-                                   0x51ac582950405194ULL, 0x0e1c8c3fad0059c0ULL, 0x0bbc3efc5008a26aULL};
-    hostec::xyzz6 base;
-    std::memcpy(base.x.l, GX, 48);
-    std::memcpy(base.y.l, GY, 48);
-    base.zz = hostec::one();
-    base.zzz = hostec::one();
-    std::vector<hostec::xyzz6> T(32 * 255);
-    for (int m = 0; m < 32; m++) {
-        hostec::xyzz6 acc = base;
-        for (int j = 1; j < 256; j++) {
-            if (j > 1) hostec::addp(acc, acc, base);
-            T[m * 255 + (j - 1)] = acc;
-        }
-        for (int k = 0; k < 8; k++) hostec::dbl(base, base);
-    }
-    std::vector<hostec::aff6> A(T.size());
-    hostec::batch_to_affine(A.data(), T.data(), T.size());
-    cx.gtable.ensure(A.size() * sizeof(hostec::aff6));
-    HIPCK(hipMemcpy(cx.gtable.p, A.data(), A.size() * sizeof(hostec::aff6), hipMemcpyHostToDevice));
-    cx.gtable_ready = true;
 }
 
 bool heads_ok(const void* const* xs, const void* const* ys, size_t n) {
@@ -1540,11 +1522,8 @@ bool fr_msm(blst_p1* ret, const blst_p1_affine points[], const void* fr_scalars,
         host_buffers_msm(cx, reinterpret_cast<const uint8_t*>(points), static_cast<const uint8_t*>(fr_scalars), n,
                          255, 0, true, true, 0, &r);
         *ret = r;
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        return false;
-    } catch (const std::exception& e) {
-        g_err = e.what();
+    } catch (...) {
+        g_err = current_error();
         return false;
     }
     g_err.clear();
@@ -1721,12 +1700,8 @@ int blst_hip_p2_msm_device(blst_p2* ret, const void* device_points, const void* 
         std::lock_guard<std::mutex> lk(cx.mu);
         JobSpec js = make_spec(device_points, device_scalars, npoints, nbits, window, false);
         device_msm_g2(cx, js, ret);
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        std::memset(ret, 0, sizeof(*ret));
-        return -1;
-    } catch (const std::exception& e) {
-        g_err = e.what();
+    } catch (...) {
+        g_err = current_error();
         std::memset(ret, 0, sizeof(*ret));
         return -1;
     }
@@ -1772,8 +1747,8 @@ int blst_hip_msm_device_tuned(blst_p1* ret, const void* device_points, const voi
         const uint32_t K = chunk ? 1 : point_ranges(npoints, kDeviceSplitMin, kDeviceRangePoints);
         if (K > 1) ranged_msm(cx, js, (nbits + 7) / 8, K, [](uint32_t, size_t, size_t) {}, ret);
         else device_msm(cx, js, ret);
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         std::memset(ret, 0, sizeof(*ret));
         return -1;
     }
@@ -1804,8 +1779,8 @@ int blst_hip_msm_batch_device(blst_p1* rets, const blst_hip_msm_job* jobs, size_
             std::memcpy(&rets[idx[k]], &jj, sizeof(jj));
         }
         g_phase[5] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         std::memset(rets, 0, njobs * sizeof(blst_p1));
         return -1;
     }
@@ -1835,8 +1810,8 @@ blst_hip_batch_plan* blst_hip_batch_plan_create(const blst_hip_msm_job* jobs, si
             HIPCK(hipStreamEndCapture(P.ws.stream, &P.graph));
             HIPCK(hipGraphInstantiate(&P.exec, P.graph, nullptr, nullptr, 0));
         }
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         return nullptr;
     }
     g_err.clear();
@@ -1857,8 +1832,8 @@ int blst_hip_batch_plan_run(blst_hip_batch_plan* plan, blst_p1* rets) {
             hostec::to_jac(jj, P.host_res[k]);
             std::memcpy(&rets[P.idx[k]], &jj, sizeof(jj));
         }
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         return -1;
     }
     g_phase[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1867,59 +1842,6 @@ int blst_hip_batch_plan_run(blst_hip_batch_plan* plan, blst_p1* rets) {
 }
 
 void blst_hip_batch_plan_destroy(blst_hip_batch_plan* plan) { delete plan; }
-
-int blst_hip_point_ops_check(const void* device_points, size_t npoints) {
-    if (npoints == 0) return 0;
-    if (!device_points || npoints > (1u << 24) || ((uintptr_t)device_points & 15)) {
-        g_err = "invalid arguments";
-        return -1;
-    }
-    try {
-        Ctx& cx = context();
-        std::lock_guard<std::mutex> lk(cx.mu);
-        cx.dbg.ensure(64);
-        fill_words(cx.ws, cx.dbg.p, 1, 0);
-        launch_k_quad_check(dim3(div_up(4 * npoints, 256)), dim3(256), cx.stream,
-                            static_cast<const uint4*>(device_points), (uint32_t)npoints, cx.dbg.as<uint32_t>());
-        HIPCK(hipGetLastError());
-        uint32_t bad = 0;
-        HIPCK(hipMemcpyAsync(&bad, cx.dbg.p, 4, hipMemcpyDeviceToHost, cx.stream));
-        HIPCK(hipStreamSynchronize(cx.stream));
-        g_err.clear();
-        return (int)bad;
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        return -1;
-    }
-}
-
-int blst_hip_booth_digits(int32_t* out, const blst_byte* scalars, size_t npoints, size_t nbits,
-                          size_t bit0, size_t window) {
-    if (npoints == 0) return 0;
-    if (!out || !scalars || nbits == 0 || nbits > 256 || window < 1 || window > 24 || bit0 > nbits) {
-        g_err = "invalid arguments";
-        return -1;
-    }
-    try {
-        Ctx& cx = context();
-        std::lock_guard<std::mutex> lk(cx.mu);
-        const size_t nbytes = (nbits + 7) / 8;
-        cx.scl.ensure(npoints * nbytes + 16);
-        cx.dbg.ensure(npoints * 4);
-        HIPCK(hipMemcpyAsync(cx.scl.p, scalars, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
-        launch_k_digits_debug(dim3(div_up(npoints, 256)), dim3(256), cx.stream,
-                           cx.scl.as<const uint8_t>(), (uint32_t)npoints, (uint32_t)nbytes, (uint32_t)nbits,
-                           (uint32_t)bit0, (uint32_t)window, cx.dbg.as<int32_t>());
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(out, cx.dbg.p, npoints * 4, hipMemcpyDeviceToHost, cx.stream));
-        HIPCK(hipStreamSynchronize(cx.stream));
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        return -1;
-    }
-    g_err.clear();
-    return 0;
-}
 
 void* blst_hip_device_upload(const void* host, size_t bytes) {
     void* d = nullptr;
@@ -2024,10 +1946,8 @@ void host_to_affine(Ctx& cx, blst_p1_affine* dst, const uint8_t* hp, size_t n) {
                 HIPCK(hipStreamWaitEvent(cx.d2h, cx.range_ready[2 * r + 1], 0));
                 cx.down.download(dst + lo, out + lo * sizeof(blst_p1_affine), m * sizeof(blst_p1_affine), cx.d2h);
             }
-        } catch (const HipFail& f) {
-            down_err = f.msg;
-        } catch (const std::exception& e) {
-            down_err = e.what();
+        } catch (...) {
+            down_err = current_error();
         }
     });
     try {
@@ -2083,12 +2003,8 @@ void blst_p1s_to_affine(blst_p1_affine dst[], const blst_p1* const points[], siz
             enqueue_to_affine(cx.stream, cx.jac_in.p, cx.aff_out.p, m);
             cx.stager.download(dst + b, cx.aff_out.p, m * sizeof(blst_p1_affine), cx.stream);
         }
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        std::memset(dst, 0, npoints * sizeof(blst_p1_affine));
-        return;
-    } catch (const std::exception& e) {
-        g_err = e.what();
+    } catch (...) {
+        g_err = current_error();
         std::memset(dst, 0, npoints * sizeof(blst_p1_affine));
         return;
     }
@@ -2107,8 +2023,8 @@ int blst_hip_p1s_to_affine_device(void* device_dst, const void* device_points, s
         std::lock_guard<std::mutex> lk(cx.mu);
         enqueue_to_affine(cx.stream, device_points, device_dst, npoints);
         HIPCK(hipStreamSynchronize(cx.stream));
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         return -1;
     }
     g_err.clear();
@@ -2157,11 +2073,8 @@ void blst_p1s_mult_wbits_precompute(blst_p1_affine table[], size_t wbits, const 
             enqueue_wbits_table(cx, rows, nr, wbits, cx.wb_table.p);
             cx.stager.download(table + row0 * m, cx.wb_table.p, nr * m * sizeof(blst_p1_affine), cx.stream);
         }
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        return;
-    } catch (const std::exception& e) {
-        g_err = e.what();
+    } catch (...) {
+        g_err = current_error();
         return;
     }
     g_err.clear();
@@ -2187,12 +2100,8 @@ void blst_p1s_mult_wbits(blst_p1* ret, const blst_p1_affine table[], size_t wbit
         std::vector<uint8_t> stage;
         const uint8_t* hs = resolve(scalars, npoints, (nbits + 7) / 8, stage);
         host_mult_wbits(cx, ret, table, wbits, npoints, hs, nbits);
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        std::memset(ret, 0, sizeof(*ret));
-        return;
-    } catch (const std::exception& e) {
-        g_err = e.what();
+    } catch (...) {
+        g_err = current_error();
         std::memset(ret, 0, sizeof(*ret));
         return;
     }
@@ -2218,8 +2127,8 @@ int blst_hip_p1s_mult_wbits_precompute_device(void* device_table, size_t wbits, 
                                 wbits, static_cast<uint8_t*>(device_table) + row0 * m * sizeof(blst_p1_affine));
         }
         HIPCK(hipStreamSynchronize(cx.stream));
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         return -1;
     }
     g_err.clear();
@@ -2246,8 +2155,8 @@ int blst_hip_p1s_mult_wbits_device(blst_p1* ret, const void* device_table, size_
         hostec::jac6 j;
         hostec::to_jac(j, r);
         std::memcpy(ret, &j, sizeof(j));
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         std::memset(ret, 0, sizeof(*ret));
         return -1;
     }
@@ -2294,8 +2203,8 @@ int blst_hip_comb_table_build(void* device_table, size_t wbits, size_t nbits, co
             }
         }
         HIPCK(hipStreamSynchronize(cx.stream));
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         return -1;
     }
     g_err.clear();
@@ -2341,46 +2250,8 @@ int blst_hip_comb_msm_batch(blst_p1* rets, const void* device_table, size_t wbit
             hostec::to_jac(j, res[k]);
             std::memcpy(&rets[k], &j, sizeof(j));
         }
-    } catch (const HipFail& f) {
-        g_err = f.msg;
-        return -1;
-    } catch (const std::exception& e) {
-        g_err = e.what();
-        return -1;
-    }
-    g_err.clear();
-    return 0;
-}
-
-int blst_hip_p1s_mult_generator(blst_p1_affine* out, const blst_byte* scalars, size_t npoints,
-                                size_t nbits) {
-    if (npoints == 0) { g_err.clear(); return 0; }
-    if (!out || !scalars || nbits == 0 || nbits > 256 || npoints >= 0x7fffffffULL) {
-        g_err = "invalid arguments";
-        return -1;
-    }
-    try {
-        Ctx& cx = context();
-        std::lock_guard<std::mutex> lk(cx.mu);
-        ensure_gtable(cx);
-        const size_t nbytes = (nbits + 7) / 8;
-        // points per lane of the conversion: enough lanes to fill the device
-        // (one inversion per lane)
-        const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, npoints / kTableAffineLanes));
-        cx.scl.ensure(npoints * nbytes + 16);
-        cx.gen_tmp.ensure(npoints * sizeof(xyzz));
-        cx.gen_out.ensure(npoints * sizeof(aff));
-        HIPCK(hipMemcpyAsync(cx.scl.p, scalars, npoints * nbytes, hipMemcpyHostToDevice, cx.stream));
-        launch_k_fixed_base(dim3(div_up(npoints, 256)), dim3(256), cx.stream,
-                           cx.scl.as<const uint8_t>(), (uint32_t)npoints, (uint32_t)nbytes, (uint32_t)nbits,
-                           cx.gtable.as<const uint4>(), cx.gen_tmp.as<xyzz>());
-        launch_k_to_affine(dim3(div_up(div_up(npoints, Q), 256)), dim3(256), cx.stream,
-                           cx.gen_tmp.as<const xyzz>(), cx.gen_out.as<aff>(), (uint32_t)npoints, Q);
-        HIPCK(hipGetLastError());
-        HIPCK(hipMemcpyAsync(out, cx.gen_out.p, npoints * sizeof(aff), hipMemcpyDeviceToHost, cx.stream));
-        HIPCK(hipStreamSynchronize(cx.stream));
-    } catch (const HipFail& f) {
-        g_err = f.msg;
+    } catch (...) {
+        g_err = current_error();
         return -1;
     }
     g_err.clear();

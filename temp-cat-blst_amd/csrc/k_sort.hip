@@ -46,15 +46,6 @@ k_digits(const job_t* __restrict__ jobs, const uint32_t* __restrict__ pbase, uin
     }
 }
 
-__global__ void k_digits_debug(const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes,
-                               uint32_t nbits, uint32_t bit0, uint32_t c, int32_t* __restrict__ out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    // the tiled sort's reader (word reads for aligned 32-byte scalars)
-    const bool vec = nbytes == 32 && ((uintptr_t)scalars & 15) == 0;
-    out[i] = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
-}
-
 // Tile = chunk k of the points of group g, from the tile table (one launch
 // per LDS size class).  LDS holds the B bucket counters of the group's
 // window.
@@ -426,10 +417,6 @@ k_scatter(const job_t* __restrict__ jobs, const uint32_t* __restrict__ ebase, ui
 
 void launch_k_digits(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ pbase, uint32_t njobs, uint32_t total_points, uint32_t* __restrict__ ent, uint32_t* __restrict__ rank, uint32_t* __restrict__ hist) {
     hipLaunchKernelGGL(k_digits, grid, block, 0, stream, jobs, pbase, njobs, total_points, ent, rank, hist);
-}
-
-void launch_k_digits_debug(dim3 grid, dim3 block, hipStream_t stream, const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits, uint32_t bit0, uint32_t c, int32_t* __restrict__ out) {
-    hipLaunchKernelGGL(k_digits_debug, grid, block, 0, stream, scalars, n, nbytes, nbits, bit0, c, out);
 }
 
 void launch_k_fill(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restrict__ dst, size_t words, uint32_t value) {

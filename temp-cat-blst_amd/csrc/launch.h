@@ -7,7 +7,6 @@
 namespace bls {
 
 void launch_k_digits(dim3 grid, dim3 block, hipStream_t stream, const job_t* __restrict__ jobs, const uint32_t* __restrict__ pbase, uint32_t njobs, uint32_t total_points, uint32_t* __restrict__ ent, uint32_t* __restrict__ rank, uint32_t* __restrict__ hist);
-void launch_k_digits_debug(dim3 grid, dim3 block, hipStream_t stream, const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits, uint32_t bit0, uint32_t c, int32_t* __restrict__ out);
 void launch_k_fill(dim3 grid, dim3 block, hipStream_t stream, uint32_t* __restrict__ dst, size_t words, uint32_t value);
 void launch_k_copy_words(hipStream_t stream, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words);
 void launch_k_add_words(hipStream_t stream, const uint32_t* __restrict__ src, uint32_t* __restrict__ dst, size_t words,

@@ -5,6 +5,7 @@
 #include "launch.h"
 #include "host_ec.h"
 
+#include <atomic>
 #include <string>
 #include <vector>
 
@@ -25,6 +26,18 @@ struct HipFail {
             throw HipFail{std::string(#expr) + ": " + hipGetErrorString(e_)};         \
     } while (0)
 
+// Device allocations larger than this many bytes fail as a device out of
+// memory would (0: no cap).  Only the test kit sets it, to drive the
+// fail-closed paths of the entry points from the host.
+extern std::atomic<size_t> g_alloc_cap;
+
+inline void device_malloc(void** p, size_t bytes) {
+    const size_t cap = g_alloc_cap.load(std::memory_order_relaxed);
+    if (cap && bytes > cap)
+        throw HipFail{"device allocation of " + std::to_string(bytes) + " bytes refused (allocation cap)"};
+    HIPCK(hipMalloc(p, bytes));
+}
+
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
@@ -40,7 +53,7 @@ struct DevBuf {
         p = nullptr;
         cap = 0;
         const size_t want = std::max<size_t>(std::max<size_t>(bytes + bytes / 8, 2 * old_cap), 4096);
-        HIPCK(hipMalloc(&p, want));
+        device_malloc(&p, want);
         cap = want;
     }
     ~DevBuf() {

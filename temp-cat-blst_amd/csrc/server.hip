@@ -55,7 +55,7 @@ public:
         if (it == free_.end()) {
             const size_t span = std::max(cap, kSlab);
             void* p = nullptr;
-            HIPCK(hipMalloc(&p, span));
+            device_malloc(&p, span);
             blocks_.push_back(p);
             allocated_ += span;
             for (size_t o = 0; o + cap <= span; o += cap) free_.emplace(cap, static_cast<uint8_t*>(p) + o);
@@ -155,6 +155,8 @@ struct Request {
     size_t d_scalars_cap = 0;
     hipEvent_t copied = nullptr;
     bool copy_issued = false;         // set by the copier thread
+    bool copying = false;             // the copier works on it with mu_ released
+    bool failed = false;              // failed while copying: the copier releases it
     bool landed = false;              // copies complete (seen by the scheduler)
     clock_t_::time_point t_submit;
     bool done = false;
@@ -226,14 +228,15 @@ public:
         Base b;
         b.n = n;
         HIPCK(hipSetDevice(device_));
-        HIPCK(hipMalloc(&b.ptr, n * sizeof(blst_p1_affine)));
+        device_malloc(&b.ptr, n * sizeof(blst_p1_affine));
         HIPCK(hipMemcpy(b.ptr, pts, n * sizeof(blst_p1_affine), hipMemcpyHostToDevice));
         used_ += n * sizeof(blst_p1_affine);
         bases_.push_back(b);
         return (int)bases_.size() - 1;
     }
 
-    int64_t submit(int base_id, const blst_p1_affine* pts, const blst_byte* scalars, size_t n, size_t nbits) {
+    int64_t submit(int base_id, const blst_p1_affine* pts, const blst_byte* scalars, size_t n, size_t nbits,
+                   std::string& err) {
         auto r = std::make_shared<Request>();
         r->base_id = base_id;
         r->host_points = pts;
@@ -243,10 +246,23 @@ public:
         r->t_submit = clock_t_::now();
         {
             std::lock_guard<std::mutex> lk(mu_);
-            if (base_id >= 0 && ((size_t)base_id >= bases_.size() || n > bases_[base_id].n)) return -1;
-            r->ticket = ++next_ticket_;
+            if (base_id >= 0 && ((size_t)base_id >= bases_.size() || n > bases_[base_id].n)) {
+                err = "unknown bases, or more points than the bases hold";
+                return -1;
+            }
+            // a batch holds every job's (window, point) entries, keys and
+            // points as 31-bit indices: a request beyond that could never run
+            const uint32_t c = make_spec(nullptr, nullptr, n, nbits, 0, true).c;
+            if ((uint64_t)(1 + nbits / c) * n >= 0x7fffffffULL) {
+                err = "request too large for one launch (31-bit entry indices)";
+                return -1;
+            }
             r->need = footprint(*r);
-            if (r->need > budget_) return -1;
+            if (r->need > budget_) {
+                err = "request larger than the HBM budget";
+                return -1;
+            }
+            r->ticket = ++next_ticket_;
             tickets_[r->ticket] = r;
             pending_.push_back(r);
             submitted_++;
@@ -255,16 +271,19 @@ public:
         return r->ticket;
     }
 
-    int wait(int64_t ticket, blst_p1* ret, double* latency_ms) {
+    int wait(int64_t ticket, blst_p1* ret, double* latency_ms, std::string& err) {
         std::unique_lock<std::mutex> lk(mu_);
         auto it = tickets_.find(ticket);
-        if (it == tickets_.end()) return -1;
+        if (it == tickets_.end()) {
+            err = "unknown ticket";
+            return -1;
+        }
         std::shared_ptr<Request> r = it->second;
         done_cv_.wait(lk, [&] { return r->done; });
         *ret = r->result;
         if (latency_ms) *latency_ms = r->latency_ms;
         tickets_.erase(it);
-        if (r->status) last_error_ = "request failed: " + fail_msg_;
+        if (r->status) err = "request failed: " + fail_msg_;
         return r->status;
     }
 
@@ -277,8 +296,6 @@ public:
         out[4] = peak_used_;
         out[5] = budget_;
     }
-
-    std::string last_error_;
 
 private:
     struct Base {
@@ -330,6 +347,7 @@ private:
             }
             auto r = to_copy_.front();
             to_copy_.pop_front();
+            r->copying = true;
             lk.unlock();
             bool ok = true;
             const size_t sbytes = r->n * ((r->nbits + 7) / 8);
@@ -348,8 +366,21 @@ private:
                                           hipMemcpyHostToDevice, copy_stream_) == hipSuccess;
             ok = ok && hipEventRecord(r->copied, copy_stream_) == hipSuccess;
             lk.lock();
-            if (!ok) copy_failed_ = true;
+            r->copying = false;
             r->copy_issued = true;
+            if (r->failed) {
+                // fail_all ran while this request was being copied and left
+                // it here: its copies finish before its buffers go back to
+                // the pool, and its host buffers are no longer read once
+                // the waiter is released
+                (void)hipStreamSynchronize(copy_stream_);
+                (void)hipGetLastError();
+                release(*r);
+                r->done = true;
+                done_cv_.notify_all();
+            } else if (!ok) {
+                copy_failed_ = true;
+            }
             cv_.notify_all();
         }
     }
@@ -460,11 +491,22 @@ private:
         open_points_ -= r.n;
     }
 
+    // Under mu_: every request not yet completed fails.  Copies and batches
+    // still queued on the device finish first (their buffers then go back to
+    // the pool); a request the copier is working on is marked and released
+    // by the copier once its copies are issued.
     void fail_all(const std::string& msg) {
         fail_msg_ = msg;
+        (void)hipStreamSynchronize(copy_stream_);
+        (void)hipStreamSynchronize(compute_stream_);
+        (void)hipGetLastError();
         auto fail = [&](std::shared_ptr<Request>& r) {
-            release(*r);
             r->status = -1;
+            if (r->copying) {
+                r->failed = true;
+                return;
+            }
+            release(*r);
             r->done = true;
         };
         for (auto& r : pending_) { r->status = -1; r->done = true; }
@@ -527,6 +569,8 @@ private:
                 }
             } catch (const HipFail& f) {
                 fail_all(f.msg);
+            } catch (const std::exception& e) {
+                fail_all(e.what());
             }
         }
     }
@@ -575,20 +619,36 @@ blst_hip_server* blst_hip_server_create(size_t hbm_budget_bytes, size_t max_batc
         return s.release();
     } catch (const bls::host::HipFail& f) {
         bls::host::g_err = f.msg;
-        return nullptr;
+    } catch (const std::exception& e) {
+        bls::host::g_err = e.what();
+    } catch (...) {
+        bls::host::g_err = "server creation failed";
     }
+    return nullptr;
 }
 
-void blst_hip_server_destroy(blst_hip_server* s) { delete s; }
+void blst_hip_server_destroy(blst_hip_server* s) {
+    try {
+        delete s;
+    } catch (...) {
+        bls::host::g_err = "server shutdown failed";
+    }
+}
 
 int blst_hip_server_add_bases(blst_hip_server* s, const blst_p1_affine* points, size_t npoints) {
     if (!s || !points || npoints == 0) { bls::host::g_err = "invalid arguments"; return -1; }
     try {
-        return s->impl->add_bases(points, npoints);
+        const int id = s->impl->add_bases(points, npoints);
+        bls::host::g_err.clear();
+        return id;
     } catch (const bls::host::HipFail& f) {
         bls::host::g_err = f.msg;
-        return -1;
+    } catch (const std::exception& e) {
+        bls::host::g_err = e.what();
+    } catch (...) {
+        bls::host::g_err = "add_bases failed";
     }
+    return -1;
 }
 
 int64_t blst_hip_server_submit(blst_hip_server* s, int base_id, const blst_p1_affine* points,
@@ -598,20 +658,42 @@ int64_t blst_hip_server_submit(blst_hip_server* s, int base_id, const blst_p1_af
         bls::host::g_err = "invalid arguments";
         return -1;
     }
-    const int64_t t = s->impl->submit(base_id, points, scalars, npoints, nbits);
-    if (t < 0) bls::host::g_err = "request rejected (unknown bases or larger than the HBM budget)";
-    return t;
+    try {
+        std::string err;
+        const int64_t t = s->impl->submit(base_id, points, scalars, npoints, nbits, err);
+        if (t < 0) bls::host::g_err = "request rejected: " + err;
+        else bls::host::g_err.clear();
+        return t;
+    } catch (const bls::host::HipFail& f) {
+        bls::host::g_err = f.msg;
+    } catch (const std::exception& e) {
+        bls::host::g_err = e.what();
+    } catch (...) {
+        bls::host::g_err = "submit failed";
+    }
+    return -1;
 }
 
 int blst_hip_server_wait(blst_hip_server* s, int64_t ticket, blst_p1* ret, double* latency_ms) {
     if (!s || !ret) { bls::host::g_err = "invalid arguments"; return -1; }
-    const int rc = s->impl->wait(ticket, ret, latency_ms);
-    if (rc) bls::host::g_err = s->impl->last_error_.empty() ? "unknown ticket" : s->impl->last_error_;
-    return rc;
+    try {
+        std::string err;
+        const int rc = s->impl->wait(ticket, ret, latency_ms, err);
+        if (rc) bls::host::g_err = err;
+        else bls::host::g_err.clear();
+        return rc;
+    } catch (...) {
+        bls::host::g_err = "wait failed";
+    }
+    return -1;
 }
 
 void blst_hip_server_stats(blst_hip_server* s, uint64_t out[6]) {
-    if (s) s->impl->stats(out);
+    try {
+        if (s) s->impl->stats(out);
+    } catch (...) {
+        bls::host::g_err = "stats failed";
+    }
 }
 
 }  // extern "C"

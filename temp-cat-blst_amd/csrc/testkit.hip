@@ -1,0 +1,274 @@
+// Test and workload-generation hooks, built as libblst_hip_testkit.so beside
+// the product library (which it links against) and never part of the drop-in
+// surface:
+//   * blst_hip_fp_ops: one device field operation (fp.hip.h) applied over
+//     arrays, so the [0, 2p) redundant field layer can be compared with exact
+//     big-integer arithmetic on chosen inputs (p, 2p - 1, R mod p, ...);
+//   * blst_hip_booth_digits: the signed window digits as the sort kernels
+//     read them;
+//   * blst_hip_point_ops_check: the four-lane point formulas against the
+//     one-lane ones;
+//   * blst_hip_p1s_mult_generator: [k_i]G on the device, for synthetic bases
+//     (the seeded scalar generators are in synth.cpp).
+#include "pipeline.h"
+#include "ec_quad.hip.h"
+#include "../../include/blst_hip_testkit.h"
+
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+namespace bls {
+
+// field operations of the device layer, selected by op
+enum : uint32_t {
+    kOpMul = 0, kOpSqr, kOpMulSum, kOpAdd, kOpSub, kOpDbl, kOpNeg, kOpCneg, kOpCanon, kOpIsZero, kOpInv,
+    kOpCount
+};
+
+__device__ __forceinline__ void load_fp(fp& r, const uint4* __restrict__ src, uint32_t i) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const uint4 v = src[3 * i + k];
+        r.l[4 * k] = v.x;
+        r.l[4 * k + 1] = v.y;
+        r.l[4 * k + 2] = v.z;
+        r.l[4 * k + 3] = v.w;
+    }
+}
+
+__device__ __forceinline__ void store_fp(uint4* __restrict__ dst, uint32_t i, const fp& r) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) dst[3 * i + k] = make_uint4(r.l[4 * k], r.l[4 * k + 1], r.l[4 * k + 2], r.l[4 * k + 3]);
+}
+
+__global__ void __launch_bounds__(256)
+k_fp_ops(uint32_t op, uint4* __restrict__ out, const uint4* __restrict__ A, const uint4* __restrict__ B,
+         const uint4* __restrict__ C, const uint4* __restrict__ D, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fp a, b, c, d, r;
+    load_fp(a, A, i);
+    load_fp(b, B, i);
+    switch (op) {
+    case kOpMul: fp_mul(r, a, b); break;
+    case kOpSqr: fp_sqr(r, a); break;
+    case kOpMulSum:
+        load_fp(c, C, i);
+        load_fp(d, D, i);
+        fp_mul_sum(r, a, b, c, d);
+        break;
+    case kOpAdd: fp_add(r, a, b); break;
+    case kOpSub: fp_sub(r, a, b); break;
+    case kOpDbl: fp_dbl(r, a); break;
+    case kOpNeg: fp_neg(r, a); break;
+    case kOpCneg: fp_cneg(r, a, (b.l[0] & 1) != 0); break;
+    case kOpCanon: fp_canon(r, a); break;
+    case kOpIsZero:
+        fp_zero(r);
+        r.l[0] = fp_is_zero(a) ? 1u : 0u;
+        break;
+    default: fp_inv(r, a); break;
+    }
+    store_fp(out, i, r);
+}
+
+__global__ void k_digits_debug(const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits,
+                               uint32_t bit0, uint32_t c, int32_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    // the tiled sort's reader (word reads for aligned 32-byte scalars)
+    const bool vec = nbytes == 32 && ((uintptr_t)scalars & 15) == 0;
+    out[i] = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
+}
+
+namespace host {
+namespace {
+
+inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+// device buffers and stream of the hooks, one set per process
+struct Kit {
+    std::mutex mu;
+    hipStream_t stream = nullptr;
+    DevBuf in[4], out, scl, gtable, gen_tmp, gen_out;
+    bool gtable_ready = false;
+    void init() {
+        if (!stream) HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+    }
+};
+
+Kit& kit() {
+    static Kit* k = new Kit();   // never destroyed: the runtime may be gone at exit
+    return *k;
+}
+
+// Comb table T[m][j-1] = [j * 2^(8m)] G, m < 32, j < 256, affine, built once
+// on the host (8160 additions) with one batched inversion.
+void ensure_gtable(Kit& k) {
+    if (k.gtable_ready) return;
+    static const uint64_t GX[6] = {0x5cb38790fd530c16ULL, 0x7817fc679976fff5ULL, 0x154f95c7143ba1c1ULL,
+                                   0xf0ae6acdf3d0e747ULL, 0xedce6ecc21dbf440ULL, 0x120177419e0bfb75ULL};
+    static const uint64_t GY[6] = {0xbaac93d50ce72271ULL, 0x8c22631a7918fd8eULL, 0xdd595f13570725ceULL,
+                                   0x51ac582950405194ULL, 0x0e1c8c3fad0059c0ULL, 0x0bbc3efc5008a26aULL};
+    hostec::xyzz6 base;
+    std::memcpy(base.x.l, GX, 48);
+    std::memcpy(base.y.l, GY, 48);
+    base.zz = hostec::one();
+    base.zzz = hostec::one();
+    std::vector<hostec::xyzz6> T(32 * 255);
+    for (int m = 0; m < 32; m++) {
+        hostec::xyzz6 acc = base;
+        for (int j = 1; j < 256; j++) {
+            if (j > 1) hostec::addp(acc, acc, base);
+            T[m * 255 + (j - 1)] = acc;
+        }
+        for (int s = 0; s < 8; s++) hostec::dbl(base, base);
+    }
+    std::vector<hostec::aff6> A(T.size());
+    hostec::batch_to_affine(A.data(), T.data(), T.size());
+    k.gtable.ensure(A.size() * sizeof(hostec::aff6));
+    HIPCK(hipMemcpy(k.gtable.p, A.data(), A.size() * sizeof(hostec::aff6), hipMemcpyHostToDevice));
+    k.gtable_ready = true;
+}
+
+// Lanes of the XYZZ -> affine conversion: enough to fill the device with one
+// inversion per lane.
+constexpr size_t kGenAffineLanes = (size_t)1 << 17;
+
+}  // namespace
+}  // namespace host
+}  // namespace bls
+
+using namespace bls;
+using namespace bls::host;
+
+extern "C" {
+
+void blst_hip_set_alloc_cap(size_t bytes) { g_alloc_cap.store(bytes); }
+
+int blst_hip_fp_ops(int op, void* out, const void* a, const void* b, const void* c, const void* d, size_t n) {
+    if (n == 0) { g_err.clear(); return 0; }
+    if (op < 0 || op >= (int)kOpCount || !out || !a || !b || n >= (1u << 26) ||
+        (op == (int)kOpMulSum && (!c || !d))) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Kit& k = kit();
+        std::lock_guard<std::mutex> lk(k.mu);
+        k.init();
+        const size_t bytes = n * 48;
+        const void* src[4] = {a, b, c, d};
+        const int nin = op == (int)kOpMulSum ? 4 : 2;
+        for (int j = 0; j < 4; j++) k.in[j].ensure(bytes);
+        for (int j = 0; j < nin; j++)
+            HIPCK(hipMemcpyAsync(k.in[j].p, src[j], bytes, hipMemcpyHostToDevice, k.stream));
+        k.out.ensure(bytes);
+        hipLaunchKernelGGL(k_fp_ops, dim3(div_up(n, 256)), dim3(256), 0, k.stream, (uint32_t)op, k.out.as<uint4>(),
+                           k.in[0].as<const uint4>(), k.in[1].as<const uint4>(), k.in[2].as<const uint4>(),
+                           k.in[3].as<const uint4>(), (uint32_t)n);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(out, k.out.p, bytes, hipMemcpyDeviceToHost, k.stream));
+        HIPCK(hipStreamSynchronize(k.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_point_ops_check(const void* device_points, size_t npoints) {
+    if (npoints == 0) return 0;
+    if (!device_points || npoints > (1u << 24) || ((uintptr_t)device_points & 15)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Kit& k = kit();
+        std::lock_guard<std::mutex> lk(k.mu);
+        k.init();
+        k.out.ensure(64);
+        HIPCK(hipMemsetAsync(k.out.p, 0, 4, k.stream));
+        launch_k_quad_check(dim3(div_up(4 * npoints, 256)), dim3(256), k.stream,
+                            static_cast<const uint4*>(device_points), (uint32_t)npoints, k.out.as<uint32_t>());
+        HIPCK(hipGetLastError());
+        uint32_t bad = 0;
+        HIPCK(hipMemcpyAsync(&bad, k.out.p, 4, hipMemcpyDeviceToHost, k.stream));
+        HIPCK(hipStreamSynchronize(k.stream));
+        g_err.clear();
+        return (int)bad;
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+}
+
+int blst_hip_booth_digits(int32_t* out, const blst_byte* scalars, size_t npoints, size_t nbits, size_t bit0,
+                          size_t window) {
+    if (npoints == 0) return 0;
+    if (!out || !scalars || nbits == 0 || nbits > 256 || window < 1 || window > 24 || bit0 > nbits ||
+        npoints >= 0x7fffffffULL) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Kit& k = kit();
+        std::lock_guard<std::mutex> lk(k.mu);
+        k.init();
+        const size_t nbytes = (nbits + 7) / 8;
+        k.scl.ensure(npoints * nbytes + 16);
+        k.out.ensure(npoints * 4);
+        HIPCK(hipMemcpyAsync(k.scl.p, scalars, npoints * nbytes, hipMemcpyHostToDevice, k.stream));
+        hipLaunchKernelGGL(k_digits_debug, dim3(div_up(npoints, 256)), dim3(256), 0, k.stream,
+                           k.scl.as<const uint8_t>(), (uint32_t)npoints, (uint32_t)nbytes, (uint32_t)nbits,
+                           (uint32_t)bit0, (uint32_t)window, k.out.as<int32_t>());
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(out, k.out.p, npoints * 4, hipMemcpyDeviceToHost, k.stream));
+        HIPCK(hipStreamSynchronize(k.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_p1s_mult_generator(blst_p1_affine* out, const blst_byte* scalars, size_t npoints, size_t nbits) {
+    if (npoints == 0) { g_err.clear(); return 0; }
+    if (!out || !scalars || nbits == 0 || nbits > 256 || npoints >= 0x7fffffffULL) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Kit& k = kit();
+        std::lock_guard<std::mutex> lk(k.mu);
+        k.init();
+        ensure_gtable(k);
+        const size_t nbytes = (nbits + 7) / 8;
+        const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, npoints / kGenAffineLanes));
+        k.scl.ensure(npoints * nbytes + 16);
+        k.gen_tmp.ensure(npoints * sizeof(xyzz));
+        k.gen_out.ensure(npoints * sizeof(aff));
+        HIPCK(hipMemcpyAsync(k.scl.p, scalars, npoints * nbytes, hipMemcpyHostToDevice, k.stream));
+        launch_k_fixed_base(dim3(div_up(npoints, 256)), dim3(256), k.stream, k.scl.as<const uint8_t>(),
+                            (uint32_t)npoints, (uint32_t)nbytes, (uint32_t)nbits, k.gtable.as<const uint4>(),
+                            k.gen_tmp.as<xyzz>());
+        launch_k_to_affine(dim3(div_up(div_up(npoints, Q), 256)), dim3(256), k.stream, k.gen_tmp.as<const xyzz>(),
+                           k.gen_out.as<aff>(), (uint32_t)npoints, Q);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(out, k.gen_out.p, npoints * sizeof(aff), hipMemcpyDeviceToHost, k.stream));
+        HIPCK(hipStreamSynchronize(k.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+}  // extern "C"

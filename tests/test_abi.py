@@ -12,8 +12,8 @@ import pyref
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    text = open(os.path.join(ROOT, "include", "blst_hip.h")).read()
+def declared_symbols(header="blst_hip.h"):
+    text = open(os.path.join(ROOT, "include", header)).read()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(blst_\w+)\s*\(", text)))
 
@@ -34,6 +34,21 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
     # and the Python binding binds exactly that surface
     assert sorted(blst_hip.SIGNATURES) == declared_symbols()
+
+
+def test_testkit_exports_its_hooks_and_the_product_does_not():
+    """The test and workload-generation hooks live in their own library: the
+    product exports none of them."""
+    import ctypes
+    import blst_hip
+    kit = declared_symbols("blst_hip_testkit.h")
+    assert "blst_hip_fp_ops" in kit and "blst_hip_synth_scalars" in kit
+    K = ctypes.CDLL(blst_hip.TESTKIT_PATH)
+    assert not [s for s in kit if not hasattr(K, s)]
+    assert sorted(blst_hip.TESTKIT_SIGNATURES) == kit
+    assert not set(kit) & set(declared_symbols())
+    L = ctypes.CDLL(blst_hip.LIB_PATH)
+    assert not [s for s in kit if hasattr(L, s)]
 
 
 def test_scratch_sizes_match_reference(orc):

@@ -121,6 +121,21 @@ def test_msm_against_textbook(orc, n, nbits):
     assert orc.compress(orc.msm_pippenger(A, S, n, nbits, threads=3)) == want
 
 
+def test_msm_batch_checker_against_textbook(orc):
+    """The job-parallel checker used for batched GPU launches: every job of a
+    ragged batch (empty, one point, short and long scalars) equals the
+    textbook sum, in any job order and thread count."""
+    specs = [(12, 255), (0, 255), (1, 256), (5, 1), (9, 64), (7, 17), (11, 255), (3, 8)]
+    jobs, want = [], []
+    for k, (n, nbits) in enumerate(specs):
+        pts, ss, A, S = _pts_and_scalars(max(n, 1), nbits, 7000 + k)
+        jobs.append((A, S, n, nbits))
+        want.append(pyref.compress(pyref.msm(pts[:n], ss[:n])))
+    for threads in (1, 3, 16):
+        got = orc.msm_batch(jobs, threads=threads)
+        assert [orc.compress(g) for g in got] == want, threads
+
+
 def test_msm_closed_form_1G_plus_2_2G(orc):
     # 1*G + 2*(2G) = 5G (C_api_tests.cpp:1488-1515 pattern)
     pts = [pyref.G, pyref.mul(2, pyref.G)]
