@@ -47,6 +47,7 @@ SIGNATURES = {
     "blst_hip_last_error": ([], ctypes.c_char_p),
     "blst_hip_available": ([], ctypes.c_int),
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
+    "blst_hip_msm_device_noalloc": ([_vp, _vp, _vp, _sz, _sz], ctypes.c_int),
     "blst_hip_set_pipeline": ([_sz], ctypes.c_int),
     "blst_hip_last_phase_ms": ([_vp], None),
     "blst_hip_device_upload": ([_vp, _sz], _vp),

@@ -678,6 +678,30 @@ void ensure_workspace(Workspace& cx, const Layout& L) {
     for_each_buffer(cx, L, [](DevBuf& b, size_t bytes) { b.ensure(bytes); });
 }
 
+inline size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t carved_bytes(const Layout& L) {
+    Workspace probe;   // only its buffer members are named; nothing is created
+    size_t off = 0;
+    for_each_buffer(probe, L, [&](DevBuf&, size_t bytes) { off = align256(off) + bytes; });
+    return align256(off) + layout_table_bytes(L);
+}
+
+void carve_workspace(Workspace& cx, const Layout& L, uint8_t* base, size_t cap) {
+    const size_t need = carved_bytes(L);
+    if (need > cap)
+        throw HipFail{"MSM needs " + std::to_string(need) + " bytes of scratch, the fixed region holds " +
+                      std::to_string(cap)};
+    size_t off = 0;
+    for_each_buffer(cx, L, [&](DevBuf& b, size_t bytes) {
+        off = align256(off);
+        b.attach(base + off, bytes);
+        off += bytes;
+    });
+    off = align256(off);
+    cx.tables.attach(base + off, cap - off);
+}
+
 // Device bytes a batch of this layout needs beyond its inputs.
 size_t workspace_bytes(const Layout& L) {
     const size_t nslots = 2 * (size_t)std::max<uint32_t>(div_up(L.total_entries, L.T), 1);
@@ -1414,6 +1438,78 @@ size_t comb_table_bytes(size_t npoints, size_t wbits, size_t nbits) {
     return b << (wbits - 1);
 }
 
+// ---------------------------------------------------------------- no-allocation MSM
+//
+// blst_hip_msm_device_noalloc is blst_hip_msm_device for at most
+// kNoAllocMaxPoints points with every device buffer of its batch carved out
+// of `scratch`, a fixed array of this translation unit: the call allocates no
+// device or page-locked memory.  Each GPU the library runs on holds its own
+// copy of the array (kNoAllocScratch bytes) until the process ends.  In host
+// code `scratch` names a host-side stand-in, not a device address: the device
+// address of each GPU's copy comes from hipGetSymbolAddress, once per device
+// with that device current, and only that address is handed to kernels.
+constexpr size_t kNoAllocMaxPoints = (size_t)1 << 16;
+constexpr size_t kNoAllocScratch = (size_t)88 << 20;   // the committed tuning table needs at most 86.2 MB (65535 points, 256 bits)
+// host staging of the layout tables, then the window sums
+constexpr size_t kNoAllocHostBytes = (size_t)1 << 20;
+
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratch];
+
+struct NoAllocDevice {
+    std::mutex mu;               // one call at a time on this device
+    uint8_t* base = nullptr;     // device address of this GPU's scratch
+    Workspace ws;                // buffers attached inside [base, base + kNoAllocScratch)
+    std::vector<uint8_t> host;   // fixed host staging, sized at set-up
+};
+
+// Set-up on the first call on a device, with that device current: the
+// address of its copy of scratch and the stream of its calls.  Kept for the
+// life of the process (the runtime may be gone when static objects die).
+NoAllocDevice& noalloc_device() {
+    static std::mutex mu;
+    static auto* devices = new std::map<int, NoAllocDevice*>();
+    int dev = 0;
+    HIPCK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = devices->find(dev);
+    if (it != devices->end()) return *it->second;
+    std::unique_ptr<NoAllocDevice> d(new NoAllocDevice());
+    void* ptr = nullptr;
+    HIPCK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr || ((uintptr_t)ptr & 255)) throw HipFail{"scratch array has no usable device address"};
+    d->base = static_cast<uint8_t*>(ptr);
+    HIPCK(hipStreamCreateWithFlags(&d->ws.stream, hipStreamNonBlocking));
+    d->ws.owns_stream = true;
+    d->host.resize(kNoAllocHostBytes);
+    d->ws.host_fixed = d->host.data();
+    d->ws.host_fixed_cap = kNoAllocHostBytes / 2;
+    NoAllocDevice* raw = d.release();
+    devices->emplace(dev, raw);
+    return *raw;
+}
+
+// The MSM of blst_hip_msm_device (tuned window, one pipeline: every size up
+// to kNoAllocMaxPoints runs as one window group) in this device's scratch.
+void noalloc_msm(const void* device_points, const void* device_scalars, size_t npoints, size_t nbits,
+                 blst_p1* ret) {
+    NoAllocDevice& d = noalloc_device();
+    std::lock_guard<std::mutex> lk(d.mu);
+    const JobSpec js = make_spec(device_points, device_scalars, npoints, nbits, 0, false);
+    const Layout L = make_layout(std::vector<JobSpec>(1, js));
+    carve_workspace(d.ws, L, d.base, kNoAllocScratch);
+    const size_t nsums = L.groups.size();
+    if (nsums * sizeof(hostec::xyzz6) > kNoAllocHostBytes / 2) throw HipFail{"too many windows"};
+    hostec::xyzz6* sums = reinterpret_cast<hostec::xyzz6*>(d.host.data() + kNoAllocHostBytes / 2);
+    const DevLayout D = upload_layout(d.ws, L);
+    enqueue_batch(d.ws, L, D, true, sums, false);
+    HIPCK(hipStreamSynchronize(d.ws.stream));
+    hostec::xyzz6 r;
+    hostec::fold_windows(r, sums, (int)nsums, (int)js.c);
+    hostec::jac6 j;
+    hostec::to_jac(j, r);
+    std::memcpy(ret, &j, sizeof(j));
+}
+
 }  // namespace host
 }  // namespace bls
 
@@ -1753,6 +1849,33 @@ int blst_hip_msm_device_tuned(blst_p1* ret, const void* device_points, const voi
         return -1;
     }
     g_phase[6] = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_msm_device_noalloc(blst_p1* ret, const void* device_points, const void* device_scalars,
+                                size_t npoints, size_t nbits) {
+    if (!ret) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    std::memset(ret, 0, sizeof(*ret));
+    if (npoints == 0 || nbits == 0) { g_err.clear(); return 0; }
+    if (nbits > 256 || !device_points || !device_scalars || ((uintptr_t)device_points % 16) != 0) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    if (npoints > kNoAllocMaxPoints) {
+        g_err = "npoints above the " + std::to_string(kNoAllocMaxPoints) + "-point limit of the no-allocation MSM";
+        return -1;
+    }
+    try {
+        noalloc_msm(device_points, device_scalars, npoints, nbits, ret);
+    } catch (...) {
+        g_err = current_error();
+        std::memset(ret, 0, sizeof(*ret));
+        return -1;
+    }
     g_err.clear();
     return 0;
 }

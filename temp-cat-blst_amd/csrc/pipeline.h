@@ -41,10 +41,18 @@ inline void device_malloc(void** p, size_t bytes) {
 struct DevBuf {
     void* p = nullptr;
     size_t cap = 0;
+    bool fixed = false;           // a region of memory this buffer does not own
     std::vector<void*> retired;   // outgrown allocations, freed with the buffer
     template <class T> T* as() const { return static_cast<T*>(p); }
+    // use [q, q + bytes) of memory owned elsewhere; ensure() then never allocates
+    void attach(void* q, size_t bytes) {
+        p = q;
+        cap = bytes;
+        fixed = true;
+    }
     void ensure(size_t bytes) {
         if (bytes <= cap && p) return;
+        if (fixed) throw HipFail{"fixed device buffer too small"};
         // hipFree waits for the whole device, which would drain work queued
         // behind this call: an outgrown buffer is kept until destruction
         // instead (geometric growth bounds what is kept to the current size)
@@ -58,7 +66,7 @@ struct DevBuf {
     }
     ~DevBuf() {
         for (void* q : retired) (void)hipFree(q);
-        if (p) (void)hipFree(p);
+        if (p && !fixed) (void)hipFree(p);
     }
 };
 
@@ -92,7 +100,15 @@ struct Workspace {
     }
     // staging for the layout tables: page-locked, so their upload is a true
     // asynchronous copy that does not wait for work queued on the stream
+    // fixed host staging (set by the owner, never reallocated): host_fixed
+    // serves both the layout tables and the results, one after the other
+    uint8_t* host_fixed = nullptr;
+    size_t host_fixed_cap = 0;
     uint8_t* tables_stage(size_t bytes) {
+        if (host_fixed) {
+            if (bytes > host_fixed_cap) throw HipFail{"fixed host staging too small"};
+            return host_fixed;
+        }
         if (bytes > tables_cap) {
             if (tables_pinned) HIPCK(hipHostFree(tables_pinned));
             tables_pinned = nullptr;
@@ -104,6 +120,10 @@ struct Workspace {
         return tables_pinned;
     }
     void* host_buf(size_t bytes) {
+        if (host_fixed) {
+            if (bytes > host_fixed_cap) throw HipFail{"fixed host staging too small"};
+            return host_fixed;
+        }
         if (bytes > pinned_cap) {
             if (pinned) HIPCK(hipHostFree(pinned));
             pinned = nullptr;
@@ -197,6 +217,13 @@ Layout make_layout(const std::vector<JobSpec>& specs);
 size_t layout_table_bytes(const Layout& L);
 DevLayout upload_layout(Workspace& cx, const Layout& L, size_t base = 0);
 void ensure_workspace(Workspace& cx, const Layout& L);
+// Bytes of device memory that carve_workspace takes for a layout (its
+// buffers and layout tables, each 256-byte aligned).
+size_t carved_bytes(const Layout& L);
+// Points the workspace's buffers (and its layout-table buffer) into
+// [base, base + cap) with the sizes ensure_workspace would allocate; throws
+// if the region is too small.  Nothing is allocated.
+void carve_workspace(Workspace& cx, const Layout& L, uint8_t* base, size_t cap);
 size_t workspace_bytes(const Layout& L);
 bool tiled_sort(const Layout& L);
 // One point range of an MSM whose points arrive in ranges: prev holds the
