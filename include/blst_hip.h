@@ -179,10 +179,11 @@ int blst_hip_msm_device(blst_p1 *ret, const void *device_points, const void *dev
  * 65536 points (nbits <= 256, tuned window), computed entirely in a fixed
  * 88 MB device array that each GPU the library runs on holds for the life of
  * the process.  The call allocates no device or page-locked memory; the
- * first call on a device takes the array's device address and creates the
- * device's stream.  Every call on a device uses that device's one array,
- * which keeps the last call's contents (the MSM clears what it needs): calls
- * that overlapped on one device would race, so the library serialises them
+ * first call on a device takes the array's device address (and sets up the
+ * library's device context if no other entry point has).  Every call on a
+ * device uses that device's one array, which keeps the last call's contents
+ * (the MSM clears what it needs): calls that overlapped on one device would
+ * race, so the library serialises them
  * with a lock, and each completes before returning.  Returns 0, or -1 for
  * invalid arguments, more than 65536 points or a device error (*ret is then
  * infinity and blst_hip_last_error() says why). */

@@ -87,15 +87,17 @@ namespace {
 
 inline uint32_t div_up(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
-// device buffers and stream of the hooks, one set per process
+// Device buffers of the hooks, one set per process.  The hooks work on the
+// null stream and create none of their own: a process gets few hardware
+// queues (four by default), and a stream created here before the MSM
+// library's streams would make two of those share one queue, which runs
+// their work one after the other.
 struct Kit {
     std::mutex mu;
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // the null stream
     DevBuf in[4], out, scl, gtable, gen_tmp, gen_out;
     bool gtable_ready = false;
-    void init() {
-        if (!stream) HIPCK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-    }
+    void init() {}
 };
 
 Kit& kit() {
