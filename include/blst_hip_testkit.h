@@ -25,6 +25,13 @@ extern "C" {
  * device error. */
 int blst_hip_fp_ops(int op, void *out, const void *a, const void *b, const void *c, const void *d, size_t n);
 
+/* Measured issue rate of v_mad_u64_u32 (32 x 32 -> 64-bit multiply-add with
+ * a 64-bit addend, the instruction of every Montgomery product) on the
+ * current device: eight independent accumulators per lane, eight waves per
+ * SIMD on every CU, best of three timed launches after a warm-up.  Writes
+ * multiply-adds per second (all lanes of the device).  Returns 0 or -1. */
+int blst_hip_mad_issue_rate(double *mad_per_s);
+
 /* Makes every later device allocation of the product library larger than
  * `bytes` fail as an out-of-memory device would (0 lifts the cap), so tests
  * can drive the fail-closed paths: infinity (or NULL / -1) and a message in

@@ -76,6 +76,7 @@ SIGNATURES = {
 TESTKIT_SIGNATURES = {
     "blst_hip_fp_ops": ([ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _sz], ctypes.c_int),
     "blst_hip_set_alloc_cap": ([_sz], None),
+    "blst_hip_mad_issue_rate": ([_vp], ctypes.c_int),
     "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_point_ops_check": ([_vp, _sz], ctypes.c_int),
     "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
@@ -412,6 +413,14 @@ def fp_ops(op, a, b=None, c=None, d=None):
     if rc:
         raise RuntimeError("blst_hip_fp_ops: " + last_error())
     return out
+
+
+def mad_issue_rate():
+    """Measured v_mad_u64_u32 multiply-adds per second of the device."""
+    out = ctypes.c_double(0)
+    if testkit().blst_hip_mad_issue_rate(ctypes.byref(out)):
+        raise RuntimeError("blst_hip_mad_issue_rate: " + last_error())
+    return out.value
 
 
 def set_alloc_cap(nbytes):

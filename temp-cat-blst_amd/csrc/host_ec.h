@@ -38,31 +38,42 @@ inline void sub_p_if_ge(fp6& r, const uint64_t t[7]) {
     for (int j = 0; j < 6; j++) r.l[j] = keep ? t[j] : d[j];
 }
 
+// Montgomery product by separated operand scanning: the full 768-bit
+// product first (column sums of 64 x 64 -> 128-bit products in a 192-bit
+// accumulator), then six word-by-word reduction rounds t += m p 2^(64i) with
+// m = t_i (-1/p) mod 2^64, the result being words 6..11 less p if needed.
 inline void mul(fp6& r, const fp6& a, const fp6& b) {
-    uint64_t t[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint64_t t[13];
+    u128 acc = 0;
+    uint64_t over = 0;   // bits 128..191 of the column accumulator
+    for (int k = 0; k < 11; k++) {
+        const int lo = k < 6 ? 0 : k - 5, hi = k < 6 ? k : 5;
+        for (int i = lo; i <= hi; i++) {
+            const u128 p = (u128)a.l[i] * b.l[k - i];
+            acc += p;
+            over += acc < p;
+        }
+        t[k] = (uint64_t)acc;
+        acc = (acc >> 64) | ((u128)over << 64);
+        over = 0;
+    }
+    t[11] = (uint64_t)acc;
+    t[12] = 0;
     for (int i = 0; i < 6; i++) {
+        const uint64_t m = t[i] * kN0;
         uint64_t c = 0;
         for (int j = 0; j < 6; j++) {
-            u128 s = (u128)a.l[j] * b.l[i] + t[j] + c;
+            const u128 s = (u128)m * kP[j] + t[i + j] + c;
+            t[i + j] = (uint64_t)s;
+            c = (uint64_t)(s >> 64);
+        }
+        for (int j = i + 6; c && j < 13; j++) {
+            const u128 s = (u128)t[j] + c;
             t[j] = (uint64_t)s;
             c = (uint64_t)(s >> 64);
         }
-        u128 s = (u128)t[6] + c;
-        t[6] = (uint64_t)s;
-        const uint64_t hi = (uint64_t)(s >> 64);
-        const uint64_t m = t[0] * kN0;
-        s = (u128)m * kP[0] + t[0];
-        c = (uint64_t)(s >> 64);
-        for (int j = 1; j < 6; j++) {
-            s = (u128)m * kP[j] + t[j] + c;
-            t[j - 1] = (uint64_t)s;
-            c = (uint64_t)(s >> 64);
-        }
-        s = (u128)t[6] + c;
-        t[5] = (uint64_t)s;
-        t[6] = hi + (uint64_t)(s >> 64);
     }
-    sub_p_if_ge(r, t);
+    sub_p_if_ge(r, t + 6);
 }
 
 inline void add(fp6& r, const fp6& a, const fp6& b) {

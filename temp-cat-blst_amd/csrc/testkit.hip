@@ -82,6 +82,37 @@ __global__ void k_digits_debug(const uint8_t* __restrict__ scalars, uint32_t n, 
     out[i] = booth_digit_at(scalars + (size_t)i * nbytes, vec, bit0, c, nbits);
 }
 
+// Issue-rate probe of v_mad_u64_u32 (the multiply-add of every Montgomery
+// product): eight independent 64-bit accumulators per lane, so a wave always
+// has a multiply-add whose inputs are ready, 64 rounds per pass.  The sum of
+// the accumulators is stored (a vector store) so nothing is optimised away.
+__global__ void __launch_bounds__(256)
+k_mad_issue(uint64_t* __restrict__ out, uint32_t passes, uint32_t seed) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t x = seed ^ (i * 0x9e3779b9u), y = x * 0x85ebca6bu + 1u;
+    uint64_t a0 = i, a1 = i + 1, a2 = i + 2, a3 = i + 3, a4 = i + 4, a5 = i + 5, a6 = i + 6, a7 = i + 7;
+#pragma unroll 1
+    for (uint32_t p = 0; p < passes; p++) {
+#pragma unroll
+        for (int r = 0; r < 64; r++) {
+            asm volatile(
+                "v_mad_u64_u32 %0, vcc, %8, %9, %0\n\t"
+                "v_mad_u64_u32 %1, vcc, %8, %9, %1\n\t"
+                "v_mad_u64_u32 %2, vcc, %8, %9, %2\n\t"
+                "v_mad_u64_u32 %3, vcc, %8, %9, %3\n\t"
+                "v_mad_u64_u32 %4, vcc, %8, %9, %4\n\t"
+                "v_mad_u64_u32 %5, vcc, %8, %9, %5\n\t"
+                "v_mad_u64_u32 %6, vcc, %8, %9, %6\n\t"
+                "v_mad_u64_u32 %7, vcc, %8, %9, %7"
+                : "+v"(a0), "+v"(a1), "+v"(a2), "+v"(a3), "+v"(a4), "+v"(a5), "+v"(a6), "+v"(a7)
+                : "v"(x), "v"(y)
+                : "vcc");
+        }
+        x += 1;
+    }
+    out[i] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
+}
+
 namespace host {
 namespace {
 
@@ -178,6 +209,47 @@ int blst_hip_fp_ops(int op, void* out, const void* a, const void* b, const void*
         return -1;
     } catch (const std::exception& e) {
         g_err = e.what();
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_mad_issue_rate(double* mad_per_s) {
+    if (!mad_per_s) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Kit& k = kit();
+        std::lock_guard<std::mutex> lk(k.mu);
+        int dev = 0, cus = 0;
+        HIPCK(hipGetDevice(&dev));
+        HIPCK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        // eight waves per SIMD on every CU
+        const uint32_t blocks = (uint32_t)cus * 8, threads = 256, passes = 512;
+        k.out.ensure((size_t)blocks * threads * 8);
+        hipEvent_t e0, e1;
+        HIPCK(hipEventCreate(&e0));
+        HIPCK(hipEventCreate(&e1));
+        float best = 0;
+        for (int rep = 0; rep < 4; rep++) {
+            HIPCK(hipEventRecord(e0, k.stream));
+            hipLaunchKernelGGL(k_mad_issue, dim3(blocks), dim3(threads), 0, k.stream, k.out.as<uint64_t>(), passes,
+                               (uint32_t)rep);
+            HIPCK(hipGetLastError());
+            HIPCK(hipEventRecord(e1, k.stream));
+            HIPCK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0 && (best == 0 || ms < best)) best = ms;   // the first launch is a warm-up
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const double mads = (double)blocks * threads * passes * 64 * 8;
+        *mad_per_s = mads / (best * 1e-3);
+    } catch (const HipFail& f) {
+        g_err = f.msg;
         return -1;
     }
     g_err.clear();
