@@ -677,7 +677,7 @@ void for_each_buffer(Workspace& cx, const Layout& L, Fn&& f) {
     f(cx.buckets, (size_t)nkeys * pb);
     f(cx.slot_key, (size_t)nslots * 4);
     f(cx.slots, (size_t)nslots * pb);
-    f(cx.heavy, ((size_t)nslots + 64) * 4);
+    f(cx.heavy, (3 * (size_t)(nslots / 2) + 64) * 4);   // list lengths, then three lists of chunk capacity
     f(cx.H, L.tree_nodes * 2 * pb);
     f(cx.W, (size_t)L.groups.size() * pb);
     f(cx.res, (size_t)L.jobs.size() * pb);
@@ -718,7 +718,7 @@ size_t workspace_bytes(const Layout& L) {
                               : tiled_sort(L) ? L.count_words * 4 : (size_t)L.total_entries * 8;
     const size_t pb = L.point_bytes();
     return (size_t)L.total_entries * 8 + sort_bytes + ((size_t)L.nkeys + 1) * 8 + (size_t)L.nkeys * pb +
-           nslots * (6 + pb) + L.tree_nodes * 2 * pb + (L.groups.size() + L.jobs.size()) * pb + (1 << 20);
+           nslots * (10 + pb) + L.tree_nodes * 2 * pb + (L.groups.size() + L.jobs.size()) * pb + (1 << 20);
 }
 
 // The point stages of a batch (accumulate, partial merges, reduction, fold)
@@ -737,27 +737,26 @@ void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool hos
     uint32_t* hist = cx.hist.as<uint32_t>();
     uint32_t* off = cx.off.as<uint32_t>();
     // buckets are not cleared: every bucket with entries is written below,
-    // and the reduction skips the empty ones by their count
+    // and the reduction skips the empty ones by their count.  The
+    // accumulation lists the buckets of more than two pieces for the merge
+    // (list lengths cleared first).
+    uint32_t* heavy = cx.heavy.as<uint32_t>();
+    merge_lists_t lists;
+    lists.counts = heavy;
+    lists.wave = heavy + 64;
+    lists.group = lists.wave + nchunks;
+    lists.few = lists.group + nchunks;
+    fill_words(cx, heavy, 3, 0);
     if (timing) HIPCK(hipEventRecord(cx.ev[2], st));
     launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.sorted.as<const uint2>(), off + nkeys, D.jobs,
                         D.kbase, njobs, cx.buckets.as<P>(), cx.slot_key.as<uint32_t>(), cx.slots.as<P>(), T,
-                        nchunks, range.prev);
+                        nchunks, range.prev, off, hist, lists);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
 
-    // buckets spanning chunks: two partials in one lane, longer spans listed
-    // for one wave each, the longest for one workgroup each
-    uint32_t* heavy = cx.heavy.as<uint32_t>();
-    uint32_t* wave_list = heavy + 64;
-    uint32_t* group_list = wave_list + nchunks;
-    fill_words(cx, heavy, 2, 0);
-    launch_k_partial_light(dim3(div_up(nchunks, 256)), dim3(256), st, cx.slot_key.as<const uint32_t>(),
-                           cx.slots.as<const P>(), off, hist, T, nchunks, cx.buckets.as<P>(), heavy, wave_list,
-                           group_list);
-    launch_k_partial_wave(dim3(std::min<uint32_t>(div_up(nchunks, 16), 2048)), st, heavy, wave_list,
-                          cx.slots.as<const P>(), off, hist, T, cx.buckets.as<P>());
-    launch_k_partial_heavy(dim3(std::min<uint32_t>(div_up(nchunks, 512), 256)), st, heavy + 1, group_list,
-                           cx.slots.as<const P>(), off, hist, T, cx.buckets.as<P>());
+    // buckets spanning chunks: every kind of merge in one launch
+    launch_k_partial_merge(st, cx.slot_key.as<const uint32_t>(), cx.slots.as<const P>(), off, hist, T, nchunks,
+                           cx.buckets.as<P>(), lists);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[4], st));
     if (range.cnt_sum) launch_k_add_words(st, hist, range.cnt_sum, nkeys, range.prev == nullptr);

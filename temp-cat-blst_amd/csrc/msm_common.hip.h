@@ -30,6 +30,27 @@ namespace bls {
 
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 
+// Buckets whose accumulation left more than two pieces (they spanned more
+// than one chunk boundary), listed by the accumulate kernel for the merge by
+// piece count m: up to kLightSpan pieces one lane each, up to kWaveSpan one
+// wave each, more one workgroup each.  counts: [wave, group, few] lengths,
+// cleared before the accumulation.
+constexpr uint32_t kLightSpan = 4;
+constexpr uint32_t kWaveSpan = 128;
+
+struct merge_lists_t {
+    uint32_t* counts;
+    uint32_t* few;
+    uint32_t* wave;
+    uint32_t* group;
+};
+
+__device__ __forceinline__ void list_bucket(const merge_lists_t& L, uint32_t m, uint32_t key) {
+    if (m <= kLightSpan) L.few[atomicAdd(&L.counts[2], 1u)] = key;
+    else if (m <= kWaveSpan) L.wave[atomicAdd(&L.counts[0], 1u)] = key;
+    else L.group[atomicAdd(&L.counts[1], 1u)] = key;
+}
+
 
 // ---------------------------------------------------------------- helpers
 
