@@ -268,6 +268,30 @@ def entry_points(bh, points, scalars, r, n, reps=5):
     return rep, jac, (table, wb, ssc, bh.compress(rw)), g2_sample
 
 
+def ba_probe(npairs=1 << 25):
+    """Batch-affine addition against the XYZZ mixed addition of the bucket
+    accumulation on the same 2^25 point pairs (test-kit probe kernels), for
+    contiguous pairs and for pairs gathered through a random index (as the
+    first level of a sorted bucket list would read them); the probe checks
+    its sums in tests/test_ba_probe_gpu.py."""
+    import blst_hip as bh
+
+    pts = bh.synth_points(2 * npairs, SEED ^ 0xBA)
+    idx = np.random.default_rng(SEED & 0xFFFFFFFF).permutation(2 * npairs).astype(np.uint32)
+    d_p, d_i = bh.DeviceBuffer(pts), bh.DeviceBuffer(idx)
+    rows = []
+    try:
+        for gather in (False, True):
+            for K in (32, 128):
+                ns, _, _ = bh.batch_affine_probe(d_p.ptr, npairs, K, d_i.ptr if gather else None, read=False)
+                rows.append({"pairs_per_lane": K, "gathered": gather, "batch_affine_ns_per_add": ns[0],
+                             "xyzz_ns_per_add": ns[1], "batch_affine_speedup": ns[1] / ns[0]})
+    finally:
+        d_p.close()
+        d_i.close()
+    print(json.dumps({"batch_affine_probe": {"pairs": npairs, "rows": rows}}))
+
+
 def copy_calibration(nbytes, reps=3):
     """Device-to-device copies of a known size: the HIP runtime's blit kernel
     reads and writes nbytes each, so FETCH_SIZE / WRITE_SIZE of these
@@ -305,11 +329,16 @@ def main():
                     help="offered request rate (req/s); 0 = --serve-load times the measured capacity")
     ap.add_argument("--serve-load", type=float, default=0.9, help="offered load as a fraction of capacity")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--ba-probe", action="store_true",
+                    help="only run the batch-affine against XYZZ addition probe")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+    if args.ba_probe:
+        ba_probe()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the MSM backend runs on a single device "

@@ -32,6 +32,20 @@ int blst_hip_fp_ops(int op, void *out, const void *a, const void *b, const void 
  * multiply-adds per second (all lanes of the device).  Returns 0 or -1. */
 int blst_hip_mad_issue_rate(double *mad_per_s);
 
+/* Probe of batch-affine addition against the XYZZ mixed addition of the
+ * bucket accumulation, on the same npairs point pairs (device points,
+ * 96-byte affine; pair j is points 2j, 2j+1, or device_index[2j],
+ * device_index[2j+1] when an index is given): batch-affine with K =
+ * pairs_per_lane (8, 32 or 128) pairs per lane and one inversion per
+ * 256-lane workgroup, writing the sums as canonical affine points to
+ * device_affine_out, and acc = P, acc += Q by madd-2008-s writing canonical
+ * XYZZ points to device_xyzz_out.  npairs must be a multiple of 256 K.
+ * Writes the best of three timed launches as nanoseconds per pair:
+ * [batch-affine, XYZZ].  Returns 0 or -1. */
+int blst_hip_batch_affine_probe(const void *device_points, const uint32_t *device_index, size_t npairs,
+                                int pairs_per_lane, double ns_per_pair[2], void *device_affine_out,
+                                void *device_xyzz_out);
+
 /* Makes every later device allocation of the product library larger than
  * `bytes` fail as an out-of-memory device would (0 lifts the cap), so tests
  * can drive the fail-closed paths: infinity (or NULL / -1) and a message in

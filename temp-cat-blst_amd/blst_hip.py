@@ -77,6 +77,7 @@ TESTKIT_SIGNATURES = {
     "blst_hip_fp_ops": ([ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _sz], ctypes.c_int),
     "blst_hip_set_alloc_cap": ([_sz], None),
     "blst_hip_mad_issue_rate": ([_vp], ctypes.c_int),
+    "blst_hip_batch_affine_probe": ([_vp, _vp, _sz, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
     "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_point_ops_check": ([_vp, _sz], ctypes.c_int),
     "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
@@ -421,6 +422,28 @@ def mad_issue_rate():
     if testkit().blst_hip_mad_issue_rate(ctypes.byref(out)):
         raise RuntimeError("blst_hip_mad_issue_rate: " + last_error())
     return out.value
+
+
+def batch_affine_probe(d_points_ptr, npairs, pairs_per_lane, d_index_ptr=None, read=True):
+    """Times batch-affine and XYZZ additions of the same pairs; returns
+    (ns per pair [batch-affine, xyzz], affine sums (npairs, 12) uint64,
+    xyzz sums (npairs, 24) uint64), the sums None when read is False."""
+    d_aff = DeviceAlloc(npairs * 96)
+    d_xz = DeviceAlloc(npairs * 192)
+    try:
+        ns = np.zeros(2, np.float64)
+        rc = testkit().blst_hip_batch_affine_probe(ctypes.c_void_p(d_points_ptr),
+                                                   ctypes.c_void_p(d_index_ptr) if d_index_ptr else None, npairs,
+                                                   pairs_per_lane, _ptr(ns), ctypes.c_void_p(d_aff.ptr),
+                                                   ctypes.c_void_p(d_xz.ptr))
+        if rc:
+            raise RuntimeError("blst_hip_batch_affine_probe: " + last_error())
+        aff = d_aff.read().view(np.uint64).reshape(npairs, 12) if read else None
+        xz = d_xz.read().view(np.uint64).reshape(npairs, 24) if read else None
+    finally:
+        d_aff.close()
+        d_xz.close()
+    return ns, aff, xz
 
 
 def set_alloc_cap(nbytes):

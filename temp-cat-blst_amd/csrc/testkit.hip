@@ -113,6 +113,9 @@ k_mad_issue(uint64_t* __restrict__ out, uint32_t passes, uint32_t seed) {
     out[i] = a0 + a1 + a2 + a3 + a4 + a5 + a6 + a7;
 }
 
+void launch_ba_probe(hipStream_t stream, int K, bool batch_affine, const uint4* pts, const uint32_t* idx, fp* pi,
+                     void* out, uint32_t npairs);
+
 namespace host {
 namespace {
 
@@ -248,6 +251,47 @@ int blst_hip_mad_issue_rate(double* mad_per_s) {
         (void)hipEventDestroy(e1);
         const double mads = (double)blocks * threads * passes * 64 * 8;
         *mad_per_s = mads / (best * 1e-3);
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_batch_affine_probe(const void* device_points, const uint32_t* device_index, size_t npairs,
+                                int pairs_per_lane, double ns_per_pair[2], void* device_affine_out,
+                                void* device_xyzz_out) {
+    const int K = pairs_per_lane;
+    if (!device_points || !ns_per_pair || !device_affine_out || !device_xyzz_out || (K != 8 && K != 32 && K != 128) ||
+        npairs == 0 || npairs % ((size_t)K * 256) != 0 || npairs >= (1u << 28) || ((uintptr_t)device_points & 15)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Kit& k = kit();
+        std::lock_guard<std::mutex> lk(k.mu);
+        k.scl.ensure(npairs * sizeof(fp));   // prefix products
+        hipEvent_t e0, e1;
+        HIPCK(hipEventCreate(&e0));
+        HIPCK(hipEventCreate(&e1));
+        for (int mode = 0; mode < 2; mode++) {
+            float best = 0;
+            for (int rep = 0; rep < 4; rep++) {
+                HIPCK(hipEventRecord(e0, k.stream));
+                launch_ba_probe(k.stream, K, mode == 0, static_cast<const uint4*>(device_points), device_index,
+                                k.scl.as<fp>(), mode == 0 ? device_affine_out : device_xyzz_out, (uint32_t)npairs);
+                HIPCK(hipGetLastError());
+                HIPCK(hipEventRecord(e1, k.stream));
+                HIPCK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCK(hipEventElapsedTime(&ms, e0, e1));
+                if (rep > 0 && (best == 0 || ms < best)) best = ms;   // the first launch is a warm-up
+            }
+            ns_per_pair[mode] = best * 1e6 / (double)npairs;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
     } catch (const HipFail& f) {
         g_err = f.msg;
         return -1;
