@@ -159,6 +159,20 @@ void blst_p2s_tile_pippenger(blst_p2 *ret, const blst_p2_affine *const points[],
                              const blst_byte *const scalars[], size_t nbits, blst_limb_t *scratch, size_t bit0,
                              size_t window);
 
+/* Replace blst_p2s_mult_pippenger_fr_scratch_sizeof, blst_p2s_mult_pippenger_fr,
+ * blst_p2s_mult_fr_scratch_sizeof and blst_p2s_mult_fr
+ * (bindings/blst_scalar.h:742-762; blst/c_api.cpp:1976-1991, 2009-2027): the contracts of
+ * the G1 blst_fr entry points over G2, with the reference's G2 scratch sizes
+ * (32 bytes per canonical scalar plus 384-byte buckets; for blst_p2s_mult_fr
+ * the capacity maximum over its batch-affine plan, which G2 uses from 2^12 to
+ * 2^14 points with 200- and 296-byte bucket elements). */
+size_t blst_p2s_mult_pippenger_fr_scratch_sizeof(size_t npoints);
+bool blst_p2s_mult_pippenger_fr(blst_p2 *ret, const blst_p2_affine points[], const blst_fr scalars[],
+                                size_t npoints, blst_limb_t *scratch, size_t scratch_size);
+size_t blst_p2s_mult_fr_scratch_sizeof(size_t max_points);
+bool blst_p2s_mult_fr(blst_p2 *ret, const blst_p2_affine points[], const blst_fr scalars[], size_t npoints,
+                      blst_limb_t *scratch, size_t scratch_size);
+
 /* ------------------------------------------------------- device extensions */
 
 /* Last error message of the calling thread ("" when the last call succeeded). */

@@ -613,10 +613,33 @@ void or_frs_to_scalars(uint8_t *out, const u64 *fr, size_t n) {
     for (size_t i = 0; i < n; i++) or_fr_to_scalar(out + 32 * i, fr + 4 * i);
 }
 
+/* The scratch-size queries of the blst_fr entry points for either group:
+ * the element types of one group's layouts
+ * (blst/scalar/curve/multi_scalar_batch_affine.cpp:16-36; Point<Field> is
+ * Jacobian (X, Y, Z), Pointxyzz<Field> four coordinates) and the largest size
+ * the planner gives to batch-affine (c_api.cpp:700-719: 2^18 points for G1,
+ * 2^14 for G2). */
+typedef struct { u64 x[6], y[6]; uint32_t generation, queued_epoch; } ba_affine_bucket_t;
+typedef struct { u64 point[18]; uint32_t generation; } ba_projective_bucket_t;
+typedef struct { u64 x[12], y[12]; uint32_t generation, queued_epoch; } ba2_affine_bucket_t;
+typedef struct { u64 point[36]; uint32_t generation; } ba2_projective_bucket_t;
+typedef struct { uint32_t point_index, bucket_index, subtract; } ba_schedule_point_t;
+
+typedef struct {
+    size_t xyzz, affine, affine_align, projective, projective_align, field, measured_max;
+} fr_layout_t;
+
+static const fr_layout_t kG1 = {4 * 48, sizeof(ba_affine_bucket_t), _Alignof(ba_affine_bucket_t),
+                                sizeof(ba_projective_bucket_t), _Alignof(ba_projective_bucket_t), 48,
+                                (size_t)1 << 18};
+static const fr_layout_t kG2 = {4 * 96, sizeof(ba2_affine_bucket_t), _Alignof(ba2_affine_bucket_t),
+                                sizeof(ba2_projective_bucket_t), _Alignof(ba2_projective_bucket_t), 96,
+                                (size_t)1 << 14};
+
 /* c_api_pippenger_fr_scratch_sizeof (blst/c_api.cpp:272-299): 32 bytes per
- * canonical scalar, and for more than one point 2^(window-1) XYZZ buckets of
- * 192 bytes (the G1PointZZ of Pippenger), 0 on overflow. */
-size_t or_pippenger_fr_scratch_sizeof(size_t npoints) {
+ * canonical scalar, and for more than one point 2^(window-1) XYZZ buckets
+ * (the GroupZZ of Pippenger), 0 on overflow. */
+static size_t pippenger_fr_scratch(size_t npoints, const fr_layout_t *G) {
     if (npoints == 0) return 0;
     if (npoints > SIZE_MAX / 32) return 0;
     const size_t scalar_bytes = npoints * 32;
@@ -624,40 +647,33 @@ size_t or_pippenger_fr_scratch_sizeof(size_t npoints) {
     const size_t window = or_pippenger_window_size(npoints);
     if (window == 0 || window - 1 >= 64) return 0;
     const size_t buckets = (size_t)1 << (window - 1);
-    if (buckets > (SIZE_MAX - scalar_bytes) / 192) return 0;
-    return scalar_bytes + buckets * 192;
+    if (buckets > (SIZE_MAX - scalar_bytes) / G->xyzz) return 0;
+    return scalar_bytes + buckets * G->xyzz;
 }
-
-/* The batch-affine workspace element types for G1
- * (blst/scalar/curve/multi_scalar_batch_affine.cpp:16-36): Fp is six 64-bit
- * limbs, Point<Fp> is Jacobian (X, Y, Z). */
-typedef struct { u64 x[6], y[6]; uint32_t generation, queued_epoch; } ba_affine_bucket_t;
-typedef struct { u64 point[18]; uint32_t generation; } ba_projective_bucket_t;
-typedef struct { uint32_t point_index, bucket_index, subtract; } ba_schedule_point_t;
-typedef struct { u64 l[6]; } ba_fp_t;
 
 static size_t ba_align(size_t v, size_t a) { return (v + a - 1) & ~(a - 1); }
 
 /* batch_affine_scratch_size (multi_scalar_batch_affine.cpp:149-168): the
- * five arrays in order, each at its element's alignment, the total rounded
- * to the limb size; window outside [1, 24] gives 0 */
-static size_t batch_affine_scratch_size(size_t window, size_t batch) {
+ * five arrays in order (affine buckets, projective buckets, the schedule,
+ * two field arrays of the batch), each at its element's alignment, the total
+ * rounded to the limb size; window outside [1, 24] gives 0 */
+static size_t batch_affine_scratch_size(size_t window, size_t batch, const fr_layout_t *G) {
     if (window < 1 || window > 24) return 0;
     const size_t B = (size_t)1 << (window - 1);
     size_t off = 0;
-    off = ba_align(off, _Alignof(ba_affine_bucket_t)) + B * sizeof(ba_affine_bucket_t);
-    off = ba_align(off, _Alignof(ba_projective_bucket_t)) + B * sizeof(ba_projective_bucket_t);
+    off = ba_align(off, G->affine_align) + B * G->affine;
+    off = ba_align(off, G->projective_align) + B * G->projective;
     off = ba_align(off, _Alignof(ba_schedule_point_t)) + batch * sizeof(ba_schedule_point_t);
-    off = ba_align(off, _Alignof(ba_fp_t)) + batch * sizeof(ba_fp_t);
-    off = ba_align(off, _Alignof(ba_fp_t)) + batch * sizeof(ba_fp_t);
+    off = ba_align(off, sizeof(u64)) + batch * G->field;
+    off = ba_align(off, sizeof(u64)) + batch * G->field;
     return ba_align(off, sizeof(u64));
 }
 
 /* c_api_batch_affine_fr_scratch_sizeof (blst/c_api.cpp:593-627) */
-size_t or_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t batch) {
+static size_t batch_affine_fr_scratch(size_t npoints, size_t window, size_t batch, const fr_layout_t *G) {
     if (npoints == 0 || npoints > 0xffffffffULL) return 0;
     if (batch != 128 && batch != 384) return 0;
-    const size_t alg = batch_affine_scratch_size(window, batch);
+    const size_t alg = batch_affine_scratch_size(window, batch, G);
     if (alg == 0) return 0;
     if (npoints > SIZE_MAX / 32) return 0;
     const size_t scalar_bytes = npoints * 32;
@@ -666,10 +682,11 @@ size_t or_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t b
     return scalar_bytes + alg;
 }
 
-/* c_api_msm_fr_plan (blst/c_api.cpp:719-755) for G1: batch-affine from 2^12
- * to 2^18 points, window by size, batch 384 from 2^18 points; 0 = Pippenger */
-static size_t msm_fr_plan(size_t n, size_t *batch) {
-    if (n < ((size_t)1 << 12) || n > ((size_t)1 << 18)) return 0;
+/* c_api_msm_fr_plan (blst/c_api.cpp:719-755): batch-affine from 2^12 points
+ * to the group's measured maximum, window by size, batch 384 from 2^18
+ * points; 0 = Pippenger */
+static size_t msm_fr_plan(size_t n, size_t *batch, const fr_layout_t *G) {
+    if (n < ((size_t)1 << 12) || n > G->measured_max) return 0;
     size_t w;
     if (n <= ((size_t)1 << 12)) w = 10;
     else if (n <= ((size_t)1 << 14)) w = 11;
@@ -682,25 +699,35 @@ static size_t msm_fr_plan(size_t n, size_t *batch) {
 }
 
 /* c_api_msm_fr_scratch_sizeof (blst/c_api.cpp:758-807): a capacity query */
-size_t or_msm_fr_scratch_sizeof(size_t max_points) {
+static size_t msm_fr_scratch(size_t max_points, const fr_layout_t *G) {
     if (max_points == 0) return 0;
-    const size_t pip = or_pippenger_fr_scratch_sizeof(max_points);
+    const size_t pip = pippenger_fr_scratch(max_points, G);
     if (pip == 0) return 0;
-    size_t required = pip, batch = 0, w = msm_fr_plan(max_points, &batch);
+    size_t required = pip, batch = 0, w = msm_fr_plan(max_points, &batch, G);
     if (w) {
-        const size_t b = or_batch_affine_fr_scratch_sizeof(max_points, w, batch);
+        const size_t b = batch_affine_fr_scratch(max_points, w, batch, G);
         if (b == 0) return 0;
         if (b > required) required = b;
     }
-    const size_t measured_max = (size_t)1 << 18;
-    if (max_points > measured_max) {
-        w = msm_fr_plan(measured_max, &batch);
-        const size_t b = or_batch_affine_fr_scratch_sizeof(measured_max, w, batch);
+    if (max_points > G->measured_max) {
+        w = msm_fr_plan(G->measured_max, &batch, G);
+        const size_t b = batch_affine_fr_scratch(G->measured_max, w, batch, G);
         if (b == 0) return 0;
         if (b > required) required = b;
     }
     return required;
 }
+
+size_t or_pippenger_fr_scratch_sizeof(size_t npoints) { return pippenger_fr_scratch(npoints, &kG1); }
+size_t or_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t batch) {
+    return batch_affine_fr_scratch(npoints, window, batch, &kG1);
+}
+size_t or_msm_fr_scratch_sizeof(size_t max_points) { return msm_fr_scratch(max_points, &kG1); }
+size_t or_p2_pippenger_fr_scratch_sizeof(size_t npoints) { return pippenger_fr_scratch(npoints, &kG2); }
+size_t or_p2_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t batch) {
+    return batch_affine_fr_scratch(npoints, window, batch, &kG2);
+}
+size_t or_p2_msm_fr_scratch_sizeof(size_t max_points) { return msm_fr_scratch(max_points, &kG2); }
 
 /* get_wval_limb (ec_mult.h:63-82): up to 25 bits starting at bit |off|,
  * reading only the bytes that hold them */

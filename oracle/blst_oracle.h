@@ -56,6 +56,9 @@ void or_frs_to_scalars(uint8_t *out, const uint64_t *fr, size_t n);
 size_t or_pippenger_fr_scratch_sizeof(size_t npoints);
 size_t or_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t batch);
 size_t or_msm_fr_scratch_sizeof(size_t max_points);
+size_t or_p2_pippenger_fr_scratch_sizeof(size_t npoints);
+size_t or_p2_batch_affine_fr_scratch_sizeof(size_t npoints, size_t window, size_t batch);
+size_t or_p2_msm_fr_scratch_sizeof(size_t max_points);
 
 /* fixed-base tables: n rows of 2^(wbits-1) affine multiples, and the MSM
  * over them (contiguous scalars of ceil(nbits/8) bytes) */

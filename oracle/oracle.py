@@ -90,6 +90,9 @@ def lib():
             ("or_pippenger_fr_scratch_sizeof", [_sz], _sz),
             ("or_batch_affine_fr_scratch_sizeof", [_sz, _sz, _sz], _sz),
             ("or_msm_fr_scratch_sizeof", [_sz], _sz),
+            ("or_p2_pippenger_fr_scratch_sizeof", [_sz], _sz),
+            ("or_p2_batch_affine_fr_scratch_sizeof", [_sz, _sz, _sz], _sz),
+            ("or_p2_msm_fr_scratch_sizeof", [_sz], _sz),
             ("or_wbits_precompute_sizeof", [_sz, _sz], _sz),
             ("or_wbits_scratch_sizeof", [_sz], _sz),
         ]:
@@ -313,6 +316,18 @@ def msm_fr_scratch_sizeof(n):
 
 
 # ---- fixed-base tables -----------------------------------------------------
+
+def p2_pippenger_fr_scratch_sizeof(n):
+    return lib().or_p2_pippenger_fr_scratch_sizeof(n)
+
+
+def p2_batch_affine_fr_scratch_sizeof(n, window, batch):
+    return lib().or_p2_batch_affine_fr_scratch_sizeof(n, window, batch)
+
+
+def p2_msm_fr_scratch_sizeof(n):
+    return lib().or_p2_msm_fr_scratch_sizeof(n)
+
 
 def wbits_precompute_sizeof(wbits, n):
     return lib().or_wbits_precompute_sizeof(wbits, n)

@@ -44,6 +44,10 @@ SIGNATURES = {
     "blst_p2s_mult_pippenger_window": ([_vp, _vp, _sz, _vp, _sz, _vp, _sz], None),
     "blst_p2s_tile_pippenger": ([_vp, _vp, _sz, _vp, _sz, _vp, _sz, _sz], None),
     "blst_hip_p2_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
+    "blst_p2s_mult_pippenger_fr_scratch_sizeof": ([_sz], _sz),
+    "blst_p2s_mult_pippenger_fr": ([_vp, _vp, _vp, _sz, _vp, _sz], ctypes.c_bool),
+    "blst_p2s_mult_fr_scratch_sizeof": ([_sz], _sz),
+    "blst_p2s_mult_fr": ([_vp, _vp, _vp, _sz, _vp, _sz], ctypes.c_bool),
     "blst_hip_last_error": ([], ctypes.c_char_p),
     "blst_hip_available": ([], ctypes.c_int),
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
@@ -223,6 +227,37 @@ def p1s_mult_fr(points, fr, entry="pippenger", scratch_size=None, ret=None, null
     for k in null:
         args[k] = None
     f = L.blst_p1s_mult_pippenger_fr if entry == "pippenger" else L.blst_p1s_mult_fr
+    ok = f(args["ret"], args["points"], args["scalars"], n, args["scratch"], scratch_size)
+    return bool(ok), ret
+
+
+def p2_fr_scratch_sizeof(n, entry="pippenger"):
+    L = load()
+    f = L.blst_p2s_mult_pippenger_fr_scratch_sizeof if entry == "pippenger" else L.blst_p2s_mult_fr_scratch_sizeof
+    return f(n)
+
+
+def p2s_mult_fr(points, fr, entry="pippenger", scratch_size=None, ret=None, null=()):
+    """G2 MSM of contiguous points (n x 24 uint64) and Montgomery-form Fr
+    scalars through blst_p2s_mult_pippenger_fr / blst_p2s_mult_fr; returns
+    (ok, 36-limb blst_p2 buffer), the buffer untouched by a call that returns
+    False."""
+    L = load()
+    points = np.ascontiguousarray(points, dtype=np.uint64)
+    fr = np.ascontiguousarray(fr, dtype=np.uint64)
+    n = points.shape[0]
+    assert fr.shape[0] == n
+    if ret is None:
+        ret = np.zeros(36, np.uint64)
+    need = p2_fr_scratch_sizeof(n, entry)
+    if scratch_size is None:
+        scratch_size = need
+    scr = np.zeros(max((scratch_size + 7) // 8, 1), np.uint64)
+    args = {"ret": _ptr(ret), "points": _ptr(points) if n else _ptr(np.zeros(24, np.uint64)),
+            "scalars": _ptr(fr) if n else _ptr(np.zeros(4, np.uint64)), "scratch": _ptr(scr)}
+    for k in null:
+        args[k] = None
+    f = L.blst_p2s_mult_pippenger_fr if entry == "pippenger" else L.blst_p2s_mult_fr
     ok = f(args["ret"], args["points"], args["scalars"], n, args["scratch"], scratch_size)
     return bool(ok), ret
 

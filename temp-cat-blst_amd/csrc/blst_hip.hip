@@ -1206,6 +1206,35 @@ void ranged_msm_g2(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stag
     std::memcpy(ret, &j, sizeof(j));
 }
 
+// A G2 MSM over contiguous host buffers (points, and scalars of nbits bits
+// or, with fr, Montgomery-form blst_fr values converted on the device): from
+// host buffers the points cross PCIe in ranges, as for G1.
+void host_buffers_msm_g2(Ctx& cx, const uint8_t* hp, const uint8_t* hs, size_t npoints, size_t nbits, size_t window,
+                         bool fr, bool fold, size_t bit0, blst_p2* ret) {
+    const size_t nbytes = fr ? 32 : (nbits + 7) / 8;
+    cx.pts.ensure(npoints * sizeof(blst_p2_affine));
+    cx.scl.ensure(npoints * nbytes + 16);
+    uint8_t* dp = static_cast<uint8_t*>(cx.pts.p);
+    uint8_t* ds = static_cast<uint8_t*>(cx.scl.p);
+    auto stage = [&](uint32_t, size_t lo, size_t m) {
+        cx.stager.upload(ds + lo * nbytes, hs + lo * nbytes, m * nbytes, cx.stream);
+        cx.stager.upload(dp + lo * sizeof(blst_p2_affine), hp + lo * sizeof(blst_p2_affine),
+                         m * sizeof(blst_p2_affine), cx.stream);
+        if (fr) launch_k_fr_to_scalar(cx.stream, reinterpret_cast<uint32_t*>(ds + lo * nbytes), (uint32_t)m);
+        HIPCK(hipGetLastError());
+    };
+    JobSpec js = make_spec(dp, ds, npoints, nbits, window, false);
+    const uint32_t K = fold ? point_ranges(npoints, kHostSplitMin, kHostRangePoints) : 1;
+    if (K > 1) {
+        ranged_msm_g2(cx, js, nbytes, K, stage, ret);
+    } else {
+        stage(0, 0, npoints);
+        js.bit_base = (uint32_t)bit0;
+        js.tile = !fold;
+        device_msm_g2(cx, js, ret);
+    }
+}
+
 int host_msm_g2(blst_p2* ret, const blst_p2_affine* const points[], size_t npoints,
                 const blst_byte* const scalars[], size_t nbits, size_t window, bool fold, size_t bit0) {
     try {
@@ -1215,26 +1244,7 @@ int host_msm_g2(blst_p2* ret, const blst_p2_affine* const points[], size_t npoin
         std::vector<uint8_t> sp, ss;
         const uint8_t* hp = resolve(points, npoints, sizeof(blst_p2_affine), sp);
         const uint8_t* hs = resolve(scalars, npoints, nbytes, ss);
-        cx.pts.ensure(npoints * sizeof(blst_p2_affine));
-        cx.scl.ensure(npoints * nbytes + 16);
-        uint8_t* dp = static_cast<uint8_t*>(cx.pts.p);
-        uint8_t* ds = static_cast<uint8_t*>(cx.scl.p);
-        auto stage = [&](uint32_t, size_t lo, size_t m) {
-            cx.stager.upload(ds + lo * nbytes, hs + lo * nbytes, m * nbytes, cx.stream);
-            cx.stager.upload(dp + lo * sizeof(blst_p2_affine), hp + lo * sizeof(blst_p2_affine),
-                             m * sizeof(blst_p2_affine), cx.stream);
-        };
-        JobSpec js = make_spec(dp, ds, npoints, nbits, window, false);
-        // from host buffers the points cross PCIe in ranges, as for G1
-        const uint32_t K = fold ? point_ranges(npoints, kHostSplitMin, kHostRangePoints) : 1;
-        if (K > 1) {
-            ranged_msm_g2(cx, js, nbytes, K, stage, ret);
-        } else {
-            stage(0, 0, npoints);
-            js.bit_base = (uint32_t)bit0;
-            js.tile = !fold;
-            device_msm_g2(cx, js, ret);
-        }
+        host_buffers_msm_g2(cx, hp, hs, npoints, nbits, window, false, fold, bit0, ret);
     } catch (...) {
         g_err = current_error();
         std::memset(ret, 0, sizeof(*ret));
@@ -1539,7 +1549,19 @@ struct blst_hip_batch_plan {
 
 namespace {
 
-size_t pippenger_fr_scratch(size_t n) {
+// The element sizes of one group's scratch layouts: the Pippenger bucket
+// (Pointxyzz), and the batch-affine workspace's affine bucket (x, y, two
+// u32), projective collision bucket (Jacobian point + u32, 8-byte aligned)
+// and field element; the largest size the reference's blst_fr planner sends
+// to batch-affine (multi_scalar_batch_affine.cpp:16-36,
+// c_api.cpp:700-719).
+struct FrLayout {
+    size_t xyzz, affine_bucket, projective_bucket, field, batch_affine_max;
+};
+constexpr FrLayout kFrG1 = {192, 104, 152, 48, (size_t)1 << 18};
+constexpr FrLayout kFrG2 = {384, 200, 296, 96, (size_t)1 << 14};
+
+size_t pippenger_fr_scratch(size_t n, const FrLayout& G = kFrG1) {
     if (n == 0) return 0;
     size_t scalar_bytes;
     if (__builtin_mul_overflow(n, (size_t)32, &scalar_bytes)) return 0;
@@ -1547,32 +1569,31 @@ size_t pippenger_fr_scratch(size_t n) {
     const size_t w = ref_window_size(n);
     if (w == 0 || w - 1 >= 8 * sizeof(size_t)) return 0;
     size_t buckets, total;
-    if (__builtin_mul_overflow((size_t)1 << (w - 1), (size_t)192, &buckets) ||
+    if (__builtin_mul_overflow((size_t)1 << (w - 1), G.xyzz, &buckets) ||
         __builtin_add_overflow(scalar_bytes, buckets, &total))
         return 0;
     return total;
 }
 
-// batch-affine workspace for G1: affine buckets (x, y, two u32: 104 B),
-// projective collision buckets (Jacobian point + u32: 152 B), the schedule
-// (3 x u32 per entry) and two Fp arrays of the batch size, each array aligned
-// to its element
-size_t batch_affine_algorithm_scratch(size_t window, size_t batch) {
+// batch-affine workspace: affine buckets, projective collision buckets, the
+// schedule (3 x u32 per entry) and two field arrays of the batch size, each
+// array aligned to its element
+size_t batch_affine_algorithm_scratch(size_t window, size_t batch, const FrLayout& G) {
     if (window < 1 || window > 24) return 0;
     auto align = [](size_t v, size_t a) { return (v + a - 1) & ~(a - 1); };
     const size_t B = (size_t)1 << (window - 1);
-    size_t off = B * 104;
-    off = align(off, 8) + B * 152;
+    size_t off = B * G.affine_bucket;
+    off = align(off, 8) + B * G.projective_bucket;
     off = align(off, 4) + batch * 12;
-    off = align(off, 8) + batch * 48;
-    off = align(off, 8) + batch * 48;
+    off = align(off, 8) + batch * G.field;
+    off = align(off, 8) + batch * G.field;
     return align(off, 8);
 }
 
-size_t batch_affine_fr_scratch(size_t n, size_t window, size_t batch) {
+size_t batch_affine_fr_scratch(size_t n, size_t window, size_t batch, const FrLayout& G = kFrG1) {
     if (n == 0 || n > 0xffffffffULL) return 0;
     if (batch != 128 && batch != 384) return 0;
-    const size_t alg = batch_affine_algorithm_scratch(window, batch);
+    const size_t alg = batch_affine_algorithm_scratch(window, batch, G);
     if (alg == 0) return 0;
     size_t scalar_bytes, total;
     if (__builtin_mul_overflow(n, (size_t)32, &scalar_bytes)) return 0;
@@ -1581,10 +1602,11 @@ size_t batch_affine_fr_scratch(size_t n, size_t window, size_t batch) {
     return total;
 }
 
-// the reference's algorithm choice for blst_p1s_mult_fr: batch-affine from
-// 2^12 to 2^18 points (window by size, batch 384 from 2^18), else Pippenger
-bool msm_fr_batch_plan(size_t n, size_t& window, size_t& batch) {
-    if (n < ((size_t)1 << 12) || n > ((size_t)1 << 18)) return false;
+// the reference's algorithm choice for blst_p1s_mult_fr / blst_p2s_mult_fr:
+// batch-affine from 2^12 points to the group's measured maximum (2^18 for
+// G1, 2^14 for G2; window by size, batch 384 from 2^18), else Pippenger
+bool msm_fr_batch_plan(size_t n, size_t& window, size_t& batch, const FrLayout& G = kFrG1) {
+    if (n < ((size_t)1 << 12) || n > G.batch_affine_max) return false;
     if (n <= ((size_t)1 << 12)) window = 10;
     else if (n <= ((size_t)1 << 14)) window = 11;
     else if (n <= ((size_t)1 << 15)) window = 12;
@@ -1595,22 +1617,22 @@ bool msm_fr_batch_plan(size_t n, size_t& window, size_t& batch) {
     return true;
 }
 
-size_t msm_fr_scratch(size_t max_points) {
+size_t msm_fr_scratch(size_t max_points, const FrLayout& G = kFrG1) {
     if (max_points == 0) return 0;
-    size_t required = pippenger_fr_scratch(max_points);
+    size_t required = pippenger_fr_scratch(max_points, G);
     if (required == 0) return 0;
     size_t window, batch;
-    if (msm_fr_batch_plan(max_points, window, batch)) {
-        const size_t b = batch_affine_fr_scratch(max_points, window, batch);
+    if (msm_fr_batch_plan(max_points, window, batch, G)) {
+        const size_t b = batch_affine_fr_scratch(max_points, window, batch, G);
         if (b == 0) return 0;
         required = std::max(required, b);
     }
     // a capacity query: storage for max_points also runs every smaller
     // input, including the largest batch-affine plan
-    const size_t measured_max = (size_t)1 << 18;
+    const size_t measured_max = G.batch_affine_max;
     if (max_points > measured_max) {
-        msm_fr_batch_plan(measured_max, window, batch);
-        const size_t b = batch_affine_fr_scratch(measured_max, window, batch);
+        msm_fr_batch_plan(measured_max, window, batch, G);
+        const size_t b = batch_affine_fr_scratch(measured_max, window, batch, G);
         if (b == 0) return 0;
         required = std::max(required, b);
     }
@@ -1635,8 +1657,26 @@ bool fr_msm(blst_p1* ret, const blst_p1_affine points[], const void* fr_scalars,
     return true;
 }
 
-// checks of c_api_pippenger_fr / c_api_msm_fr (c_api.cpp:532-552, 809-828)
-bool fr_entry(blst_p1* ret, const blst_p1_affine points[], const void* scalars, size_t n, blst_limb_t* scratch,
+bool fr_msm(blst_p2* ret, const blst_p2_affine points[], const void* fr_scalars, size_t n) {
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        blst_p2 r;
+        host_buffers_msm_g2(cx, reinterpret_cast<const uint8_t*>(points), static_cast<const uint8_t*>(fr_scalars), n,
+                            255, 0, true, true, 0, &r);
+        *ret = r;
+    } catch (...) {
+        g_err = current_error();
+        return false;
+    }
+    g_err.clear();
+    return true;
+}
+
+// checks of c_api_pippenger_fr / c_api_msm_fr (c_api.cpp:532-552, 809-828),
+// the same for both groups
+template <class Ret, class Aff>
+bool fr_entry(Ret* ret, const Aff points[], const void* scalars, size_t n, blst_limb_t* scratch,
               size_t scratch_size, size_t required) {
     if (ret == nullptr) {
         g_err = "ret is NULL";
@@ -1740,6 +1780,20 @@ size_t blst_p2s_mult_pippenger_scratch_sizeof(size_t npoints) {
 size_t blst_p2s_mult_pippenger_window_scratch_sizeof(size_t window) {
     if (window < 1 || window > 24) return 0;
     return (size_t)384 << (window - 1);
+}
+
+size_t blst_p2s_mult_pippenger_fr_scratch_sizeof(size_t npoints) { return pippenger_fr_scratch(npoints, kFrG2); }
+
+bool blst_p2s_mult_pippenger_fr(blst_p2* ret, const blst_p2_affine points[], const blst_fr scalars[], size_t npoints,
+                                blst_limb_t* scratch, size_t scratch_size) {
+    return fr_entry(ret, points, scalars, npoints, scratch, scratch_size, pippenger_fr_scratch(npoints, kFrG2));
+}
+
+size_t blst_p2s_mult_fr_scratch_sizeof(size_t max_points) { return msm_fr_scratch(max_points, kFrG2); }
+
+bool blst_p2s_mult_fr(blst_p2* ret, const blst_p2_affine points[], const blst_fr scalars[], size_t npoints,
+                      blst_limb_t* scratch, size_t scratch_size) {
+    return fr_entry(ret, points, scalars, npoints, scratch, scratch_size, msm_fr_scratch(npoints, kFrG2));
 }
 
 void blst_p2s_mult_pippenger(blst_p2* ret, const blst_p2_affine* const points[], size_t npoints,

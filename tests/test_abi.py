@@ -146,6 +146,31 @@ def test_fr_scratch_sizes_match_oracle(orc):
         prev = cur
 
 
+def test_g2_fr_scratch_sizes_match_oracle(orc):
+    import blst_hip
+    for n in [0, 1, 2, 3, 5, 64, 4095, 4096, 4097, 16383, 16384, 16385, 1 << 17, 1 << 20, (1 << 32) + 1, 1 << 57]:
+        assert blst_hip.p2_fr_scratch_sizeof(n, "pippenger") == orc.p2_pippenger_fr_scratch_sizeof(n), n
+        assert blst_hip.p2_fr_scratch_sizeof(n, "msm") == orc.p2_msm_fr_scratch_sizeof(n), n
+
+
+def test_g2_fr_fail_closed_without_gpu_work():
+    """The G2 blst_fr entries keep the checks of c_api_pippenger_fr /
+    c_api_msm_fr (c_api.cpp:532-552, 809-828): false with ret untouched, or
+    true with infinity for no points."""
+    import blst_hip
+    pts = np.ones((4, 24), np.uint64)
+    fr = np.ones((4, 4), np.uint64)
+    for entry in ("pippenger", "msm"):
+        for kw in ({"null": ("ret",)}, {"null": ("points",)}, {"null": ("scalars",)}, {"null": ("scratch",)},
+                   {"scratch_size": blst_hip.p2_fr_scratch_sizeof(4, entry) - 1}, {"scratch_size": 0}):
+            ret = np.full(36, 7, np.uint64)
+            ok, out = blst_hip.p2s_mult_fr(pts, fr, entry, ret=ret, **kw)
+            assert not ok and (out == 7).all(), (entry, kw)
+        ret = np.full(36, 7, np.uint64)
+        ok, out = blst_hip.p2s_mult_fr(pts[:0], fr[:0], entry, ret=ret, null=("points", "scalars", "scratch"))
+        assert ok and not out.any()
+
+
 def test_fr_fail_closed_without_gpu_work():
     """Argument checks of c_api_pippenger_fr / c_api_msm_fr
     (c_api.cpp:532-552, 809-828) that return before any device call: false

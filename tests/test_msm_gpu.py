@@ -841,6 +841,59 @@ def test_g2_host_buffers_in_point_ranges(hip, orc, n):
     assert orc.p2_compress(hip.p2s_mult_pippenger(pts, sc, 255)) == orc.p2_compress(want)
 
 
+@pytest.mark.parametrize("entry", FR_ENTRIES)
+def test_g2_fr_known_answer_ten_g(hip, orc, entry):
+    """Scalars 1..4 in Montgomery form on four copies of the G2 generator
+    give [10]G2; a scratch one byte short is refused with ret untouched
+    (BindingHeader_tests.cpp:642-690 on G2)."""
+    g = orc.p2_generator_affine()
+    pts = np.ascontiguousarray(np.tile(g, (4, 1)))
+    fr = orc.fr_from_values([1, 2, 3, 4])
+    ok, ret = hip.p2s_mult_fr(pts, fr, entry)
+    assert ok
+    want = orc.p2_mult(orc.p2_from_affine(g), (10).to_bytes(1, "little"), 8)
+    assert orc.p2_compress(ret) == orc.p2_compress(want)
+    short = np.full(36, 7, np.uint64)
+    ok, out = hip.p2s_mult_fr(pts, fr, entry, scratch_size=hip.p2_fr_scratch_sizeof(4, entry) - 1, ret=short)
+    assert not ok and (out == 7).all()
+
+
+@pytest.mark.parametrize("entry", FR_ENTRIES)
+@pytest.mark.parametrize("n", [1, 3, 100, 2048])
+def test_g2_fr_random_matches_oracle(hip, orc, g2_points, entry, n):
+    """Random Montgomery-form scalars with 0, one, r - 1, r, 2^256 - 1 among
+    them, against the oracle's conversion and G2 MSM."""
+    rnd = random.Random(n * 11 + len(entry))
+    pts = np.ascontiguousarray(g2_points[:n])
+    vals = [rnd.randrange(R) for _ in range(n)]
+    for k, v in enumerate([0, 1, R - 1]):
+        if k < n:
+            vals[k] = v
+    fr = orc.fr_from_values(vals)
+    if n > 8:
+        fr[5] = np.uint64(0xFFFFFFFFFFFFFFFF)
+        fr[6] = orc.limbs(R, 4)
+    want = orc.p2_compress(orc.msm_p2_pippenger(pts, orc.fr_to_scalars(fr), n, 255))
+    ok, ret = hip.p2s_mult_fr(pts, fr, entry)
+    assert ok and orc.p2_compress(ret) == want
+
+
+def test_g2_fr_capacity_scratch_2pow16(hip, orc):
+    """At 2^16 points (past the reference's G2 batch-affine range) the
+    blst_p2s_mult_fr entry with a capacity-sized scratch for 2^17 equals the
+    byte-scalar G2 entry on the oracle-converted scalars."""
+    n = 1 << 16
+    pts = hip.synth_p2_points(n, 0x77)
+    rng = np.random.default_rng(8)
+    fr = rng.integers(0, 1 << 63, size=(n, 4), dtype=np.uint64)
+    fr[:, 3] &= np.uint64(0x0FFFFFFFFFFFFFFF)
+    fr[::1024, 3] |= np.uint64(0xF000000000000000)
+    sc = orc.fr_to_scalars(fr)
+    ok, got = hip.p2s_mult_fr(pts, fr, "msm", scratch_size=hip.p2_fr_scratch_sizeof(1 << 17, "msm"))
+    assert ok
+    assert orc.p2_compress(got) == orc.p2_compress(hip.p2s_mult_pippenger(pts, sc, 255))
+
+
 def test_g2_hash_to_curve_points_textbook(hip):
     """The 10 G2 hash-to-curve outputs as bases with small scalars, against
     the textbook Fp2 model."""

@@ -343,6 +343,59 @@ def test_fr_scratch_sizes(orc):
         prev = cur
 
 
+def test_g2_fr_scratch_sizes(orc):
+    """The G2 blst_fr sizes (c_api.cpp:1976-2027 over the templates of
+    :272-299, 593-627, 700-807): the 384-byte Pointxyzz<Fp2> buckets of the
+    plain G2 entry (twice the G1 buckets pinned above), the batch-affine
+    element sizes C's layout rules give the Fp2 structs of
+    multi_scalar_batch_affine.cpp:16-36 (computed by ctypes), and the G2
+    planner's batch-affine range of 2^12 .. 2^14 points."""
+    import ctypes
+    assert orc.p2_pippenger_fr_scratch_sizeof(0) == 0
+    assert orc.p2_pippenger_fr_scratch_sizeof(1) == 32
+    assert orc.p2_msm_fr_scratch_sizeof(0) == 0
+    assert orc.p2_msm_fr_scratch_sizeof(1) == 32
+    for n in (2, 3, 5, 64, 4096, 1 << 16, 1 << 20):
+        assert orc.p2_pippenger_fr_scratch_sizeof(n) == 32 * n + 2 * orc.scratch_sizeof(n), n
+
+    class AffBucket(ctypes.Structure):
+        _fields_ = [("x", ctypes.c_uint64 * 12), ("y", ctypes.c_uint64 * 12),
+                    ("generation", ctypes.c_uint32), ("queued_epoch", ctypes.c_uint32)]
+
+    class ProjBucket(ctypes.Structure):
+        _fields_ = [("point", ctypes.c_uint64 * 36), ("generation", ctypes.c_uint32)]
+
+    assert (ctypes.sizeof(AffBucket), ctypes.sizeof(ProjBucket)) == (200, 296)
+
+    def align(v, a):
+        return -(-v // a) * a
+
+    def ba(window, batch):
+        B = 1 << (window - 1)
+        off = B * ctypes.sizeof(AffBucket)
+        off = align(off, ctypes.alignment(ProjBucket)) + B * ctypes.sizeof(ProjBucket)
+        off = align(off, 4) + batch * 12
+        off = align(off, 8) + batch * 96
+        off = align(off, 8) + batch * 96
+        return align(off, 8)
+
+    for n, w, b in ((4096, 10, 128), (2, 1, 128), (5000, 11, 384), (7, 24, 128)):
+        assert orc.p2_batch_affine_fr_scratch_sizeof(n, w, b) == 32 * n + ba(w, b)
+    for bad in ((0, 10, 128), (5, 0, 128), (5, 25, 128), (5, 10, 256)):
+        assert orc.p2_batch_affine_fr_scratch_sizeof(*bad) == 0
+    assert orc.p2_msm_fr_scratch_sizeof(4095) == orc.p2_pippenger_fr_scratch_sizeof(4095)
+    assert orc.p2_msm_fr_scratch_sizeof(4096) == max(orc.p2_pippenger_fr_scratch_sizeof(4096), 32 * 4096 + ba(10, 128))
+    top = 32 * (1 << 14) + ba(11, 128)
+    assert orc.p2_msm_fr_scratch_sizeof(1 << 14) == max(orc.p2_pippenger_fr_scratch_sizeof(1 << 14), top)
+    # past 2^14 the plan is Pippenger, the capacity never below the 2^14 plan
+    assert orc.p2_msm_fr_scratch_sizeof((1 << 14) + 1) == max(orc.p2_pippenger_fr_scratch_sizeof((1 << 14) + 1), top)
+    prev = 0
+    for n in list(range(1, 5000)) + list(range(5000, (1 << 18), 97)) + [1 << 20]:
+        cur = orc.p2_msm_fr_scratch_sizeof(n)
+        assert cur >= prev and cur >= orc.p2_pippenger_fr_scratch_sizeof(n), n
+        prev = cur
+
+
 def test_batch_to_affine_against_textbook(orc):
     """points_to_affine (multi_scalar.cpp:160-238) against x = X/Z^2,
     y = Y/Z^3 in Python big integers, with Z = 0 points (all-zero output)
