@@ -25,6 +25,20 @@ extern "C" {
  * device error. */
 int blst_hip_fp_ops(int op, void *out, const void *a, const void *b, const void *c, const void *d, size_t n);
 
+/* One device operation of the 28-bit limb field of the G1 bucket
+ * accumulation over n rows of 14 32-bit words (limbs of 28 bits; value
+ * sum l_i 2^(28 i), Montgomery radix 2^392): op 0 a*b, 1 a^2, 2 a*b + c*d,
+ * 3 a reduced to [0, 2p), 4 a - b + 4p (lazy), 5 4p - a, 6 the X3 step
+ * a - (b + 2c) + 6p reduced (a - 2c + 6p when bit 0 of d's first limb is
+ * set), 7 whether a is zero modulo p (1 or 0 in the first word), 8, 9, 10
+ * the limbs of w 2^8, w 2^4, w where w is the 384-bit number in a's first
+ * twelve words, 11 a's value as twelve 32-bit words (then two zero words),
+ * 12 4a reduced.  Inputs must respect each operation's stated limb and value
+ * bounds (fq.hip.h).  All four inputs are read (pass a for unused ones).
+ * Returns 0 on success, -1 on invalid arguments or a device error. */
+int blst_hip_fq_ops(int op, uint32_t *out, const uint32_t *a, const uint32_t *b, const uint32_t *c,
+                    const uint32_t *d, size_t n);
+
 /* Measured issue rate of v_mad_u64_u32 (32 x 32 -> 64-bit multiply-add with
  * a 64-bit addend, the instruction of every Montgomery product) on the
  * current device: eight independent accumulators per lane, eight waves per

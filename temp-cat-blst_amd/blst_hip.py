@@ -79,6 +79,7 @@ SIGNATURES = {
 
 TESTKIT_SIGNATURES = {
     "blst_hip_fp_ops": ([ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _sz], ctypes.c_int),
+    "blst_hip_fq_ops": ([ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _sz], ctypes.c_int),
     "blst_hip_set_alloc_cap": ([_sz], None),
     "blst_hip_mad_issue_rate": ([_vp], ctypes.c_int),
     "blst_hip_batch_affine_probe": ([_vp, _vp, _sz, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
@@ -448,6 +449,25 @@ def fp_ops(op, a, b=None, c=None, d=None):
     rc = testkit().blst_hip_fp_ops(FP_OPS[op], _ptr(out), _ptr(A), _ptr(B), _ptr(C), _ptr(D), n)
     if rc:
         raise RuntimeError("blst_hip_fp_ops: " + last_error())
+    return out
+
+
+FQ_OPS = {"mul": 0, "sqr": 1, "mul_sum": 2, "reduce": 3, "sub_lazy": 4, "neg_lazy": 5, "x3": 6, "is_zero": 7,
+          "from_words8": 8, "from_words4": 9, "from_words0": 10, "to_words": 11, "times4": 12}
+
+
+def fq_ops(op, a, b=None, c=None, d=None):
+    """One device operation of the G1 accumulation's 28-bit limb field over
+    arrays of (n, 14) uint32 limbs: returns (n, 14) uint32."""
+    A = np.ascontiguousarray(a, dtype=np.uint32).reshape(-1, 14)
+    n = A.shape[0]
+    B, C, D = (np.ascontiguousarray(x, dtype=np.uint32).reshape(-1, 14) if x is not None else A for x in (b, c, d))
+    for X in (B, C, D):
+        assert X.shape[0] == n
+    out = np.zeros((n, 14), np.uint32)
+    rc = testkit().blst_hip_fq_ops(FQ_OPS[op], _ptr(out), _ptr(A), _ptr(B), _ptr(C), _ptr(D), n)
+    if rc:
+        raise RuntimeError("blst_hip_fq_ops: " + last_error())
     return out
 
 

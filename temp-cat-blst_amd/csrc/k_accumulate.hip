@@ -1,9 +1,60 @@
 // Bucket accumulation: chunks of the sorted list per lane, XYZZ mixed
 // additions of gathered affine points.
 #include "msm_common.hip.h"
+#include "fq.hip.h"
 #include "launch.h"
 
 namespace bls {
+
+// The running bucket sum of one lane.  G2 sums in the XYZZ form of
+// ec.hip.h.  G1 sums in fq (28-bit limbs, no carry folds in its products)
+// and converts where a bucket sum is loaded or stored: the stored sums keep
+// the 32-bit layout that the merge and reduction kernels read.
+template <class F>
+struct bucket_sum {
+    xyzz_t<F> a;
+    __device__ __forceinline__ void set_inf() { xyzz_set_inf(a); }
+    __device__ __forceinline__ void load(const xyzz_t<F>* __restrict__ src) { load_xyzz(a, src); }
+    __device__ __forceinline__ void store(xyzz_t<F>* __restrict__ dst) const { store_xyzz(dst, a); }
+    // point v & 0x7fffffff of pts, negated when bit 31 is set
+    __device__ __forceinline__ void add(const uint4* __restrict__ pts, uint32_t v) {
+        aff_t<F> q;
+        load_aff(q, pts, v & 0x7fffffffu);
+        // an all-zero affine point is infinity and contributes nothing
+        // (Pointxyzz::dadd with an all-zero affine input, point.cpp:2036-2039)
+        if (!aff_is_inf(q)) {
+            if (v >> 31) fp_neg(q.y, q.y);
+            xyzz_madd(a, q);
+        }
+    }
+};
+
+template <>
+struct bucket_sum<fp> {
+    fq_xyzz a;
+    __device__ __forceinline__ void set_inf() { fq_xyzz_set_inf(a); }
+    __device__ __forceinline__ void load(const xyzz* __restrict__ src) {
+        xyzz w;
+        load_xyzz(w, src);
+        fq_xyzz_from_words(a, w.x.l, w.y.l, w.zz.l, w.zzz.l);
+    }
+    __device__ __forceinline__ void store(xyzz* __restrict__ dst) const {
+        xyzz w;
+        fq_xyzz_to_words(w.x.l, w.y.l, w.zz.l, w.zzz.l, a);
+        store_xyzz(dst, w);
+    }
+    __device__ __forceinline__ void add(const uint4* __restrict__ pts, uint32_t v) {
+        aff q;
+        load_aff(q, pts, v & 0x7fffffffu);
+        if (!aff_is_inf(q)) {
+            if (v >> 31) fp_neg(q.y, q.y);   // 2p - y: below 2p, as fq_xyzz_madd takes it
+            fq x, y16;
+            fq_from_words<8>(x, q.x.l);
+            fq_from_words<4>(y16, q.y.l);
+            fq_xyzz_madd(a, x, y16);
+        }
+    }
+};
 
 // One lane per chunk of T sorted entries.  A bucket segment that starts and
 // ends inside the chunk is final and goes to buckets[key].  Otherwise, with
@@ -14,8 +65,13 @@ namespace bls {
 // With prev (the entry counts of earlier point ranges of the same MSM), a
 // bucket that already holds a sum continues from it: the lane holding the
 // bucket's first entry starts from buckets[key] instead of infinity.
+// waves per SIMD the kernel is compiled for: three for G1 (at most 168
+// VGPRs), what the register use allows for G2
+template <class F> constexpr int kAccumulateBlocksPerCU = 1;
+template <> constexpr int kAccumulateBlocksPerCU<fp> = 3;
+
 template <class F>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, kAccumulateBlocksPerCU<F>)
 k_accumulate(const uint2* __restrict__ sorted,
              const uint32_t* __restrict__ total, const job_t* __restrict__ jobs,
              const uint32_t* __restrict__ kbase, uint32_t njobs, xyzz_t<F>* __restrict__ buckets,
@@ -36,9 +92,9 @@ k_accumulate(const uint2* __restrict__ sorted,
     const uint4* pts = jobs[j].points;
     const bool before = start > 0 && sorted[start - 1].x == cur;
     bool first = true;
-    xyzz_t<F> acc;
-    if (prev && !before && prev[cur]) load_xyzz(acc, &buckets[cur]);
-    else xyzz_set_inf(acc);
+    bucket_sum<F> acc;
+    if (prev && !before && prev[cur]) acc.load(&buckets[cur]);
+    else acc.set_inf();
     #pragma unroll 1
     for (uint32_t e = start; e < end; e++) {
         const uint2 kv = sorted[e];
@@ -46,41 +102,33 @@ k_accumulate(const uint2* __restrict__ sorted,
         if (key != cur) {
             if (first && before) {
                 slot_key[2 * chunk] = cur;
-                store_xyzz(&slots[2 * chunk], acc);
+                acc.store(&slots[2 * chunk]);
             } else {
-                store_xyzz(&buckets[cur], acc);
+                acc.store(&buckets[cur]);
             }
             first = false;
             cur = key;
-            if (prev && prev[key]) load_xyzz(acc, &buckets[key]);
-            else xyzz_set_inf(acc);
+            if (prev && prev[key]) acc.load(&buckets[key]);
+            else acc.set_inf();
             if (key >= next_kb) {
                 while (j + 1 < njobs && kbase[j + 1] <= key) j++;
                 next_kb = (j + 1 < njobs) ? kbase[j + 1] : 0xffffffffu;
                 pts = jobs[j].points;
             }
         }
-        const uint32_t v = kv.y;
-        aff_t<F> a;
-        load_aff(a, pts, v & 0x7fffffffu);
-        // an all-zero affine point is infinity and contributes nothing
-        // (Pointxyzz::dadd with an all-zero affine input, point.cpp:2036-2039)
-        if (!aff_is_inf(a)) {
-            if (v >> 31) fp_neg(a.y, a.y);
-            xyzz_madd(acc, a);
-        }
+        acc.add(pts, kv.y);
     }
     const bool after = end < M && sorted[end].x == cur;
     if (first && before) {
         slot_key[2 * chunk] = cur;
-        store_xyzz(&slots[2 * chunk], acc);
+        acc.store(&slots[2 * chunk]);
     } else if (after) {
         slot_key[2 * chunk + 1] = cur;
-        store_xyzz(&slots[2 * chunk + 1], acc);
+        acc.store(&slots[2 * chunk + 1]);
         const uint32_t m = (off[cur] + cnt[cur] - 1) / T - chunk + 1;   // pieces
         if (m > 2) list_bucket(lists, m, cur);
     } else {
-        store_xyzz(&buckets[cur], acc);
+        acc.store(&buckets[cur]);
     }
 }
 

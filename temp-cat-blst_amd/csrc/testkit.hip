@@ -4,6 +4,9 @@
 //   * blst_hip_fp_ops: one device field operation (fp.hip.h) applied over
 //     arrays, so the [0, 2p) redundant field layer can be compared with exact
 //     big-integer arithmetic on chosen inputs (p, 2p - 1, R mod p, ...);
+//   * blst_hip_fq_ops: the same for the 28-bit limb field of the G1 bucket
+//     accumulation (fq.hip.h), including its lazy values and its
+//     conversions from and to the 32-bit word image;
 //   * blst_hip_booth_digits: the signed window digits as the sort kernels
 //     read them;
 //   * blst_hip_point_ops_check: the four-lane point formulas against the
@@ -12,6 +15,7 @@
 //     (the seeded scalar generators are in synth.cpp).
 #include "pipeline.h"
 #include "ec_quad.hip.h"
+#include "fq.hip.h"
 #include "../../include/blst_hip_testkit.h"
 
 #include <cstring>
@@ -71,6 +75,45 @@ k_fp_ops(uint32_t op, uint4* __restrict__ out, const uint4* __restrict__ A, cons
     default: fp_inv(r, a); break;
     }
     store_fp(out, i, r);
+}
+
+// operations of the 28-bit limb field, selected by op; rows of 14 words
+enum : uint32_t {
+    kFqMul = 0, kFqSqr, kFqMulSum, kFqReduce, kFqSubLazy, kFqNegLazy, kFqX3, kFqIsZero, kFqFromWords8, kFqFromWords4,
+    kFqFromWords0, kFqToWords, kFqTimes4, kFqCount
+};
+
+__global__ void __launch_bounds__(256)
+k_fq_ops(uint32_t op, uint32_t* __restrict__ out, const uint32_t* __restrict__ A, const uint32_t* __restrict__ B,
+         const uint32_t* __restrict__ C, const uint32_t* __restrict__ D, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    fq a, b, c, d, r;
+#pragma unroll
+    for (int k = 0; k < 14; k++) {
+        a.l[k] = A[14 * i + k];
+        b.l[k] = B[14 * i + k];
+        c.l[k] = C[14 * i + k];
+        d.l[k] = D[14 * i + k];
+    }
+    fq_zero(r);
+    switch (op) {
+    case kFqMul: fq_mul(r, a, b); break;
+    case kFqSqr: fq_sqr(r, a); break;
+    case kFqMulSum: fq_mul_sum(r, a, b, c, d); break;
+    case kFqReduce: fq_reduce(r, a); break;
+    case kFqSubLazy: fq_sub_lazy(r, a, b); break;
+    case kFqNegLazy: fq_neg_lazy(r, a); break;
+    case kFqX3: fq_x3(r, a, b, c, (d.l[0] & 1) != 0); break;
+    case kFqIsZero: r.l[0] = fq_is_zero(a) ? 1u : 0u; break;
+    case kFqFromWords8: fq_from_words<8>(r, a.l); break;
+    case kFqFromWords4: fq_from_words<4>(r, a.l); break;
+    case kFqFromWords0: fq_from_words<0>(r, a.l); break;
+    case kFqToWords: fq_to_words(r.l, a); break;
+    default: fq_times4(r, a); break;
+    }
+#pragma unroll
+    for (int k = 0; k < 14; k++) out[14 * i + k] = r.l[k];
 }
 
 __global__ void k_digits_debug(const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits,
@@ -204,6 +247,40 @@ int blst_hip_fp_ops(int op, void* out, const void* a, const void* b, const void*
         hipLaunchKernelGGL(k_fp_ops, dim3(div_up(n, 256)), dim3(256), 0, k.stream, (uint32_t)op, k.out.as<uint4>(),
                            k.in[0].as<const uint4>(), k.in[1].as<const uint4>(), k.in[2].as<const uint4>(),
                            k.in[3].as<const uint4>(), (uint32_t)n);
+        HIPCK(hipGetLastError());
+        HIPCK(hipMemcpyAsync(out, k.out.p, bytes, hipMemcpyDeviceToHost, k.stream));
+        HIPCK(hipStreamSynchronize(k.stream));
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_fq_ops(int op, uint32_t* out, const uint32_t* a, const uint32_t* b, const uint32_t* c,
+                    const uint32_t* d, size_t n) {
+    if (n == 0) { g_err.clear(); return 0; }
+    if (op < 0 || op >= (int)kFqCount || !out || !a || !b || !c || !d || n >= (1u << 26)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Kit& k = kit();
+        std::lock_guard<std::mutex> lk(k.mu);
+        const size_t bytes = n * 56;
+        const void* src[4] = {a, b, c, d};
+        for (int j = 0; j < 4; j++) {
+            k.in[j].ensure(bytes);
+            HIPCK(hipMemcpyAsync(k.in[j].p, src[j], bytes, hipMemcpyHostToDevice, k.stream));
+        }
+        k.out.ensure(bytes);
+        hipLaunchKernelGGL(k_fq_ops, dim3(div_up(n, 256)), dim3(256), 0, k.stream, (uint32_t)op,
+                           k.out.as<uint32_t>(), k.in[0].as<const uint32_t>(), k.in[1].as<const uint32_t>(),
+                           k.in[2].as<const uint32_t>(), k.in[3].as<const uint32_t>(), (uint32_t)n);
         HIPCK(hipGetLastError());
         HIPCK(hipMemcpyAsync(out, k.out.p, bytes, hipMemcpyDeviceToHost, k.stream));
         HIPCK(hipStreamSynchronize(k.stream));
