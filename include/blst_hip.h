@@ -290,6 +290,13 @@ void blst_hip_tuning_get(int batch, size_t npoints, size_t nbits, size_t out[3])
  * from 2^17 points).  Returns 0, or -1 for groups > 16. */
 int blst_hip_set_pipeline(size_t groups);
 
+/* Share of the windows, in percent, given to the last (lowest) window group
+ * of a split MSM; the other groups share the rest equally.  The last group's
+ * merges, reduction and host fold follow every accumulation, so a smaller
+ * share shortens what nothing overlaps.  0 restores equal groups.  Returns 0,
+ * or -1 for percent > 50. */
+int blst_hip_set_pipeline_share(size_t percent);
+
 /* blst_p1s_to_affine on device memory: npoints Jacobian points (144 bytes
  * each) to affine (96 bytes each), both 16-byte aligned and not overlapping;
  * returns when done, 0 or -1 (npoints below 2^31 - 1). */

@@ -53,6 +53,7 @@ SIGNATURES = {
     "blst_hip_msm_device": ([_vp, _vp, _vp, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_msm_device_noalloc": ([_vp, _vp, _vp, _sz, _sz], ctypes.c_int),
     "blst_hip_set_pipeline": ([_sz], ctypes.c_int),
+    "blst_hip_set_pipeline_share": ([_sz], ctypes.c_int),
     "blst_hip_last_phase_ms": ([_vp], None),
     "blst_hip_device_upload": ([_vp, _sz], _vp),
     "blst_hip_device_free": ([_vp], None),
@@ -520,6 +521,12 @@ def set_pipeline(groups):
     """Window-group pipelines per lone MSM (0 = automatic)."""
     if load().blst_hip_set_pipeline(groups):
         raise RuntimeError("blst_hip_set_pipeline: " + last_error())
+
+
+def set_pipeline_share(percent):
+    """Percent of the windows in the last window group (0: equal groups)."""
+    if load().blst_hip_set_pipeline_share(percent):
+        raise RuntimeError("blst_hip_set_pipeline_share: " + last_error())
 
 
 def point_ops_check(d_points_ptr, n):

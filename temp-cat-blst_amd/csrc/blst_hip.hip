@@ -876,6 +876,23 @@ void finish_single(const JobSpec& js, const std::vector<hostec::xyzz6>& wins, bl
 // latency-bound merges and reduction of a group overlap the accumulation of
 // the next.  0 = automatic.
 std::atomic<uint32_t> g_pipeline{0};
+// percent of the windows in the last group (0: equal groups)
+std::atomic<uint32_t> g_last_share{0};
+
+// First window of group g of G over nwin windows; group 0 holds the top
+// windows and group G - 1 the lowest.  The last group's results are the ones
+// no other work hides (its merges, reduction and host fold run after every
+// accumulation), so it may be given fewer windows than the others.
+uint32_t group_win0(uint32_t g, uint32_t G, uint32_t nwin) {
+    if (g + 1 >= G) return 0;
+    const uint32_t pct = g_last_share.load();
+    uint32_t last = pct ? (nwin * pct + 50) / 100 : nwin / G;
+    last = std::max<uint32_t>(1, std::min<uint32_t>(last, nwin - (G - 1)));
+    const uint32_t rest = nwin - last;
+    return last + (G - 2 - g) * rest / (G - 1);
+}
+
+uint32_t group_win_end(uint32_t g, uint32_t G, uint32_t nwin) { return g == 0 ? nwin : group_win0(g - 1, G, nwin); }
 
 uint32_t pipeline_groups(const JobSpec& js) {
     if (js.tile) return 1;
@@ -896,8 +913,8 @@ void fold_groups(Ctx& cx, uint32_t G, uint32_t nwin, uint32_t c, const std::vect
     hostec::set_inf(acc);
     std::chrono::steady_clock::time_point t0;
     for (uint32_t g = 0; g < G; g++) {
-        const uint32_t base = (G - 1 - g) * nwin / G;
-        if (g > 0) hostec::scale_pow2(acc, acc, (int)(c * ((G - g) * nwin / G - base)));
+        const uint32_t base = group_win0(g, G, nwin);
+        if (g > 0) hostec::scale_pow2(acc, acc, (int)(c * (group_win_end(g, G, nwin) - base)));
         HIPCK(hipStreamSynchronize(cx.pipe[g]->stream));
         if (g + 1 == G) t0 = std::chrono::steady_clock::now();
         hostec::xyzz6 part;
@@ -937,8 +954,8 @@ void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
     // and the host folds it while the lower groups still run
     for (uint32_t g = 0; g < G; g++) {
         JobSpec s = js;
-        s.win0 = (G - 1 - g) * nwin / G;
-        s.nwin = (G - g) * nwin / G - s.win0;
+        s.win0 = group_win0(g, G, nwin);
+        s.nwin = group_win_end(g, G, nwin) - s.win0;
         Ls[g] = make_layout(std::vector<JobSpec>(1, s));
         Workspace& w = *cx.pipe[g];
         ensure_workspace(w, Ls[g]);
@@ -1000,8 +1017,8 @@ void ranged_msm(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stage&&
             s.scalars = ds + lo[r] * nbytes;
             s.n = (uint32_t)(lo[r + 1] - lo[r]);
             s.T = make_spec(s.points, s.scalars, s.n, full.nbits, full.c, false).T;
-            s.win0 = (G - 1 - g) * nwin / G;
-            s.nwin = (G - g) * nwin / G - s.win0;
+            s.win0 = group_win0(g, G, nwin);
+            s.nwin = group_win_end(g, G, nwin) - s.win0;
             Layout& L = Ls[g * K + r];
             L = make_layout(std::vector<JobSpec>(1, s));
             ensure_workspace(w, L);
@@ -2473,6 +2490,16 @@ int blst_hip_set_pipeline(size_t groups) {
         return -1;
     }
     g_pipeline.store((uint32_t)groups);
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_set_pipeline_share(size_t percent) {
+    if (percent > 50) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    g_last_share.store((uint32_t)percent);
     g_err.clear();
     return 0;
 }

@@ -340,3 +340,13 @@ def test_tuning_rows_by_scalar_width():
         for nbits in (8, 32, 160):
             c, T, L = blst_hip.tuning_get(0, n, nbits)
             assert 1 <= c <= 24 and T >= 1 and L >= 1
+
+
+def test_pipeline_settings_range():
+    import blst_hip
+    L = blst_hip.load()
+    assert L.blst_hip_set_pipeline(17) == -1 and blst_hip.last_error()
+    assert L.blst_hip_set_pipeline_share(51) == -1 and blst_hip.last_error()
+    for v in (0, 1, 25, 50):
+        assert L.blst_hip_set_pipeline_share(v) == 0
+    assert L.blst_hip_set_pipeline(0) == 0 and L.blst_hip_set_pipeline_share(0) == 0

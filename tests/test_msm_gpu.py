@@ -160,8 +160,14 @@ def test_window_group_pipelines(hip, orc, base_points):
         for groups in (1, 2, 3, 4, 0):
             hip.set_pipeline(groups)
             assert hip.compress(hip.msm_device(dp.ptr, ds.ptr, n, 255)) == want, groups
+        # unequal groups: the last one with 1, 4 and 7 of the 16 windows
+        for groups, share in ((2, 6), (2, 25), (2, 44), (3, 25)):
+            hip.set_pipeline(groups)
+            hip.set_pipeline_share(share)
+            assert hip.compress(hip.msm_device(dp.ptr, ds.ptr, n, 255)) == want, (groups, share)
     finally:
         hip.set_pipeline(0)
+        hip.set_pipeline_share(0)
         dp.close()
         ds.close()
 
