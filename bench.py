@@ -292,6 +292,31 @@ def ba_probe(npairs=1 << 25):
     print(json.dumps({"batch_affine_probe": {"pairs": npairs, "rows": rows}}))
 
 
+def gather_probe(n=1 << 20, windows=16):
+    """The accumulation's point gathers at the bench workload's shape (n
+    points, each read once per window in a random order: n * windows entries)
+    from the 96-byte blst_p1_affine array and from a copy padded to 128-byte
+    lines, summed with the accumulation's arithmetic (test-kit probe kernel;
+    the sums are checked in tests/test_ba_probe_gpu.py)."""
+    import blst_hip as bh
+
+    pts = bh.synth_points(n, SEED ^ 1)
+    pad = np.zeros((n, 16), np.uint64)
+    pad[:, :12] = pts
+    rng = np.random.default_rng(SEED & 0xFFFFFFFF)
+    idx = np.concatenate([rng.permutation(n) for _ in range(windows)]).astype(np.uint32)
+    d_p, d_q, d_i = bh.DeviceBuffer(pts), bh.DeviceBuffer(pad), bh.DeviceBuffer(idx)
+    try:
+        ns96, s96 = bh.gather_probe(d_p.ptr, 96, d_i.ptr, idx.size)
+        ns128, s128 = bh.gather_probe(d_q.ptr, 128, d_i.ptr, idx.size)
+    finally:
+        d_p.close()
+        d_q.close()
+        d_i.close()
+    print(json.dumps({"gather_probe": {"points": n, "entries": int(idx.size), "ns_per_entry_96B": ns96,
+                                       "ns_per_entry_128B": ns128, "same_sums": bool(np.array_equal(s96, s128))}}))
+
+
 def copy_calibration(nbytes, reps=3):
     """Device-to-device copies of a known size: the HIP runtime's blit kernel
     reads and writes nbytes each, so FETCH_SIZE / WRITE_SIZE of these
@@ -331,6 +356,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--ba-probe", action="store_true",
                     help="only run the batch-affine against XYZZ addition probe")
+    ap.add_argument("--gather-probe", action="store_true",
+                    help="only run the point-gather probe at 96- and 128-byte point strides")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -339,6 +366,9 @@ def main():
         return
     if args.ba_probe:
         ba_probe()
+        return
+    if args.gather_probe:
+        gather_probe()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the MSM backend runs on a single device "

@@ -60,6 +60,16 @@ int blst_hip_batch_affine_probe(const void *device_points, const uint32_t *devic
                                 int pairs_per_lane, double ns_per_pair[2], void *device_affine_out,
                                 void *device_xyzz_out);
 
+/* Gather probe of the bucket accumulation's memory side: lane g of
+ * nentries / 32 lanes sums the affine points device_index[k L + g], k < 32
+ * (L lanes), with the accumulation's arithmetic, reading the points at a
+ * stride of stride_bytes (96: a blst_p1_affine array; 128: each point padded
+ * to one 128-byte line), and writes the canonical XYZZ sums (192 bytes each)
+ * to device_xyzz_out.  nentries must be a multiple of 8192.  Writes the best
+ * of three timed launches in nanoseconds per entry.  Returns 0 or -1. */
+int blst_hip_gather_probe(const void *device_points, size_t stride_bytes, const uint32_t *device_index,
+                          size_t nentries, double *ns_per_entry, void *device_xyzz_out);
+
 /* Makes every later device allocation of the product library larger than
  * `bytes` fail as an out-of-memory device would (0 lifts the cap), so tests
  * can drive the fail-closed paths: infinity (or NULL / -1) and a message in

@@ -84,6 +84,7 @@ TESTKIT_SIGNATURES = {
     "blst_hip_set_alloc_cap": ([_sz], None),
     "blst_hip_mad_issue_rate": ([_vp], ctypes.c_int),
     "blst_hip_batch_affine_probe": ([_vp, _vp, _sz, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
+    "blst_hip_gather_probe": ([_vp, _sz, _vp, _sz, _vp, _vp], ctypes.c_int),
     "blst_hip_booth_digits": ([_vp, _vp, _sz, _sz, _sz, _sz], ctypes.c_int),
     "blst_hip_point_ops_check": ([_vp, _sz], ctypes.c_int),
     "blst_hip_p1s_mult_generator": ([_vp, _vp, _sz, _sz], ctypes.c_int),
@@ -470,6 +471,21 @@ def fq_ops(op, a, b=None, c=None, d=None):
     if rc:
         raise RuntimeError("blst_hip_fq_ops: " + last_error())
     return out
+
+
+def gather_probe(d_points_ptr, stride_bytes, d_index_ptr, nentries, read=True):
+    """The accumulation's gathers at a 96- or 128-byte point stride: returns
+    (ns per entry, (nentries // 32, 24) uint64 canonical XYZZ sums or None)."""
+    d_out = DeviceAlloc((nentries // 32) * 192)
+    try:
+        ns = np.zeros(1, np.float64)
+        rc = testkit().blst_hip_gather_probe(ctypes.c_void_p(d_points_ptr), stride_bytes, ctypes.c_void_p(d_index_ptr),
+                                             nentries, _ptr(ns), ctypes.c_void_p(d_out.ptr))
+        if rc:
+            raise RuntimeError("blst_hip_gather_probe: " + last_error())
+        return float(ns[0]), (d_out.read().view(np.uint64).reshape(-1, 24) if read else None)
+    finally:
+        d_out.close()
 
 
 def mad_issue_rate():

@@ -78,6 +78,13 @@ k_accumulate(const uint2* __restrict__ sorted,
              uint32_t* __restrict__ slot_key, xyzz_t<F>* __restrict__ slots, uint32_t T, uint32_t nchunks,
              const uint32_t* __restrict__ prev, const uint32_t* __restrict__ off, const uint32_t* __restrict__ cnt,
              merge_lists_t lists) {
+    // Each lane's next eight sorted entries (one 64-byte segment), loaded at
+    // once and read from LDS: read one by one from memory, a wave's 64 lanes
+    // touch 64 lines T entries apart, and the point gathers in between evict
+    // those lines before the next entry is read.  Slot q of lane l is
+    // stage[q][l], so the writes and reads of a wave fall on distinct banks.
+    __shared__ uint4 stage[4][256];
+    const bool staged = (T & 7) == 0;
     const uint32_t chunk = blockIdx.x * blockDim.x + threadIdx.x;
     if (chunk >= nchunks) return;
     slot_key[2 * chunk] = kInvalid;
@@ -97,7 +104,27 @@ k_accumulate(const uint2* __restrict__ sorted,
     else acc.set_inf();
     #pragma unroll 1
     for (uint32_t e = start; e < end; e++) {
-        const uint2 kv = sorted[e];
+        uint2 kv;
+        if (staged) {
+            const uint32_t i = (e - start) & 7;
+            if (i == 0) {
+                // start and e are multiples of 8: 64-byte aligned
+                const uint4* src = reinterpret_cast<const uint4*>(sorted + e);
+#pragma unroll
+                for (uint32_t q = 0; q < 4; q++) {
+                    if (e + 2 * q + 1 < end) {
+                        stage[q][threadIdx.x] = src[q];
+                    } else if (e + 2 * q < end) {
+                        const uint2 t = sorted[e + 2 * q];
+                        stage[q][threadIdx.x] = make_uint4(t.x, t.y, 0u, 0u);
+                    }
+                }
+            }
+            const uint4 v = stage[i >> 1][threadIdx.x];
+            kv = (i & 1) ? make_uint2(v.z, v.w) : make_uint2(v.x, v.y);
+        } else {
+            kv = sorted[e];
+        }
         const uint32_t key = kv.x;
         if (key != cur) {
             if (first && before) {

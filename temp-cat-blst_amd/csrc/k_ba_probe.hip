@@ -26,6 +26,7 @@
 // a bucket list).  Lanes take pairs j = k * G + g (G lanes in all) so that
 // the lanes of a wave read consecutive pairs.
 #include "msm_common.hip.h"
+#include "fq.hip.h"
 
 namespace bls {
 
@@ -216,6 +217,52 @@ k_madd_pairs(const uint4* __restrict__ pts, const uint32_t* __restrict__ idx, xy
         xyzz_canon(c, acc);
         store_xyzz(&out[j], c);
     }
+}
+
+// The memory side of the bucket accumulation: lane g folds the points
+// idx[k G + g], k < kGatherK, into one bucket sum with the accumulation
+// kernel's arithmetic (fq_xyzz_madd), reading them at a stride of S16 uint4
+// (6: the 96-byte blst_p1_affine array; 8: the same points padded to 128
+// bytes, one cache line each), and stores the sum's word image.
+constexpr uint32_t kGatherK = 32;
+
+template <int S16>
+__global__ void __launch_bounds__(kProbeThreads, 3)
+k_gather_sums(const uint4* __restrict__ pts, const uint32_t* __restrict__ idx, xyzz* __restrict__ out, uint32_t G) {
+    const uint32_t g = blockIdx.x * kProbeThreads + threadIdx.x;
+    fq_xyzz acc;
+    fq_xyzz_set_inf(acc);
+#pragma unroll 1
+    for (uint32_t k = 0; k < kGatherK; k++) {
+        const uint4* s = pts + (size_t)idx[k * G + g] * S16;
+        uint32_t w[24];
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const uint4 v = s[q];
+            w[4 * q] = v.x; w[4 * q + 1] = v.y; w[4 * q + 2] = v.z; w[4 * q + 3] = v.w;
+        }
+        uint32_t z = 0;
+#pragma unroll
+        for (int i = 0; i < 24; i++) z |= w[i];
+        if (z) {
+            fq x, y16;
+            fq_from_words<8>(x, w);
+            fq_from_words<4>(y16, w + 12);
+            fq_xyzz_madd(acc, x, y16);
+        }
+    }
+    xyzz o, c;
+    fq_xyzz_to_words(o.x.l, o.y.l, o.zz.l, o.zzz.l, acc);
+    xyzz_canon(c, o);
+    store_xyzz(&out[g], c);
+}
+
+void launch_gather_probe(hipStream_t stream, int stride_bytes, const uint4* pts, const uint32_t* idx, void* out,
+                         uint32_t nentries) {
+    const uint32_t G = nentries / kGatherK;
+    const dim3 grid(G / kProbeThreads), block(kProbeThreads);
+    if (stride_bytes == 128) hipLaunchKernelGGL(k_gather_sums<8>, grid, block, 0, stream, pts, idx, (xyzz*)out, G);
+    else hipLaunchKernelGGL(k_gather_sums<6>, grid, block, 0, stream, pts, idx, (xyzz*)out, G);
 }
 
 // lanes G = npairs / K; npairs a multiple of 256 K

@@ -158,6 +158,8 @@ k_mad_issue(uint64_t* __restrict__ out, uint32_t passes, uint32_t seed) {
 
 void launch_ba_probe(hipStream_t stream, int K, bool batch_affine, const uint4* pts, const uint32_t* idx, fp* pi,
                      void* out, uint32_t npairs);
+void launch_gather_probe(hipStream_t stream, int stride_bytes, const uint4* pts, const uint32_t* idx, void* out,
+                         uint32_t nentries);
 
 namespace host {
 namespace {
@@ -369,6 +371,43 @@ int blst_hip_batch_affine_probe(const void* device_points, const uint32_t* devic
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
+    } catch (const HipFail& f) {
+        g_err = f.msg;
+        return -1;
+    }
+    g_err.clear();
+    return 0;
+}
+
+int blst_hip_gather_probe(const void* device_points, size_t stride_bytes, const uint32_t* device_index,
+                          size_t nentries, double* ns_per_entry, void* device_xyzz_out) {
+    if (!device_points || !device_index || !ns_per_entry || !device_xyzz_out ||
+        (stride_bytes != 96 && stride_bytes != 128) || nentries == 0 || nentries % (32 * 256) != 0 ||
+        nentries >= (1u << 28) || ((uintptr_t)device_points & 15)) {
+        g_err = "invalid arguments";
+        return -1;
+    }
+    try {
+        Kit& k = kit();
+        std::lock_guard<std::mutex> lk(k.mu);
+        hipEvent_t e0, e1;
+        HIPCK(hipEventCreate(&e0));
+        HIPCK(hipEventCreate(&e1));
+        float best = 0;
+        for (int rep = 0; rep < 4; rep++) {
+            HIPCK(hipEventRecord(e0, k.stream));
+            launch_gather_probe(k.stream, (int)stride_bytes, static_cast<const uint4*>(device_points), device_index,
+                                device_xyzz_out, (uint32_t)nentries);
+            HIPCK(hipGetLastError());
+            HIPCK(hipEventRecord(e1, k.stream));
+            HIPCK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCK(hipEventElapsedTime(&ms, e0, e1));
+            if (rep > 0 && (best == 0 || ms < best)) best = ms;   // the first launch is a warm-up
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        *ns_per_entry = best * 1e6 / (double)nentries;
     } catch (const HipFail& f) {
         g_err = f.msg;
         return -1;
