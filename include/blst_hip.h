@@ -173,6 +173,20 @@ size_t blst_p2s_mult_fr_scratch_sizeof(size_t max_points);
 bool blst_p2s_mult_fr(blst_p2 *ret, const blst_p2_affine points[], const blst_fr scalars[], size_t npoints,
                       blst_limb_t *scratch, size_t scratch_size);
 
+/* Replace blst_p2s_to_affine, blst_p2s_mult_wbits_precompute_sizeof,
+ * blst_p2s_mult_wbits_precompute, blst_p2s_mult_wbits_scratch_sizeof and
+ * blst_p2s_mult_wbits (bindings/blst_scalar.h:722-732; blst/c_api.cpp:
+ * 1806-1817, 1829-1888): the contracts of the G1 forms above over G2, with
+ * 192-byte table entries (rows of [1]P .. [2^(wbits-1)]P, wbits 2..14) and a
+ * scratch of 288 bytes per point up to 4096 points, which is not written. */
+void blst_p2s_to_affine(blst_p2_affine dst[], const blst_p2 *const points[], size_t npoints);
+size_t blst_p2s_mult_wbits_precompute_sizeof(size_t wbits, size_t npoints);
+void blst_p2s_mult_wbits_precompute(blst_p2_affine table[], size_t wbits, const blst_p2_affine *const points[],
+                                    size_t npoints);
+size_t blst_p2s_mult_wbits_scratch_sizeof(size_t npoints);
+void blst_p2s_mult_wbits(blst_p2 *ret, const blst_p2_affine table[], size_t wbits, size_t npoints,
+                         const blst_byte *const scalars[], size_t nbits, blst_limb_t *scratch);
+
 /* ------------------------------------------------------- device extensions */
 
 /* Last error message of the calling thread ("" when the last call succeeded). */

@@ -1208,10 +1208,20 @@ static void gather_booth(jac_t *out, const aff_t *row, size_t wbits, u64 enc) {
     jac_from_aff(out, &a);
 }
 
-/* msm_mult_table (multi_scalar.cpp:620-707): the top window holds
- * nbits % wbits bits (with the overlap bit below it), then full windows from
- * the top, the lowest without an overlap bit; per window the gathered
- * entries are summed into ret, then ret is doubled wbits times. */
+/* The Booth encoding of point scalar s in the window of msm_mult_table
+ * (multi_scalar.cpp:620-707) whose lowest bit is rem: the top window
+ * (rem = nbits - nbits % wbits) holds nbits - rem bits, every lower one
+ * wbits, each read with the overlap bit below it except the lowest. */
+u64 or_table_booth(const uint8_t *s, size_t nbits, size_t wbits, size_t rem) {
+    const size_t cur = nbits - rem < wbits ? nbits - rem : wbits;
+    const u64 mask = ((u64)1 << (cur + 1)) - 1;
+    const u64 wval = rem ? get_wval_limb(s, rem - 1, cur + 1) & mask : (get_wval_limb(s, 0, cur) << 1) & mask;
+    return booth_encode(wval, wbits);
+}
+
+/* msm_mult_table: from the top window down, the gathered entries of every
+ * point are summed into ret, and ret is doubled wbits times between
+ * windows. */
 void or_mult_wbits(u64 ret[18], const u64 *table, size_t wbits, size_t npoints, const uint8_t *scalars,
                    size_t nbits) {
     jac_t *r = (jac_t *)ret;
@@ -1219,28 +1229,13 @@ void or_mult_wbits(u64 ret[18], const u64 *table, size_t wbits, size_t npoints, 
     if (npoints == 0 || nbits == 0 || nbits > 256 || wbits < 2 || wbits > 14) return;
     const size_t m = (size_t)1 << (wbits - 1), nbytes = (nbits + 7) / 8;
     const aff_t *tab = (const aff_t *)table;
-    size_t cur = nbits % wbits;
-    u64 mask = ((u64)1 << (cur + 1)) - 1;
-    size_t rem = nbits - cur;
-    const size_t no_overlap = rem == 0, overlap = no_overlap ^ 1;
     jac_t g;
-    for (size_t i = 0; i < npoints; i++) {
-        const uint8_t *s = scalars + nbytes * i;
-        u64 wval = (get_wval_limb(s, rem - overlap, cur + overlap) << no_overlap) & mask;
-        gather_booth(&g, tab + m * i, wbits, booth_encode(wval, wbits));
-        jac_add(r, r, &g);
-    }
-    while (rem > 0) {
-        for (size_t k = 0; k < wbits; k++) jac_double(r, r);
-        cur = wbits;
-        mask = ((u64)1 << (cur + 1)) - 1;
-        rem -= cur;
+    for (size_t rem = nbits - nbits % wbits;; rem -= wbits) {
         for (size_t i = 0; i < npoints; i++) {
-            const uint8_t *s = scalars + nbytes * i;
-            const u64 wval = rem ? get_wval_limb(s, rem - 1, cur + 1) & mask
-                                 : (get_wval_limb(s, 0, cur) << 1) & mask;
-            gather_booth(&g, tab + m * i, wbits, booth_encode(wval, wbits));
+            gather_booth(&g, tab + m * i, wbits, or_table_booth(scalars + nbytes * i, nbits, wbits, rem));
             jac_add(r, r, &g);
         }
+        if (rem == 0) break;
+        for (size_t k = 0; k < wbits; k++) jac_double(r, r);
     }
 }

@@ -67,6 +67,8 @@ size_t or_wbits_scratch_sizeof(size_t npoints);
 void or_wbits_precompute(uint64_t *table, size_t wbits, const uint64_t *points, size_t npoints);
 void or_mult_wbits(uint64_t ret[18], const uint64_t *table, size_t wbits, size_t npoints,
                    const uint8_t *scalars, size_t nbits);
+/* the encoded digit of a table window (lowest bit rem) of one scalar */
+uint64_t or_table_booth(const uint8_t *scalar, size_t nbits, size_t wbits, size_t rem);
 
 /* MSM over contiguous arrays (points: n x 12 limbs, scalars: n x ceil(nbits/8)) */
 void or_msm_pippenger(uint64_t ret[18], const uint64_t *points, const uint8_t *scalars,
@@ -103,6 +105,11 @@ void or_msm_p2_tile(uint64_t ret[36], const uint64_t *points, const uint8_t *sca
 void or_msm_p2_pippenger(uint64_t ret[36], const uint64_t *points, const uint8_t *scalars, size_t npoints,
                          size_t nbits, size_t window);
 void or_gen_p2_points(uint64_t *out_affine, size_t n, uint64_t seed);
+/* G2 fixed-base tables: rows of 2^(wbits-1) affine multiples (24 limbs
+ * each) and the MSM over them, as or_wbits_precompute / or_mult_wbits */
+void or_p2_wbits_precompute(uint64_t *table, size_t wbits, const uint64_t *points, size_t npoints);
+void or_p2_mult_wbits(uint64_t ret[36], const uint64_t *table, size_t wbits, size_t npoints,
+                      const uint8_t *scalars, size_t nbits);
 
 #ifdef __cplusplus
 }

@@ -450,6 +450,64 @@ void or_msm_p2_pippenger(u64 ret[36], const u64 *points, const uint8_t *scalars,
     free(buckets);
 }
 
+/* point_precompute_row over Fp2 (multi_scalar.cpp:268-286): row[0] = P,
+ * row[1] = 2P, then row[k] = row[k-1] + P and row[k+1] = 2 row[j] for
+ * k = 2, 4, ..., j = 1, 2, ...; an all-zero P gives an all-zero row.
+ * msm_precompute_wbits<Fp2> (multi_scalar.cpp:481-580) writes row i, in
+ * affine form, at table[i * 2^(wbits-1)]. */
+void or_p2_wbits_precompute(u64 *table, size_t wbits, const u64 *points, size_t npoints) {
+    const size_t m = (size_t)1 << (wbits - 1);
+    jac2_t *row = malloc(m * sizeof(jac2_t));
+    for (size_t i = 0; i < npoints; i++) {
+        const aff2_t *p = (const aff2_t *)(points + 24 * i);
+        aff2_t *dst = (aff2_t *)(table + 24 * m * i);
+        if (aff2_is_inf(p)) {
+            memset(dst, 0, m * sizeof(aff2_t));
+            continue;
+        }
+        jac2_from_aff(&row[0], p);
+        if (m > 1) jac2_double(&row[1], &row[0]);
+        for (size_t k = 2, j = 1; k < m; k += 2, j++) {
+            jac2_add(&row[k], &row[k - 1], &row[0]);
+            jac2_double(&row[k + 1], &row[j]);
+        }
+        or_p2s_to_affine((u64 *)dst, (const u64 *)row, m);
+    }
+    free(row);
+}
+
+/* point_gather_booth_wbits over Fp2 (multi_scalar.h:262-276) */
+static void gather_booth2(jac2_t *out, const aff2_t *row, size_t wbits, u64 enc) {
+    const u64 sign = (enc >> wbits) & 1;
+    const u64 idx = enc & (((u64)1 << wbits) - 1);
+    if (idx == 0) {
+        memset(out, 0, sizeof(*out));
+        return;
+    }
+    aff2_t a = row[idx - 1];
+    if (sign) f2_neg(&a.y, &a.y);
+    jac2_from_aff(out, &a);
+}
+
+/* msm_mult_table<Fp2> (multi_scalar.cpp:620-707), as or_mult_wbits */
+void or_p2_mult_wbits(u64 ret[36], const u64 *table, size_t wbits, size_t npoints, const uint8_t *scalars,
+                      size_t nbits) {
+    jac2_t *r = (jac2_t *)ret;
+    memset(r, 0, sizeof(*r));
+    if (npoints == 0 || nbits == 0 || nbits > 256 || wbits < 2 || wbits > 14) return;
+    const size_t m = (size_t)1 << (wbits - 1), nbytes = (nbits + 7) / 8;
+    const aff2_t *tab = (const aff2_t *)table;
+    jac2_t g;
+    for (size_t rem = nbits - nbits % wbits;; rem -= wbits) {
+        for (size_t i = 0; i < npoints; i++) {
+            gather_booth2(&g, tab + m * i, wbits, or_table_booth(scalars + nbytes * i, nbits, wbits, rem));
+            jac2_add(r, r, &g);
+        }
+        if (rem == 0) break;
+        for (size_t k = 0; k < wbits; k++) jac2_double(r, r);
+    }
+}
+
 /* Synthetic G2 bases: P_i = [a + i b] G2 for a, b drawn from the seeded
  * stream (one addition per point, then one batched normalisation). */
 void or_gen_p2_points(u64 *out_affine, size_t n, uint64_t seed) {

@@ -74,6 +74,8 @@ def lib():
             "or_msm_p2_pippenger": [_u64p, _u64p, _u8p, _sz, _sz, _sz],
             "or_gen_p2_points": [_u64p, _sz, ctypes.c_uint64],
             "or_mult_wbits": [_u64p, _u64p, _sz, _sz, _u8p, _sz],
+            "or_p2_wbits_precompute": [_u64p, _sz, _u64p, _sz],
+            "or_p2_mult_wbits": [_u64p, _u64p, _sz, _sz, _u8p, _sz],
         }
         for name, args in sigs.items():
             f = getattr(L, name)
@@ -95,6 +97,7 @@ def lib():
             ("or_p2_msm_fr_scratch_sizeof", [_sz], _sz),
             ("or_wbits_precompute_sizeof", [_sz, _sz], _sz),
             ("or_wbits_scratch_sizeof", [_sz], _sz),
+            ("or_table_booth", [_u8p, _sz, _sz, _sz], ctypes.c_uint64),
         ]:
             f = getattr(L, name)
             f.argtypes = args
@@ -423,6 +426,29 @@ def msm_p2_tile(points, scalars, n, nbits, bit0, window):
     sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
     lib().or_msm_p2_tile(u64p(r), u64p(pts), u8p(sc), n, nbits, bit0, window)
     return r
+
+
+def p2_wbits_precompute(points, wbits):
+    """(n, 24) G2 affine -> (n * 2^(wbits-1), 24) affine table, rows of [j]P."""
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    n = pts.shape[0]
+    out = np.zeros((n << (wbits - 1), 24), np.uint64)
+    if n:
+        lib().or_p2_wbits_precompute(u64p(out.reshape(-1)), wbits, u64p(pts.reshape(-1)), n)
+    return out
+
+
+def p2_mult_wbits(table, wbits, scalars, n, nbits):
+    r = np.zeros(36, np.uint64)
+    tab = np.ascontiguousarray(table, dtype=np.uint64).reshape(-1)
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8).reshape(-1)
+    lib().or_p2_mult_wbits(u64p(r), u64p(tab), wbits, n, u8p(sc), nbits)
+    return r
+
+
+def table_booth(scalar, nbits, wbits, rem):
+    s = np.ascontiguousarray(scalar, dtype=np.uint8)
+    return int(lib().or_table_booth(u8p(s), nbits, wbits, rem))
 
 
 def gen_p2_points(n, seed):

@@ -32,6 +32,11 @@ SIGNATURES = {
     "blst_p1s_mult_wbits_precompute": ([_vp, _sz, _vp, _sz], None),
     "blst_p1s_mult_wbits_scratch_sizeof": ([_sz], _sz),
     "blst_p1s_mult_wbits": ([_vp, _vp, _sz, _sz, _vp, _sz, _vp], None),
+    "blst_p2s_to_affine": ([_vp, _vp, _sz], None),
+    "blst_p2s_mult_wbits_precompute_sizeof": ([_sz, _sz], _sz),
+    "blst_p2s_mult_wbits_precompute": ([_vp, _sz, _vp, _sz], None),
+    "blst_p2s_mult_wbits_scratch_sizeof": ([_sz], _sz),
+    "blst_p2s_mult_wbits": ([_vp, _vp, _sz, _sz, _vp, _sz, _vp], None),
     "blst_hip_p1s_mult_wbits_precompute_device": ([_vp, _sz, _vp, _sz], ctypes.c_int),
     "blst_hip_p1s_mult_wbits_device": ([_vp, _vp, _sz, _sz, _vp, _sz], ctypes.c_int),
     "blst_hip_comb_table_bytes": ([_sz, _sz, _sz], _sz),
@@ -323,6 +328,63 @@ def mult_wbits(table, wbits, scalars, nbits, mode="contiguous", k=1):
     err = last_error()
     if err and n > 0:
         raise RuntimeError("blst_p1s_mult_wbits: " + err)
+    return ret
+
+
+def _descriptor_array(base, n, stride, mode, k):
+    if mode == "contiguous":
+        return np.array([base, 0], np.uint64)
+    if mode == "exact":
+        return np.array([base + stride * i for i in range(n)], np.uint64)
+    if mode == "mixed":
+        return np.array([base + stride * i if i < k else 0 for i in range(n)], np.uint64)
+    raise ValueError(mode)
+
+
+def p2s_to_affine(points, mode="contiguous", k=1):
+    """(n, 36) uint64 G2 Jacobian points -> (n, 24) affine through
+    blst_p2s_to_affine."""
+    L = load()
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    n = pts.shape[0]
+    pp = _descriptor_array(pts.ctypes.data, n, 288, mode, k)
+    out = np.zeros((max(n, 1), 24), np.uint64)
+    L.blst_p2s_to_affine(_ptr(out), _ptr(pp), n)
+    err = last_error()
+    if err and n > 0:
+        raise RuntimeError("blst_p2s_to_affine: " + err)
+    return out[:n]
+
+
+def p2_wbits_precompute(points, wbits, mode="contiguous", k=1):
+    """(n, 24) G2 affine points -> (n * 2^(wbits-1), 24) table through
+    blst_p2s_mult_wbits_precompute."""
+    L = load()
+    pts = np.ascontiguousarray(points, dtype=np.uint64)
+    n = pts.shape[0]
+    pp = _descriptor_array(pts.ctypes.data, n, 192, mode, k)
+    size = L.blst_p2s_mult_wbits_precompute_sizeof(wbits, n)
+    table = np.zeros((max(size // 192, 1), 24), np.uint64)
+    L.blst_p2s_mult_wbits_precompute(_ptr(table), wbits, _ptr(pp), n)
+    err = last_error()
+    if err and n > 0:
+        raise RuntimeError("blst_p2s_mult_wbits_precompute: " + err)
+    return table[:size // 192]
+
+
+def p2_mult_wbits(table, wbits, scalars, nbits, mode="contiguous", k=1):
+    """G2 MSM over a wbits table through blst_p2s_mult_wbits."""
+    L = load()
+    sc = np.ascontiguousarray(scalars, dtype=np.uint8)
+    n = sc.shape[0]
+    tab = np.ascontiguousarray(table, dtype=np.uint64)
+    ss = _descriptor_array(sc.ctypes.data, n, sc.shape[1], mode, k)
+    scr = np.zeros(max(L.blst_p2s_mult_wbits_scratch_sizeof(n) // 8, 1), np.uint64)
+    ret = np.zeros(36, np.uint64)
+    L.blst_p2s_mult_wbits(_ptr(ret), _ptr(tab), wbits, n, _ptr(ss), nbits, _ptr(scr))
+    err = last_error()
+    if err and n > 0:
+        raise RuntimeError("blst_p2s_mult_wbits: " + err)
     return ret
 
 

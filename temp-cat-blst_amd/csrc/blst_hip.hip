@@ -1321,9 +1321,10 @@ constexpr size_t kTableAffineLanes = 131072;
 // points per host round trip of blst_p1s_to_affine (2.4 GB in, 1.6 GB out)
 constexpr size_t kAffineChunk = (size_t)1 << 24;
 
+template <class F = fp>
 void enqueue_to_affine(hipStream_t stream, const void* d_in, void* d_out, size_t n) {
     const uint32_t T = (uint32_t)std::min(n, kAffineLanes);
-    launch_k_jac_to_affine(stream, static_cast<const uint4*>(d_in), static_cast<aff*>(d_out), (uint32_t)n, T);
+    launch_k_jac_to_affine(stream, static_cast<const uint4*>(d_in), static_cast<aff_t<F>*>(d_out), (uint32_t)n, T);
     HIPCK(hipGetLastError());
 }
 
@@ -1335,21 +1336,69 @@ constexpr size_t kWbitsListBytes = (size_t)256 << 20;   // gathered entries per 
 
 bool valid_wbits(size_t wbits) { return wbits >= 2 && wbits <= 14; }
 
+// The groups of the fixed-base tables: point layouts, the device field and
+// the host's XYZZ arithmetic for the window fold.
+struct G1Ops {
+    using Aff = blst_p1_affine;
+    using Jac = blst_p1;
+    using F = fp;
+    using HX = hostec::xyzz6;
+    static void fold(HX& r, const HX* w, int n, int c) { hostec::fold_windows(r, w, n, c); }
+    static void addp(HX& r, const HX& a, const HX& b) { hostec::addp(r, a, b); }
+    static void finish(Jac* ret, const HX& x) {
+        hostec::jac6 j;
+        hostec::to_jac(j, x);
+        std::memcpy(ret, &j, sizeof(j));
+    }
+    // y = -y of a staged affine point (0 stays 0)
+    static void neg_y(uint8_t* pt) {
+        hostec::fp6 zero{}, y;
+        std::memcpy(&y, pt + 48, 48);
+        hostec::sub(y, zero, y);
+        std::memcpy(pt + 48, &y, 48);
+    }
+};
+
+struct G2Ops {
+    using Aff = blst_p2_affine;
+    using Jac = blst_p2;
+    using F = fp2;
+    using HX = hostec2::xyzz12;
+    static void fold(HX& r, const HX* w, int n, int c) { hostec2::fold_windows(r, w, n, c); }
+    static void addp(HX& r, const HX& a, const HX& b) { hostec2::addp(r, a, b); }
+    static void finish(Jac* ret, const HX& x) {
+        hostec2::jac12 j;
+        hostec2::to_jac(j, x);
+        std::memcpy(ret, &j, sizeof(j));
+    }
+    static void neg_y(uint8_t* pt) {
+        hostec::fp6 zero{}, y;
+        for (int c = 0; c < 2; c++) {
+            std::memcpy(&y, pt + 96 + 48 * c, 48);
+            hostec::sub(y, zero, y);
+            std::memcpy(pt + 96 + 48 * c, &y, 48);
+        }
+    }
+};
+
 // rows of n points (device, affine) into table (device, n * 2^(wbits-1)
 // affine entries)
+template <class G = G1Ops>
 void enqueue_wbits_table(Ctx& cx, const void* d_pts, size_t n, size_t wbits, void* d_table) {
+    using X = xyzz_t<typename G::F>;
+    using A = aff_t<typename G::F>;
     const size_t m = (size_t)1 << (wbits - 1);
     const uint32_t L = (uint32_t)std::min<size_t>(m, 64);
     const size_t R = std::max<size_t>(1, kWbitsRowEntries / m);
-    cx.wb_rows.ensure(std::min(n, R) * m * sizeof(xyzz));
+    cx.wb_rows.ensure(std::min(n, R) * m * sizeof(X));
     for (size_t row0 = 0; row0 < n; row0 += R) {
         const size_t nr = std::min(R, n - row0);
         const size_t count = nr * m;
         launch_k_wbits_rows(cx.stream, static_cast<const uint4*>(d_pts), (uint32_t)row0, (uint32_t)nr, (uint32_t)m, L,
-                            cx.wb_rows.as<xyzz>());
+                            cx.wb_rows.as<X>());
         const uint32_t Q = (uint32_t)std::min<size_t>(64, std::max<size_t>(1, count / kTableAffineLanes));
-        launch_k_to_affine(dim3(div_up(div_up(count, Q), 256)), dim3(256), cx.stream, cx.wb_rows.as<const xyzz>(),
-                           static_cast<aff*>(d_table) + row0 * m, (uint32_t)count, Q);
+        launch_k_to_affine(dim3(div_up(div_up(count, Q), 256)), dim3(256), cx.stream, cx.wb_rows.as<const X>(),
+                           static_cast<A*>(d_table) + row0 * m, (uint32_t)count, Q);
         HIPCK(hipGetLastError());
     }
 }
@@ -1357,25 +1406,28 @@ void enqueue_wbits_table(Ctx& cx, const void* d_pts, size_t n, size_t wbits, voi
 // window sums S_w of n (window, point) entries, from a device table and
 // device scalars or from a gathered list (window-major, D * n entries),
 // added into W (D windows)
+template <class G = G1Ops>
 void wbits_window_sums(Ctx& cx, const void* d_table, const void* d_scl, size_t nbits, const void* d_list, size_t n,
-                       size_t wbits, std::vector<hostec::xyzz6>& W) {
+                       size_t wbits, std::vector<typename G::HX>& W) {
+    using X = xyzz_t<typename G::F>;
     const size_t D = 1 + nbits / wbits;
     const size_t nbytes = (nbits + 7) / 8;
     const size_t C = std::min(n, std::max<size_t>(4, (n * D + kWbitsSumLanes - 1) / kWbitsSumLanes));
     const size_t nch = (n + C - 1) / C;
-    cx.wb_part.ensure(D * nch * sizeof(xyzz));
-    cx.wb_W.ensure(D * sizeof(xyzz));
+    cx.wb_part.ensure(D * nch * sizeof(X));
+    cx.wb_W.ensure(D * sizeof(X));
     const uint32_t vec = (nbytes == 32 && ((uintptr_t)d_scl % 16) == 0) ? 1u : 0u;
     launch_k_wbits_sum(cx.stream, static_cast<const uint4*>(d_table), static_cast<const uint8_t*>(d_scl),
                        (uint32_t)nbytes, (uint32_t)nbits, vec, static_cast<const uint4*>(d_list), (uint32_t)n,
-                       (uint32_t)wbits, (uint32_t)D, (uint32_t)C, (uint32_t)nch, cx.wb_part.as<xyzz>());
-    launch_k_wbits_tree(cx.stream, cx.wb_part.as<const xyzz>(), (uint32_t)nch, (uint32_t)D, cx.wb_W.as<xyzz>());
+                       (uint32_t)wbits, (uint32_t)D, (uint32_t)C, (uint32_t)nch, cx.wb_part.as<X>());
+    launch_k_wbits_tree(cx.stream, cx.wb_part.as<const X>(), (uint32_t)nch, (uint32_t)D, cx.wb_W.as<X>());
     HIPCK(hipGetLastError());
-    std::vector<hostec::xyzz6> part(D);
-    HIPCK(hipMemcpyAsync(part.data(), cx.wb_W.p, D * sizeof(xyzz), hipMemcpyDeviceToHost, cx.stream));
+    static_assert(sizeof(typename G::HX) == sizeof(X), "host and device XYZZ layouts");
+    std::vector<typename G::HX> part(D);
+    HIPCK(hipMemcpyAsync(part.data(), cx.wb_W.p, D * sizeof(X), hipMemcpyDeviceToHost, cx.stream));
     HIPCK(hipStreamSynchronize(cx.stream));
-    if (W.empty()) W.assign(D, hostec::xyzz6{});
-    for (size_t w = 0; w < D; w++) hostec::addp(W[w], W[w], part[w]);
+    if (W.empty()) W.assign(D, typename G::HX{});
+    for (size_t w = 0; w < D; w++) G::addp(W[w], W[w], part[w]);
 }
 
 // Booth digit of the window at bit0 (<= nbits) of width c, from the scalar's
@@ -1409,32 +1461,31 @@ void* wbits_host_stage(Ctx& cx, size_t bytes) {
 // The host-table MSM: the host computes digits and gathers the signed
 // entries of a run of points for every window into page-locked memory (the
 // table stays where the caller keeps it), the device sums them.
-void host_mult_wbits(Ctx& cx, blst_p1* ret, const blst_p1_affine* table, size_t wbits, size_t n,
+template <class G = G1Ops>
+void host_mult_wbits(Ctx& cx, typename G::Jac* ret, const typename G::Aff* table, size_t wbits, size_t n,
                      const uint8_t* scalars, size_t nbits) {
+    using Aff = typename G::Aff;
     const size_t D = 1 + nbits / wbits, nbytes = (nbits + 7) / 8, m = (size_t)1 << (wbits - 1);
-    const size_t run = std::max<size_t>(1, std::min(n, kWbitsListBytes / (D * sizeof(blst_p1_affine))));
-    uint8_t* stage = static_cast<uint8_t*>(wbits_host_stage(cx, run * D * sizeof(blst_p1_affine)));
-    cx.wb_list.ensure(run * D * sizeof(blst_p1_affine));
-    std::vector<hostec::xyzz6> W;
+    const size_t run = std::max<size_t>(1, std::min(n, kWbitsListBytes / (D * sizeof(Aff))));
+    uint8_t* stage = static_cast<uint8_t*>(wbits_host_stage(cx, run * D * sizeof(Aff)));
+    cx.wb_list.ensure(run * D * sizeof(Aff));
+    std::vector<typename G::HX> W;
     const unsigned nthreads = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
     for (size_t i0 = 0; i0 < n; i0 += run) {
         const size_t cnt = std::min(run, n - i0);
         auto gather = [&](size_t a, size_t b) {
             for (size_t i = a; i < b; i++) {
                 const uint8_t* sc = scalars + (i0 + i) * nbytes;
-                const blst_p1_affine* row = table + (i0 + i) * m;
+                const Aff* row = table + (i0 + i) * m;
                 for (size_t w = 0; w < D; w++) {
-                    hostec::fp6* dst = reinterpret_cast<hostec::fp6*>(stage + (w * cnt + i) * sizeof(blst_p1_affine));
+                    uint8_t* dst = stage + (w * cnt + i) * sizeof(Aff);
                     const int32_t d = host_booth_digit(sc, (uint32_t)(w * wbits), (uint32_t)wbits, (uint32_t)nbits);
                     if (d == 0) {
-                        std::memset(dst, 0, sizeof(blst_p1_affine));
+                        std::memset(dst, 0, sizeof(Aff));
                         continue;
                     }
-                    std::memcpy(dst, &row[(d < 0 ? -d : d) - 1], sizeof(blst_p1_affine));
-                    if (d < 0 && !hostec::is_zero(dst[1])) {
-                        hostec::fp6 zero{};
-                        hostec::sub(dst[1], zero, dst[1]);
-                    }
+                    std::memcpy(dst, &row[(d < 0 ? -d : d) - 1], sizeof(Aff));
+                    if (d < 0) G::neg_y(dst);
                 }
             }
         };
@@ -1449,15 +1500,124 @@ void host_mult_wbits(Ctx& cx, blst_p1* ret, const blst_p1_affine* table, size_t 
             }
             for (auto& t : th) t.join();
         }
-        HIPCK(hipMemcpyAsync(cx.wb_list.p, stage, cnt * D * sizeof(blst_p1_affine), hipMemcpyHostToDevice,
-                             cx.stream));
-        wbits_window_sums(cx, nullptr, nullptr, nbits, cx.wb_list.p, cnt, wbits, W);
+        HIPCK(hipMemcpyAsync(cx.wb_list.p, stage, cnt * D * sizeof(Aff), hipMemcpyHostToDevice, cx.stream));
+        wbits_window_sums<G>(cx, nullptr, nullptr, nbits, cx.wb_list.p, cnt, wbits, W);
     }
-    hostec::xyzz6 r;
-    hostec::fold_windows(r, W.data(), (int)D, (int)wbits);
-    hostec::jac6 j;
-    hostec::to_jac(j, r);
-    std::memcpy(ret, &j, sizeof(j));
+    typename G::HX r;
+    G::fold(r, W.data(), (int)D, (int)wbits);
+    G::finish(ret, r);
+}
+
+// The fixed-base entry points of either group (blst/c_api.cpp:1310-1369 for
+// G1, 1829-1888 for G2): the reference's checks in its order, then the
+// device table build or the host-table MSM.
+template <class G>
+size_t wbits_precompute_bytes(size_t wbits, size_t npoints) {
+    if (!valid_wbits(wbits)) return 0;
+    size_t bytes;
+    if (__builtin_mul_overflow(npoints, sizeof(typename G::Aff), &bytes)) return 0;
+    if (bytes > (SIZE_MAX >> (wbits - 1))) return 0;
+    return bytes << (wbits - 1);
+}
+
+template <class G>
+void wbits_precompute_entry(typename G::Aff table[], size_t wbits, const typename G::Aff* const points[],
+                            size_t npoints) {
+    using Aff = typename G::Aff;
+    if (!valid_wbits(wbits) || (npoints != 0 && table == nullptr) ||
+        (npoints != 0 && (points == nullptr || points[0] == nullptr))) {
+        g_err = "invalid arguments";
+        return;
+    }
+    if (npoints == 0) { g_err.clear(); return; }
+    if (wbits_precompute_bytes<G>(wbits, npoints) == 0 || npoints >= 0x7fffffffULL) {
+        g_err = "table size out of range";
+        return;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        std::vector<uint8_t> stage;
+        const uint8_t* hp = resolve(points, npoints, sizeof(Aff), stage);
+        const size_t m = (size_t)1 << (wbits - 1);
+        cx.wb_pts.ensure(npoints * sizeof(Aff));
+        // every point is on the device before any row is written back: a
+        // packed input may occupy the table's tail
+        cx.stager.upload(cx.wb_pts.p, hp, npoints * sizeof(Aff), cx.stream);
+        const size_t R = std::max<size_t>(1, kWbitsRowEntries / m);
+        cx.wb_table.ensure(std::min(npoints, R) * m * sizeof(Aff));
+        for (size_t row0 = 0; row0 < npoints; row0 += R) {
+            const size_t nr = std::min(R, npoints - row0);
+            const uint8_t* rows = static_cast<const uint8_t*>(cx.wb_pts.p) + row0 * sizeof(Aff);
+            enqueue_wbits_table<G>(cx, rows, nr, wbits, cx.wb_table.p);
+            cx.stager.download(table + row0 * m, cx.wb_table.p, nr * m * sizeof(Aff), cx.stream);
+        }
+    } catch (...) {
+        g_err = current_error();
+        return;
+    }
+    g_err.clear();
+}
+
+template <class G>
+void mult_wbits_entry(typename G::Jac* ret, const typename G::Aff table[], size_t wbits, size_t npoints,
+                      const blst_byte* const scalars[], size_t nbits, blst_limb_t* scratch) {
+    if (npoints == 0 || nbits == 0) { std::memset(ret, 0, sizeof(*ret)); g_err.clear(); return; }
+    if (nbits > 256 || !valid_wbits(wbits) || table == nullptr || scalars == nullptr || scalars[0] == nullptr ||
+        scratch == nullptr) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "invalid arguments";
+        return;
+    }
+    if (wbits_precompute_bytes<G>(wbits, npoints) == 0 || npoints >= 0x7fffffffULL) {
+        std::memset(ret, 0, sizeof(*ret));
+        g_err = "table size out of range";
+        return;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        std::vector<uint8_t> stage;
+        const uint8_t* hs = resolve(scalars, npoints, (nbits + 7) / 8, stage);
+        host_mult_wbits<G>(cx, ret, table, wbits, npoints, hs, nbits);
+    } catch (...) {
+        g_err = current_error();
+        std::memset(ret, 0, sizeof(*ret));
+        return;
+    }
+    g_err.clear();
+}
+
+// blst_p2s_to_affine (blst/c_api.cpp:1806-1817): chunks of the Jacobian
+// points up, the batch normalisation, the affine points down
+void p2s_to_affine(blst_p2_affine dst[], const blst_p2* const points[], size_t npoints) {
+    if (npoints == 0) { g_err.clear(); return; }
+    if (dst == nullptr) { g_err = "dst is NULL"; return; }
+    if (points == nullptr || points[0] == nullptr) {
+        std::memset(dst, 0, npoints * sizeof(blst_p2_affine));
+        g_err.clear();
+        return;
+    }
+    try {
+        Ctx& cx = context();
+        std::lock_guard<std::mutex> lk(cx.mu);
+        std::vector<uint8_t> stage;
+        const uint8_t* hp = resolve(points, npoints, sizeof(blst_p2), stage);
+        const size_t chunk = std::min(npoints, kAffineChunk / 2);
+        cx.jac_in.ensure(chunk * sizeof(blst_p2));
+        cx.aff_out.ensure(chunk * sizeof(blst_p2_affine));
+        for (size_t b = 0; b < npoints; b += chunk) {
+            const size_t m = std::min(chunk, npoints - b);
+            cx.stager.upload(cx.jac_in.p, hp + b * sizeof(blst_p2), m * sizeof(blst_p2), cx.stream);
+            enqueue_to_affine<fp2>(cx.stream, cx.jac_in.p, cx.aff_out.p, m);
+            cx.stager.download(dst + b, cx.aff_out.p, m * sizeof(blst_p2_affine), cx.stream);
+        }
+    } catch (...) {
+        g_err = current_error();
+        std::memset(dst, 0, npoints * sizeof(blst_p2_affine));
+        return;
+    }
+    g_err.clear();
 }
 
 // ---------------------------------------------------------------- comb tables
@@ -2235,11 +2395,7 @@ int blst_hip_p1s_to_affine_device(void* device_dst, const void* device_points, s
 }
 
 size_t blst_p1s_mult_wbits_precompute_sizeof(size_t wbits, size_t npoints) {
-    if (!valid_wbits(wbits)) return 0;
-    size_t bytes;
-    if (__builtin_mul_overflow(npoints, sizeof(blst_p1_affine), &bytes)) return 0;
-    if (bytes > (SIZE_MAX >> (wbits - 1))) return 0;
-    return bytes << (wbits - 1);
+    return wbits_precompute_bytes<G1Ops>(wbits, npoints);
 }
 
 size_t blst_p1s_mult_wbits_scratch_sizeof(size_t npoints) {
@@ -2248,67 +2404,37 @@ size_t blst_p1s_mult_wbits_scratch_sizeof(size_t npoints) {
 
 void blst_p1s_mult_wbits_precompute(blst_p1_affine table[], size_t wbits, const blst_p1_affine* const points[],
                                     size_t npoints) {
-    if (!valid_wbits(wbits) || (npoints != 0 && table == nullptr) ||
-        (npoints != 0 && (points == nullptr || points[0] == nullptr))) {
-        g_err = "invalid arguments";
-        return;
-    }
-    if (npoints == 0) { g_err.clear(); return; }
-    if (blst_p1s_mult_wbits_precompute_sizeof(wbits, npoints) == 0 || npoints >= 0x7fffffffULL) {
-        g_err = "table size out of range";
-        return;
-    }
-    try {
-        Ctx& cx = context();
-        std::lock_guard<std::mutex> lk(cx.mu);
-        std::vector<uint8_t> stage;
-        const uint8_t* hp = resolve(points, npoints, sizeof(blst_p1_affine), stage);
-        const size_t m = (size_t)1 << (wbits - 1);
-        cx.wb_pts.ensure(npoints * sizeof(blst_p1_affine));
-        // every point is on the device before any row is written back: a
-        // packed input may occupy the table's tail
-        cx.stager.upload(cx.wb_pts.p, hp, npoints * sizeof(blst_p1_affine), cx.stream);
-        const size_t R = std::max<size_t>(1, kWbitsRowEntries / m);
-        cx.wb_table.ensure(std::min(npoints, R) * m * sizeof(blst_p1_affine));
-        for (size_t row0 = 0; row0 < npoints; row0 += R) {
-            const size_t nr = std::min(R, npoints - row0);
-            const uint8_t* rows = static_cast<const uint8_t*>(cx.wb_pts.p) + row0 * sizeof(blst_p1_affine);
-            enqueue_wbits_table(cx, rows, nr, wbits, cx.wb_table.p);
-            cx.stager.download(table + row0 * m, cx.wb_table.p, nr * m * sizeof(blst_p1_affine), cx.stream);
-        }
-    } catch (...) {
-        g_err = current_error();
-        return;
-    }
-    g_err.clear();
+    wbits_precompute_entry<G1Ops>(table, wbits, points, npoints);
 }
 
 void blst_p1s_mult_wbits(blst_p1* ret, const blst_p1_affine table[], size_t wbits, size_t npoints,
                          const blst_byte* const scalars[], size_t nbits, blst_limb_t* scratch) {
-    if (npoints == 0 || nbits == 0) { std::memset(ret, 0, sizeof(*ret)); g_err.clear(); return; }
-    if (nbits > 256 || !valid_wbits(wbits) || table == nullptr || scalars == nullptr || scalars[0] == nullptr ||
-        scratch == nullptr) {
-        std::memset(ret, 0, sizeof(*ret));
-        g_err = "invalid arguments";
-        return;
-    }
-    if (blst_p1s_mult_wbits_precompute_sizeof(wbits, npoints) == 0 || npoints >= 0x7fffffffULL) {
-        std::memset(ret, 0, sizeof(*ret));
-        g_err = "table size out of range";
-        return;
-    }
-    try {
-        Ctx& cx = context();
-        std::lock_guard<std::mutex> lk(cx.mu);
-        std::vector<uint8_t> stage;
-        const uint8_t* hs = resolve(scalars, npoints, (nbits + 7) / 8, stage);
-        host_mult_wbits(cx, ret, table, wbits, npoints, hs, nbits);
-    } catch (...) {
-        g_err = current_error();
-        std::memset(ret, 0, sizeof(*ret));
-        return;
-    }
-    g_err.clear();
+    mult_wbits_entry<G1Ops>(ret, table, wbits, npoints, scalars, nbits, scratch);
+}
+
+size_t blst_p2s_mult_wbits_precompute_sizeof(size_t wbits, size_t npoints) {
+    return wbits_precompute_bytes<G2Ops>(wbits, npoints);
+}
+
+// one BatchAffineNode<Fp2> (affine point + Fp2, 288 bytes) per point, at
+// most fixed_table_scratch_capacity<Fp2>() = 4096 (multi_scalar.h:218-222,
+// 310-315)
+size_t blst_p2s_mult_wbits_scratch_sizeof(size_t npoints) {
+    return 288 * std::min<size_t>(npoints, 4096);
+}
+
+void blst_p2s_mult_wbits_precompute(blst_p2_affine table[], size_t wbits, const blst_p2_affine* const points[],
+                                    size_t npoints) {
+    wbits_precompute_entry<G2Ops>(table, wbits, points, npoints);
+}
+
+void blst_p2s_mult_wbits(blst_p2* ret, const blst_p2_affine table[], size_t wbits, size_t npoints,
+                         const blst_byte* const scalars[], size_t nbits, blst_limb_t* scratch) {
+    mult_wbits_entry<G2Ops>(ret, table, wbits, npoints, scalars, nbits, scratch);
+}
+
+void blst_p2s_to_affine(blst_p2_affine dst[], const blst_p2* const points[], size_t npoints) {
+    p2s_to_affine(dst, points, npoints);
 }
 
 int blst_hip_p1s_mult_wbits_precompute_device(void* device_table, size_t wbits, const void* device_points,

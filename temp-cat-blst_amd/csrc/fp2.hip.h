@@ -57,4 +57,15 @@ __device__ __forceinline__ void fp_sqr(fp2& r, const fp2& a) {
     fp_dbl(r.c1, m);
 }
 
+// 1 / (a0 + a1 i) = (a0 - a1 i) / (a0^2 + a1^2), one Fp inversion; 0 for 0
+// (a0^2 + a1^2 by fp_mul_sum is below 1.9p, which fp_inv takes)
+__device__ __forceinline__ void fp_inv(fp2& r, const fp2& a) {
+    fp n, ni, t;
+    fp_mul_sum(n, a.c0, a.c0, a.c1, a.c1);
+    fp_inv(ni, n);
+    fp_mul(r.c0, a.c0, ni);
+    fp_neg(t, a.c1);
+    fp_mul(r.c1, t, ni);
+}
+
 }  // namespace bls

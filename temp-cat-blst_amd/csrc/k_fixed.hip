@@ -29,16 +29,17 @@ k_fixed_base(const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, u
 
 // XYZZ -> affine, Q points per lane sharing one inversion of the product of
 // ZZ*ZZZ (= Z^5); prefix products are parked in the output's x field.
+template <class F>
 __global__ void __launch_bounds__(256)
-k_to_affine(const xyzz* __restrict__ in, aff* __restrict__ out, uint32_t n, uint32_t Q) {
+k_to_affine(const xyzz_t<F>* __restrict__ in, aff_t<F>* __restrict__ out, uint32_t n, uint32_t Q) {
     const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t begin = lane * Q;
     if (begin >= n) return;
     const uint32_t end = min(begin + Q, n);
-    fp acc, t;
+    F acc, t;
     fp_one(acc);
     for (uint32_t i = begin; i < end; i++) {
-        xyzz p;
+        xyzz_t<F> p;
         load_xyzz(p, &in[i]);
         out[i].x = acc;
         if (!xyzz_is_inf(p)) {
@@ -46,18 +47,18 @@ k_to_affine(const xyzz* __restrict__ in, aff* __restrict__ out, uint32_t n, uint
             fp_mul(acc, acc, t);
         }
     }
-    fp ia;
+    F ia;
     fp_inv(ia, acc);
     for (uint32_t i = end; i-- > begin;) {
-        xyzz p;
+        xyzz_t<F> p;
         load_xyzz(p, &in[i]);
-        aff o;
+        aff_t<F> o;
         if (xyzz_is_inf(p)) {
             fp_zero(o.x);
             fp_zero(o.y);
         } else {
-            fp it, izz, izzz;
-            const fp pre = out[i].x;
+            F it, izz, izzz;
+            const F pre = out[i].x;
             fp_mul(t, p.zz, p.zzz);
             fp_mul(it, ia, pre);
             fp_mul(ia, ia, t);
@@ -76,8 +77,13 @@ void launch_k_fixed_base(dim3 grid, dim3 block, hipStream_t stream, const uint8_
     hipLaunchKernelGGL(k_fixed_base, grid, block, 0, stream, scalars, n, nbytes, nbits, table, out);
 }
 
-void launch_k_to_affine(dim3 grid, dim3 block, hipStream_t stream, const xyzz* __restrict__ in, aff* __restrict__ out, uint32_t n, uint32_t Q) {
-    hipLaunchKernelGGL(k_to_affine, grid, block, 0, stream, in, out, n, Q);
+template <class F>
+void launch_k_to_affine(dim3 grid, dim3 block, hipStream_t stream, const xyzz_t<F>* __restrict__ in,
+                        aff_t<F>* __restrict__ out, uint32_t n, uint32_t Q) {
+    hipLaunchKernelGGL(k_to_affine<F>, grid, block, 0, stream, in, out, n, Q);
 }
+
+template void launch_k_to_affine<fp>(dim3, dim3, hipStream_t, const xyzz*, aff*, uint32_t, uint32_t);
+template void launch_k_to_affine<fp2>(dim3, dim3, hipStream_t, const xyzz2*, aff2*, uint32_t, uint32_t);
 
 }  // namespace bls

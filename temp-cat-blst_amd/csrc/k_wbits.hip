@@ -1,5 +1,7 @@
-// Fixed-base tables (blst_p1s_mult_wbits_precompute / blst_p1s_mult_wbits,
-// blst/scalar/curve/multi_scalar.cpp:268-286, 481-707).
+// Fixed-base tables (blst_p1s_mult_wbits_precompute / blst_p1s_mult_wbits and
+// their G2 forms, blst/scalar/curve/multi_scalar.cpp:268-286, 481-707).  The
+// row, sum and tree kernels are templates over the field (fp: G1, fp2: G2);
+// the comb kernels below them are G1 only.
 //
 // A table holds, for each point P_i, the row [1]P_i ... [m]P_i (m =
 // 2^(wbits-1)) in affine form.  The MSM over a table needs no buckets: with
@@ -22,6 +24,14 @@
 
 namespace bls {
 
+__device__ __forceinline__ void load_aff_at(aff2& a, const uint4* __restrict__ base, size_t idx) {
+    const uint4* s = base + idx * 12;
+    fp_from_uint4(a.x.c0, s);
+    fp_from_uint4(a.x.c1, s + 3);
+    fp_from_uint4(a.y.c0, s + 6);
+    fp_from_uint4(a.y.c1, s + 9);
+}
+
 __device__ __forceinline__ void load_aff_at(aff& a, const uint4* __restrict__ base, size_t idx) {
     const uint4* s = base + idx * 6;
 #pragma unroll
@@ -34,16 +44,17 @@ __device__ __forceinline__ void load_aff_at(aff& a, const uint4* __restrict__ ba
 
 // rows [row0, row0 + nrows) of the table, as XYZZ multiples: out[r * m + j]
 // = [j + 1] P_{row0 + r}; all-zero points give infinity entries.
+template <class F>
 __global__ void __launch_bounds__(256)
 k_wbits_rows(const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint32_t m, uint32_t L,
-             xyzz* __restrict__ out) {
+             xyzz_t<F>* __restrict__ out) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= nrows * L) return;
     const uint32_t r = g / L, t = g % L, S = m / L;
-    aff P;
+    aff_t<F> P;
     load_aff_at(P, pts, (size_t)row0 + r);
-    xyzz* dst = out + (size_t)r * m + (size_t)t * S;
-    xyzz acc;
+    xyzz_t<F>* dst = out + (size_t)r * m + (size_t)t * S;
+    xyzz_t<F> acc;
     xyzz_set_inf(acc);
     if (aff_is_inf(P)) {
         for (uint32_t s = 0; s < S; s++) store_xyzz(&dst[s], acc);
@@ -67,19 +78,20 @@ k_wbits_rows(const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint3
 // (device memory, n rows of m entries) the lane computes the Booth digit of
 // each scalar itself; without one, list holds the D * n entries already
 // gathered and signed (window-major, all-zero = nothing to add).
+template <class F>
 __global__ void __launch_bounds__(256)
 k_wbits_sum(const uint4* __restrict__ table, const uint8_t* __restrict__ scalars, uint32_t nbytes,
             uint32_t nbits, uint32_t vec, const uint4* __restrict__ list, uint32_t n, uint32_t wbits, uint32_t D,
-            uint32_t C, uint32_t nch, xyzz* __restrict__ part) {
+            uint32_t C, uint32_t nch, xyzz_t<F>* __restrict__ part) {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= D * nch) return;
     const uint32_t w = g / nch, c = g % nch;
     const uint32_t i0 = c * C, i1 = min(i0 + C, n);
     const size_t m = (size_t)1 << (wbits - 1);
-    xyzz acc;
+    xyzz_t<F> acc;
     xyzz_set_inf(acc);
     for (uint32_t i = i0; i < i1; i++) {
-        aff a;
+        aff_t<F> a;
         if (table) {
             const int32_t d = booth_digit_at(scalars + (size_t)i * nbytes, vec != 0, w * wbits, wbits, nbits);
             if (d == 0) continue;
@@ -95,44 +107,45 @@ k_wbits_sum(const uint4* __restrict__ table, const uint8_t* __restrict__ scalars
     store_xyzz(&part[g], acc);
 }
 
-constexpr uint32_t kWbitsTreeThreads = 1024;
-constexpr uint32_t kWbitsTreeQuads = kWbitsTreeThreads / 4;
-
-// Window w's nch partials -> W[w] (canonical XYZZ); one workgroup per window.
-__global__ void __launch_bounds__(kWbitsTreeThreads)
-k_wbits_tree(const xyzz* __restrict__ part, uint32_t nch, xyzz* __restrict__ W) {
-    __shared__ xyzz node[kWbitsTreeQuads];
+// Window w's nch partials -> W[w] (canonical XYZZ); one workgroup of
+// THREADS per window (1024 for G1, 512 for G2, whose LDS nodes are twice as
+// large).
+template <class F, uint32_t THREADS>
+__global__ void __launch_bounds__(THREADS)
+k_wbits_tree(const xyzz_t<F>* __restrict__ part, uint32_t nch, xyzz_t<F>* __restrict__ W) {
+    constexpr uint32_t kQuads = THREADS / 4;
+    __shared__ xyzz_t<F> node[kQuads];
     const uint32_t w = blockIdx.x;
     const uint32_t u = threadIdx.x >> 2, ln = threadIdx.x & 3;
-    const xyzz* src = part + (size_t)w * nch;
-    fp acc;
+    const xyzz_t<F>* src = part + (size_t)w * nch;
+    F acc;
     fp_zero(acc);
-    for (uint32_t j = u; j < nch; j += kWbitsTreeQuads) {
-        fp x;
+    for (uint32_t j = u; j < nch; j += kQuads) {
+        F x;
         quad_load(x, &src[j]);
         quad_add(acc, acc, x);
     }
-    reinterpret_cast<fp*>(&node[u])[ln] = acc;
+    reinterpret_cast<F*>(&node[u])[ln] = acc;
     __syncthreads();
     // pairwise tree over the quads that hold partials; at least four quads
     // (16 lanes) run each level, the extra ones repeating a merge without
     // storing it
     uint32_t width = 1;
-    while (width < min(nch, kWbitsTreeQuads)) width <<= 1;
+    while (width < min(nch, kQuads)) width <<= 1;
     for (uint32_t half = width / 2; half >= 1; half >>= 1) {
-        fp r;
+        F r;
         if (u < max(half, 4u)) {
             const uint32_t v = u % half;
-            const fp a = reinterpret_cast<const fp*>(&node[v])[ln];
-            const fp b = reinterpret_cast<const fp*>(&node[v + half])[ln];
+            const F a = reinterpret_cast<const F*>(&node[v])[ln];
+            const F b = reinterpret_cast<const F*>(&node[v + half])[ln];
             quad_add(r, a, b);
         }
         __syncthreads();
-        if (u < half) reinterpret_cast<fp*>(&node[u])[ln] = r;
+        if (u < half) reinterpret_cast<F*>(&node[u])[ln] = r;
         __syncthreads();
     }
     if (u == 0) {
-        fp r = reinterpret_cast<const fp*>(&node[0])[ln];
+        F r = reinterpret_cast<const F*>(&node[0])[ln];
         fp_canon(r, r);
         quad_store(&W[w], r);
     }
@@ -204,23 +217,36 @@ void launch_k_comb_sum(hipStream_t stream, const uint4* __restrict__ table, cons
                        vec, n, wbits, D, njobs, C, nch, part);
 }
 
+template <class F>
 void launch_k_wbits_rows(hipStream_t stream, const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint32_t m,
-                         uint32_t L, xyzz* __restrict__ out) {
+                         uint32_t L, xyzz_t<F>* __restrict__ out) {
     const uint32_t lanes = nrows * L;
-    hipLaunchKernelGGL(k_wbits_rows, dim3((lanes + 255) / 256), dim3(256), 0, stream, pts, row0, nrows, m, L, out);
+    hipLaunchKernelGGL(k_wbits_rows<F>, dim3((lanes + 255) / 256), dim3(256), 0, stream, pts, row0, nrows, m, L, out);
 }
 
+template <class F>
 void launch_k_wbits_sum(hipStream_t stream, const uint4* __restrict__ table, const uint8_t* __restrict__ scalars,
                         uint32_t nbytes, uint32_t nbits, uint32_t vec, const uint4* __restrict__ list, uint32_t n,
-                        uint32_t wbits, uint32_t D, uint32_t C, uint32_t nch, xyzz* __restrict__ part) {
+                        uint32_t wbits, uint32_t D, uint32_t C, uint32_t nch, xyzz_t<F>* __restrict__ part) {
     const uint32_t lanes = D * nch;
-    hipLaunchKernelGGL(k_wbits_sum, dim3((lanes + 255) / 256), dim3(256), 0, stream, table, scalars, nbytes, nbits,
+    hipLaunchKernelGGL(k_wbits_sum<F>, dim3((lanes + 255) / 256), dim3(256), 0, stream, table, scalars, nbytes, nbits,
                        vec, list, n, wbits, D, C, nch, part);
 }
 
-void launch_k_wbits_tree(hipStream_t stream, const xyzz* __restrict__ part, uint32_t nch, uint32_t D,
-                         xyzz* __restrict__ W) {
-    hipLaunchKernelGGL(k_wbits_tree, dim3(D), dim3(kWbitsTreeThreads), 0, stream, part, nch, W);
+template <class F>
+void launch_k_wbits_tree(hipStream_t stream, const xyzz_t<F>* __restrict__ part, uint32_t nch, uint32_t D,
+                         xyzz_t<F>* __restrict__ W) {
+    constexpr uint32_t threads = sizeof(F) == sizeof(fp) ? 1024 : 512;
+    hipLaunchKernelGGL((k_wbits_tree<F, threads>), dim3(D), dim3(threads), 0, stream, part, nch, W);
 }
+
+#define BLS_WBITS_INSTANCES(F)                                                                                      \
+    template void launch_k_wbits_rows<F>(hipStream_t, const uint4*, uint32_t, uint32_t, uint32_t, uint32_t,        \
+                                         xyzz_t<F>*);                                                               \
+    template void launch_k_wbits_sum<F>(hipStream_t, const uint4*, const uint8_t*, uint32_t, uint32_t, uint32_t,   \
+                                        const uint4*, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, xyzz_t<F>*); \
+    template void launch_k_wbits_tree<F>(hipStream_t, const xyzz_t<F>*, uint32_t, uint32_t, xyzz_t<F>*);
+BLS_WBITS_INSTANCES(fp)
+BLS_WBITS_INSTANCES(fp2)
 
 }  // namespace bls

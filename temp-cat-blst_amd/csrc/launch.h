@@ -36,10 +36,15 @@ void launch_k_fold(hipStream_t stream, const job_t* __restrict__ jobs, uint32_t 
 void launch_k_fixed_base(dim3 grid, dim3 block, hipStream_t stream, const uint8_t* __restrict__ scalars, uint32_t n, uint32_t nbytes, uint32_t nbits, const uint4* __restrict__ table, xyzz* __restrict__ out);
 void launch_k_shift_points(hipStream_t stream, const uint4* __restrict__ in, uint32_t n, uint32_t s, xyzz* __restrict__ out);
 void launch_k_comb_sum(hipStream_t stream, const uint4* __restrict__ table, const uint8_t* __restrict__ scalars, uint32_t nbytes, uint32_t nbits, uint32_t vec, uint32_t n, uint32_t wbits, uint32_t D, uint32_t njobs, uint32_t C, uint32_t nch, xyzz* __restrict__ part);
-void launch_k_wbits_rows(hipStream_t stream, const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint32_t m, uint32_t L, xyzz* __restrict__ out);
-void launch_k_wbits_sum(hipStream_t stream, const uint4* __restrict__ table, const uint8_t* __restrict__ scalars, uint32_t nbytes, uint32_t nbits, uint32_t vec, const uint4* __restrict__ list, uint32_t n, uint32_t wbits, uint32_t D, uint32_t C, uint32_t nch, xyzz* __restrict__ part);
-void launch_k_wbits_tree(hipStream_t stream, const xyzz* __restrict__ part, uint32_t nch, uint32_t D, xyzz* __restrict__ W);
-void launch_k_jac_to_affine(hipStream_t stream, const uint4* __restrict__ in, aff* __restrict__ out, uint32_t n, uint32_t T);
-void launch_k_to_affine(dim3 grid, dim3 block, hipStream_t stream, const xyzz* __restrict__ in, aff* __restrict__ out, uint32_t n, uint32_t Q);
+template <class F>
+void launch_k_wbits_rows(hipStream_t stream, const uint4* __restrict__ pts, uint32_t row0, uint32_t nrows, uint32_t m, uint32_t L, xyzz_t<F>* __restrict__ out);
+template <class F>
+void launch_k_wbits_sum(hipStream_t stream, const uint4* __restrict__ table, const uint8_t* __restrict__ scalars, uint32_t nbytes, uint32_t nbits, uint32_t vec, const uint4* __restrict__ list, uint32_t n, uint32_t wbits, uint32_t D, uint32_t C, uint32_t nch, xyzz_t<F>* __restrict__ part);
+template <class F>
+void launch_k_wbits_tree(hipStream_t stream, const xyzz_t<F>* __restrict__ part, uint32_t nch, uint32_t D, xyzz_t<F>* __restrict__ W);
+template <class F>
+void launch_k_jac_to_affine(hipStream_t stream, const uint4* __restrict__ in, aff_t<F>* __restrict__ out, uint32_t n, uint32_t T);
+template <class F>
+void launch_k_to_affine(dim3 grid, dim3 block, hipStream_t stream, const xyzz_t<F>* __restrict__ in, aff_t<F>* __restrict__ out, uint32_t n, uint32_t Q);
 
 }  // namespace bls

@@ -260,6 +260,62 @@ def test_wbits_sizes_and_checks_without_gpu_work(orc):
         assert (tab == 7).all(), args
 
 
+def test_g2_wbits_sizes_and_checks_without_gpu_work():
+    """G2 fixed-base sizes: points_mult_wbits_precompute_sizeof<Fp2>
+    (multi_scalar.h:244-256; sizeof(AffinePoint<Fp2>) = 192, zero for wbits
+    outside 2..14 or on overflow) and points_mult_wbits_scratch_sizeof<Fp2>
+    (multi_scalar.h:310-315: BatchAffineNode<Fp2>, an affine point and an Fp2,
+    per point up to 4096, multi_scalar.h:218-222), with the node size from C
+    layout rules; then the argument checks of c_api.cpp:1806-1888 that return
+    before any device call."""
+    import ctypes
+    import blst_hip
+
+    class Node(ctypes.Structure):
+        _fields_ = [("point", ctypes.c_uint64 * 24), ("work", ctypes.c_uint64 * 12)]
+
+    L = blst_hip.load()
+    for wb in range(0, 17):
+        for n in (0, 1, 5, 4096, 1 << 20, (1 << 64) - 1, 1 << 50, 1 << 56):
+            want = 0
+            if 2 <= wb <= 14 and 192 * n < (1 << 64) and (192 * n) << (wb - 1) < (1 << 64):
+                want = (192 * n) << (wb - 1)
+            assert L.blst_p2s_mult_wbits_precompute_sizeof(wb, n) == want, (wb, n)
+    for n in (0, 1, 100, 4095, 4096, 4097, 1 << 30):
+        assert L.blst_p2s_mult_wbits_scratch_sizeof(n) == ctypes.sizeof(Node) * min(n, 4096) == 288 * min(n, 4096)
+    P = blst_hip._ptr
+    sc = np.zeros((2, 32), np.uint8)
+    ss = np.array([sc.ctypes.data, 0], np.uint64)
+    tab = np.zeros((4, 24), np.uint64)
+    scr = np.zeros(128, np.uint64)
+    ret = np.full(36, 7, np.uint64)
+    L.blst_p2s_mult_wbits(P(ret), P(tab), 2, 0, P(ss), 255, P(scr))
+    assert not ret.any()
+    for args in ((P(tab), 2, 2, P(ss), 0, P(scr)), (P(tab), 2, 2, P(ss), 257, P(scr)),
+                 (P(tab), 1, 2, P(ss), 255, P(scr)), (P(tab), 15, 2, P(ss), 255, P(scr)),
+                 (None, 2, 2, P(ss), 255, P(scr)), (P(tab), 2, 2, None, 255, P(scr)),
+                 (P(tab), 2, 2, P(np.zeros(2, np.uint64)), 255, P(scr)), (P(tab), 2, 2, P(ss), 255, None)):
+        ret[:] = 7
+        L.blst_p2s_mult_wbits(P(ret), *args)
+        assert not ret.any(), args
+    pts = np.ones((2, 24), np.uint64)
+    pp = np.array([pts.ctypes.data, 0], np.uint64)
+    tab[:] = 7
+    for args in ((P(tab), 1, P(pp), 2), (P(tab), 15, P(pp), 2), (None, 2, P(pp), 2),
+                 (P(tab), 2, None, 2), (P(tab), 2, P(np.zeros(2, np.uint64)), 2), (P(tab), 2, P(pp), 0)):
+        L.blst_p2s_mult_wbits_precompute(*args)
+        assert (tab == 7).all(), args
+    # batch normalisation: no points leaves dst alone, a NULL array or head
+    # writes all-zero points (c_api.cpp:1806-1817)
+    dst = np.full((3, 24), 7, np.uint64)
+    L.blst_p2s_to_affine(P(dst), P(pp), 0)
+    assert (dst == 7).all()
+    for head in (None, P(np.zeros(2, np.uint64))):
+        dst[:] = 7
+        L.blst_p2s_to_affine(P(dst), head, 3)
+        assert not dst.any()
+
+
 def test_g2_scratch_sizes_and_checks(orc):
     """G2 scratch sizes (c_api.cpp:1890-1899: 384-byte Pointxyzz<Fp2>
     buckets) and the early returns that need no device."""

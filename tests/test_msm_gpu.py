@@ -900,6 +900,62 @@ def test_g2_fr_capacity_scratch_2pow16(hip, orc):
     assert orc.p2_compress(got) == orc.p2_compress(hip.p2s_mult_pippenger(pts, sc, 255))
 
 
+@pytest.mark.parametrize("wbits", [2, 4, 8])
+def test_g2_wbits_table_matches_oracle(hip, orc, g2_points, wbits):
+    """blst_p2s_mult_wbits_precompute writes the oracle's table bit for bit
+    (canonical affine rows of [1]P .. [2^(wbits-1)]P; an infinity point gives
+    an all-zero row)."""
+    pts = np.ascontiguousarray(g2_points[:37]).copy()
+    pts[5] = 0
+    got = hip.p2_wbits_precompute(pts, wbits)
+    assert np.array_equal(got, orc.p2_wbits_precompute(pts, wbits))
+
+
+@pytest.mark.parametrize("wbits,nbits", [(2, 255), (4, 64), (5, 256), (8, 255), (13, 17)])
+def test_g2_wbits_msm_matches_oracle(hip, orc, g2_points, wbits, nbits):
+    """blst_p2s_mult_wbits over the oracle's table equals the oracle's table
+    MSM and its G2 Pippenger MSM, with zero scalars, infinity and P, -P."""
+    n = 300
+    pts = np.ascontiguousarray(g2_points[:n]).copy()
+    pts[3] = 0
+    pts[7] = orc.p2_to_affine(orc.p2_mult(orc.p2_from_affine(pts[6]), (R - 1).to_bytes(32, "little"), 255))
+    rnd = random.Random(wbits * 1000 + nbits)
+    nb = (nbits + 7) // 8
+    ks = [rnd.getrandbits(nbits) for _ in range(n)]
+    ks[0] = 0
+    ks[7] = ks[6]
+    S = _scal(ks, nb)
+    tab = orc.p2_wbits_precompute(pts, wbits)
+    want = orc.p2_compress(orc.msm_p2_pippenger(pts, S, n, nbits))
+    assert orc.p2_compress(orc.p2_mult_wbits(tab, wbits, S, n, nbits)) == want
+    assert orc.p2_compress(hip.p2_mult_wbits(tab, wbits, S, nbits)) == want
+
+
+@pytest.mark.parametrize("mode", ["exact", "mixed"])
+def test_g2_wbits_descriptors(hip, orc, g2_points, mode):
+    n = 40
+    pts = np.ascontiguousarray(g2_points[:n])
+    S = orc.gen_scalars(n, 0x3A)
+    tab = hip.p2_wbits_precompute(pts, 4, mode=mode, k=7)
+    assert np.array_equal(tab, orc.p2_wbits_precompute(pts, 4))
+    want = orc.p2_compress(orc.msm_p2_pippenger(pts, S, n, 255))
+    assert orc.p2_compress(hip.p2_mult_wbits(tab, 4, S, 255, mode=mode, k=7)) == want
+
+
+@pytest.mark.parametrize("n", [1, 5, 1000, 1 << 16])
+def test_g2_to_affine_matches_oracle(hip, orc, g2_points, n):
+    """blst_p2s_to_affine against the oracle's batch normalisation, with
+    Z = 0 points (all-zero output) among Jacobian inputs of varied Z."""
+    G = orc.p2_from_affine(orc.p2_generator_affine())
+    base = np.stack([orc.p2_mult(G, (1000003 * i + 7).to_bytes(8, "little"), 64) for i in range(64)])
+    jac = np.ascontiguousarray(np.tile(base, (-(-n // 64), 1))[:n]).copy()
+    jac[::17] = 0
+    got = hip.p2s_to_affine(jac)
+    assert np.array_equal(got, orc.p2s_to_affine(jac))
+    if n == 1000:
+        assert np.array_equal(hip.p2s_to_affine(jac, mode="exact"), got)
+
+
 def test_g2_hash_to_curve_points_textbook(hip):
     """The 10 G2 hash-to-curve outputs as bases with small scalars, against
     the textbook Fp2 model."""

@@ -574,3 +574,61 @@ def test_g2_msm_against_textbook(orc, n, nbits):
             if p.any() and dig:
                 wt = pyref2.add(wt, pyref2.mul(dig % pyref.R_ORDER, pyref2.limbs_to_affine(list(p))))
         assert orc.p2_compress(t) == pyref2.compress(wt)
+
+
+def test_table_window_digits_reconstruct_the_scalar(orc):
+    """The digits of msm_mult_table's windows (or_table_booth: the top window
+    holds nbits % wbits bits) recombine to the scalar itself:
+    sum_w d_w 2^(rem_w) = k for every nbits and wbits."""
+    rnd = random.Random(23)
+    for nbits in (1, 2, 7, 8, 13, 64, 129, 255, 256):
+        nb = (nbits + 7) // 8
+        for wbits in (2, 3, 4, 7, 13, 14):
+            for k in [0, 1, (1 << nbits) - 1, 1 << (nbits - 1)] + [rnd.getrandbits(nbits) for _ in range(20)]:
+                s = np.array(list(k.to_bytes(nb, "little")), np.uint8)
+                total, rem = 0, nbits - nbits % wbits
+                while True:
+                    e = orc.table_booth(s, nbits, wbits, rem)
+                    mag = e & ((1 << wbits) - 1)
+                    assert mag <= 1 << (wbits - 1)
+                    total += (-mag if (e >> wbits) & 1 else mag) << rem
+                    if rem == 0:
+                        break
+                    rem -= wbits
+                assert total == k, (nbits, wbits, k)
+
+
+def test_g2_wbits_tables_and_msm(orc):
+    """G2 table rows hold [1]P ... [2^(wbits-1)]P (against the oracle's
+    doubling-and-add scalar multiplication, pinned above against the textbook
+    Fp2 model), an all-zero point gives all-zero rows, and the table MSM
+    equals the G2 Pippenger MSM (pinned above) over nbits and wbits, with
+    zero scalars, infinity, P and -P."""
+    pts = orc.gen_p2_points(5, 7)
+    pts[2] = 0
+    pts[3] = orc.p2_to_affine(orc.p2_mult(orc.p2_from_affine(pts[1]), (pyref.R_ORDER - 1).to_bytes(32, "little"),
+                                          255))
+    for wb in (2, 3, 5):
+        tab = orc.p2_wbits_precompute(pts, wb)
+        m = 1 << (wb - 1)
+        for i in range(5):
+            for j in range(m):
+                e = tab[i * m + j]
+                if i == 2:
+                    assert not e.any()
+                else:
+                    want = orc.p2_to_affine(orc.p2_mult(orc.p2_from_affine(pts[i]), (j + 1).to_bytes(2, "little"), 16))
+                    assert np.array_equal(e, want), (wb, i, j)
+    rnd = random.Random(29)
+    for nbits in (1, 5, 8, 64, 255, 256):
+        nb = (nbits + 7) // 8
+        ks = [rnd.getrandbits(8 * nb) for _ in range(5)]
+        ks[0] = 0
+        ks[3] = ks[1]
+        S = np.array([list(k.to_bytes(nb, "little")) for k in ks], np.uint8)
+        want = orc.p2_compress(orc.msm_p2_pippenger(pts, S, 5, nbits))
+        for wb in (2, 4, 7, 10):
+            if wb == 10 and nbits < 64:
+                continue
+            r = orc.p2_mult_wbits(orc.p2_wbits_precompute(pts, wb), wb, S, 5, nbits)
+            assert orc.p2_compress(r) == want, (nbits, wb)
