@@ -1644,7 +1644,7 @@ size_t comb_table_bytes(size_t npoints, size_t wbits, size_t nbits) {
 // address of each GPU's copy comes from hipGetSymbolAddress, once per device
 // with that device current, and only that address is handed to kernels.
 constexpr size_t kNoAllocMaxPoints = (size_t)1 << 16;
-constexpr size_t kNoAllocScratch = (size_t)88 << 20;   // the committed tuning table needs at most 86.2 MB (65535 points, 256 bits)
+constexpr size_t kNoAllocScratch = (size_t)96 << 20;   // the committed tuning table needs at most 90.0 MiB (65535 points, 256 bits)
 // host staging of the layout tables, then the window sums
 constexpr size_t kNoAllocHostBytes = (size_t)1 << 20;
 
