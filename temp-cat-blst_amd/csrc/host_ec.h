@@ -38,42 +38,50 @@ inline void sub_p_if_ge(fp6& r, const uint64_t t[7]) {
     for (int j = 0; j < 6; j++) r.l[j] = keep ? t[j] : d[j];
 }
 
-// Montgomery product by separated operand scanning: the full 768-bit
-// product first (column sums of 64 x 64 -> 128-bit products in a 192-bit
-// accumulator), then six word-by-word reduction rounds t += m p 2^(64i) with
-// m = t_i (-1/p) mod 2^64, the result being words 6..11 less p if needed.
+// Column accumulator of the product below: three words, bits 0..191.
+struct acc192 { uint64_t lo, hi, ov; };
+
+// s += x y.  The high word of a 64 x 64-bit product is at most 2^64 - 2, so
+// the carry of the low word added to it cannot wrap.
+inline void mac(acc192& s, uint64_t x, uint64_t y) {
+    const u128 p = (u128)x * y;
+    unsigned long long t;
+    const uint64_t hi = (uint64_t)(p >> 64) + __builtin_uaddll_overflow(s.lo, (uint64_t)p, &t);
+    s.lo = t;
+    s.ov += __builtin_uaddll_overflow(s.hi, hi, &t);
+    s.hi = t;
+}
+
+// Montgomery product by product scanning with the reduction interleaved per
+// column (the order of the device product in fp.hip.h, on 64-bit words):
+// column k gathers a_i b_(k-i) and m_i p_(k-i) in the 192-bit accumulator,
+// the low word of column k < 6 fixes m_k = lo (-1/p) mod 2^64 (which clears
+// it), and columns 6..11 are the result words, less p if needed.  Inputs
+// below 2p give a canonical result (a b + m p < 4p^2 + 2^384 p < 2^384 2p).
 inline void mul(fp6& r, const fp6& a, const fp6& b) {
-    uint64_t t[13];
-    u128 acc = 0;
-    uint64_t over = 0;   // bits 128..191 of the column accumulator
-    for (int k = 0; k < 11; k++) {
-        const int lo = k < 6 ? 0 : k - 5, hi = k < 6 ? k : 5;
-        for (int i = lo; i <= hi; i++) {
-            const u128 p = (u128)a.l[i] * b.l[k - i];
-            acc += p;
-            over += acc < p;
+    uint64_t m[6], t[7];
+    acc192 s = {0, 0, 0};
+    for (int k = 0; k < 6; k++) {
+        for (int i = 0; i < k; i++) {
+            mac(s, a.l[i], b.l[k - i]);
+            mac(s, m[i], kP[k - i]);
         }
-        t[k] = (uint64_t)acc;
-        acc = (acc >> 64) | ((u128)over << 64);
-        over = 0;
+        mac(s, a.l[k], b.l[0]);
+        m[k] = s.lo * kN0;
+        mac(s, m[k], kP[0]);
+        s = {s.hi, s.ov, 0};
     }
-    t[11] = (uint64_t)acc;
-    t[12] = 0;
-    for (int i = 0; i < 6; i++) {
-        const uint64_t m = t[i] * kN0;
-        uint64_t c = 0;
-        for (int j = 0; j < 6; j++) {
-            const u128 s = (u128)m * kP[j] + t[i + j] + c;
-            t[i + j] = (uint64_t)s;
-            c = (uint64_t)(s >> 64);
+    for (int k = 6; k < 11; k++) {
+        for (int i = k - 5; i < 6; i++) {
+            mac(s, a.l[i], b.l[k - i]);
+            mac(s, m[i], kP[k - i]);
         }
-        for (int j = i + 6; c && j < 13; j++) {
-            const u128 s = (u128)t[j] + c;
-            t[j] = (uint64_t)s;
-            c = (uint64_t)(s >> 64);
-        }
+        t[k - 6] = s.lo;
+        s = {s.hi, s.ov, 0};
     }
-    sub_p_if_ge(r, t + 6);
+    t[5] = s.lo;
+    t[6] = s.hi;
+    sub_p_if_ge(r, t);
 }
 
 inline void add(fp6& r, const fp6& a, const fp6& b) {
