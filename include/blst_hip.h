@@ -260,8 +260,11 @@ void blst_hip_batch_plan_destroy(blst_hip_batch_plan *plan);
  * plus batch workspace; 0 = 80% of the free memory at creation) and
  * max_batch_points the points of one batch (0 = 2^24).  Requests are admitted
  * in arrival order as memory frees up, copied on a copy stream, and launched
- * in batches on two alternating streams, so requests that arrive while a
- * batch runs join the next one.  Results are identical to a lone MSM. */
+ * in batches queued on one compute stream (at most two on the device while
+ * copies, batch set-up and completions overlap them), so requests that arrive
+ * while a batch runs join the next one.  Results are identical to a lone MSM.
+ * submit, wait and stats may be called from any threads.  destroy finishes
+ * the requests in flight first; call it once no thread is inside wait. */
 typedef struct blst_hip_server blst_hip_server;
 blst_hip_server *blst_hip_server_create(size_t hbm_budget_bytes, size_t max_batch_points);
 void blst_hip_server_destroy(blst_hip_server *server);
@@ -275,7 +278,9 @@ int blst_hip_server_add_bases(blst_hip_server *server, const blst_p1_affine *poi
 int64_t blst_hip_server_submit(blst_hip_server *server, int base_id, const blst_p1_affine *points,
                                const blst_byte *scalars, size_t npoints, size_t nbits);
 /* Blocks until the ticket completes; writes the result and the latency from
- * submission to completion (ms).  Returns 0 on success. */
+ * submission to completion (ms), and forgets the ticket (a later wait for it
+ * returns -1, "unknown ticket"; threads already waiting for it all get the
+ * result).  Returns 0 on success. */
 int blst_hip_server_wait(blst_hip_server *server, int64_t ticket, blst_p1 *ret, double *latency_ms);
 /* [submitted, completed, batches, max batches in flight, peak admitted
  * bytes, HBM budget] */

@@ -282,7 +282,9 @@ public:
         done_cv_.wait(lk, [&] { return r->done; });
         *ret = r->result;
         if (latency_ms) *latency_ms = r->latency_ms;
-        tickets_.erase(it);
+        // by key: another thread waiting for the same ticket may have
+        // erased it already (it holds r, which stays valid)
+        tickets_.erase(ticket);
         if (r->status) err = "request failed: " + fail_msg_;
         return r->status;
     }

@@ -185,3 +185,34 @@ def test_server_rejects_what_it_cannot_run(hip, inputs):
         assert hip.compress(got) != bytes(48)
     finally:
         srv.close()
+
+
+def test_server_ticket_waited_from_two_threads(hip, orc, inputs):
+    """Two threads waiting for one ticket both get the result (the ticket is
+    forgotten by key once, not erased twice); a later wait is refused."""
+    pts, sc = inputs
+    want = orc.compress(orc.msm_pippenger(pts, sc, pts.shape[0], 255, threads=16))
+    srv = hip.Server()
+    try:
+        t = srv.submit(sc, 255, points=pts)   # 2^19 + 77 fresh points: both threads wait well before it ends
+        got, errors = [], []
+
+        def waiter():
+            try:
+                got.append(hip.compress(srv.wait(t)[0]))
+            except RuntimeError as e:         # the other thread forgot the ticket first
+                errors.append(str(e))
+
+        threads = [threading.Thread(target=waiter) for _ in range(2)]
+        for th in threads:
+            th.start()
+        for th in threads:
+            th.join()
+        assert got and all(g == want for g in got), (len(got), errors)
+        assert all("unknown ticket" in e for e in errors), errors
+        with pytest.raises(RuntimeError, match="unknown ticket"):
+            srv.wait(t)
+        got2, _ = srv.wait(srv.submit(sc[:64], 255, points=pts[:64]))   # still serving
+        assert hip.compress(got2) == orc.compress(orc.msm_pippenger(pts[:64], sc[:64], 64, 255))
+    finally:
+        srv.close()
