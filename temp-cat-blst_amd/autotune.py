@@ -4,7 +4,8 @@ For every MSM size class 2^k it measures, on the GPU it runs on, the window
 width c (which is also the radix of the one-pass counting sort: 2^(c-1)
 bucket keys per window), the accumulation tile T (sorted bucket entries per
 lane) and the reduction tile L (buckets per lane), by coordinate descent
-(c first at the default tile, then T, then L), separately for
+(every window near the starting one with the small tiles T and every
+segment L, then the larger tiles), separately for
 
   single  one MSM per call (latency of the whole pipeline), and
   batch   a job inside a batch of same-size jobs totalling about 2^21 points
@@ -13,7 +14,7 @@ lane) and the reduction tile L (buckets per lane), by coordinate descent
 and writes the table the library loads at start-up (tuning.txt next to
 libblst_hip.so, or the path in BLST_HIP_TUNING).
 
-    python3 temp-cat-blst_amd/autotune.py [--out PATH] [--single 6-20] [--batch 6-20]
+    python3 temp-cat-blst_amd/autotune.py [--out PATH] [--single 6-20] [--batch 6-20|none]
 
 With --nbits 64 or 128 it measures lone MSMs of scalars that wide (rows
 "single64" / "single128", used for 33..64-bit and 65..128-bit scalars);
@@ -31,8 +32,12 @@ import numpy as np  # noqa: E402
 
 import blst_hip as bh  # noqa: E402
 
-T_CHOICES = (16, 32, 64, 128)
+T_CHOICES = (8, 16, 32, 64, 128)
 L_CHOICES = (2, 4, 8, 16)
+# tiles tried together with each window: small lone MSMs are latency-bound,
+# and their best window depends on the tile (a window fixed at a tile of 32
+# first left 2^6 .. 2^14 points about 20% slower than their best)
+T_JOINT = (8, 16, 32)
 BATCH_POINTS = 1 << 21
 
 
@@ -47,22 +52,21 @@ def _median_time(fn, reps):
 
 
 def _descend(measure, c0):
-    """coordinate descent over c, then T, then L"""
+    """every (window, small tile, segment) combination around the starting
+    window, then every tile at the best window and segment"""
     best = None
-    for c in range(max(1, c0 - 2), min(20, c0 + 2) + 1):
-        t = measure(c, 32, 8)
-        if best is None or t < best[0]:
-            best = (t, c, 32, 8)
+    for c in range(max(1, c0 - 3), min(20, c0 + 3) + 1):
+        for T in T_JOINT:
+            for L in L_CHOICES:
+                t = measure(c, T, L)
+                if best is None or t < best[0]:
+                    best = (t, c, T, L)
     _, c, T, L = best
     for T2 in T_CHOICES:
-        t = measure(c, T2, L)
-        if t < best[0]:
-            best = (t, c, T2, L)
-    _, c, T, L = best
-    for L2 in L_CHOICES:
-        t = measure(c, T, L2)
-        if t < best[0]:
-            best = (t, c, T, L2)
+        if T2 not in T_JOINT:
+            t = measure(c, T2, L)
+            if t < best[0]:
+                best = (t, c, T2, L)
     return best
 
 
@@ -118,6 +122,8 @@ def write_table(rows, path, append=False):
 
 
 def _range(s):
+    if s == "none":
+        return range(0)
     a, b = s.split("-")
     return range(int(a), int(b) + 1)
 

@@ -53,12 +53,13 @@ def test_noalloc_equals_device_msm_on_calls_in_a_row(hip, orc, device_inputs):
 
 
 def test_noalloc_every_tuned_size_class_fits(hip, device_inputs):
-    """The largest size of every size class and scalar widths of 256, 64 and
-    1 bits (the window, and so the scratch a call needs, follows the class)."""
+    """The largest size of every size class and scalar widths of 256, 128, 64
+    and 1 bits (the window and tile, and so the scratch a call needs, follow
+    the class and the width's row of the tuning table)."""
     d_p, d_s, _, _ = device_inputs
     for k in range(16, 4, -1):
         n = min((2 << k) - 1, MAX_POINTS)
-        for nbits in (256, 64, 1):
+        for nbits in (256, 128, 64, 1):
             rc, got = _noalloc(hip, d_p.ptr, d_s.ptr, n, nbits)
             assert rc == 0, (n, nbits, hip.last_error())
             assert hip.compress(got) == hip.compress(_plain(hip, d_p.ptr, d_s.ptr, n, nbits)), (n, nbits)
