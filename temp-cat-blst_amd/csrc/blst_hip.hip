@@ -537,6 +537,19 @@ Layout make_layout(const std::vector<JobSpec>& specs) {
     L.total_entries = (uint32_t)ent;
     L.nkeys = (uint32_t)keys;
     L.total_segments = (uint32_t)segs;
+    // a dispatch counts its work-items in 32 bits: every launch of the batch
+    // must stay below 2^32 of them (many jobs of many narrow windows reach it
+    // first in the sort, one 512-thread workgroup per (job, window) tile)
+    {
+        const uint64_t nchunks = std::max<uint64_t>((ent + L.T - 1) / L.T, 1);
+        const uint64_t merge_blocks = 2 * ((nchunks + 255) / 256) + 256 + 512;
+        uint64_t items = (uint64_t)L.ntiles * kSortThreads;
+        items = std::max<uint64_t>(items, merge_blocks * 256);
+        if (L.binned) items = std::max<uint64_t>(items, (keys >> kBinBits) * 256);
+        if (L.g2) items = std::max<uint64_t>(items, 4 * segs);
+        for (uint32_t b : L.tree_blocks) items = std::max<uint64_t>(items, (uint64_t)b * 256);
+        if (items > 0xffffffffULL) throw HipFail{"batch too large for one launch"};
+    }
     return L;
 }
 

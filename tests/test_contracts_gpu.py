@@ -216,3 +216,20 @@ def test_server_ticket_waited_from_two_threads(hip, orc, inputs):
         assert hip.compress(got2) == orc.compress(orc.msm_pippenger(pts[:64], sc[:64], 64, 255))
     finally:
         srv.close()
+
+
+def test_batch_beyond_one_dispatch_is_refused(hip, orc, inputs):
+    """A batch whose launches would pass 2^32 work-items (32,768 jobs of 64
+    points in 256 one-bit windows: 2^23 sort tiles of 512 threads) is refused
+    before anything runs, with the reason; the next batch runs normally."""
+    pts, sc = inputs
+    dp, ds = hip.DeviceBuffer(pts[:64]), hip.DeviceBuffer(sc[:64])
+    try:
+        with pytest.raises(RuntimeError, match="too large for one launch"):
+            hip.msm_batch_device([(dp.ptr, ds.ptr, 64, 255, 1)] * 32768)
+        got = hip.msm_batch_device([(dp.ptr, ds.ptr, 64, 255, 1)] * 3)
+        want = orc.compress(orc.msm_pippenger(pts[:64], sc[:64], 64, 255))
+        assert all(hip.compress(g) == want for g in got)
+    finally:
+        dp.close()
+        ds.close()
