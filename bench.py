@@ -372,7 +372,7 @@ def main():
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the MSM backend runs on a single device "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "(sharding an MSM across GPUs is not built)", file=sys.stderr)
         sys.exit(2)
 
     import blst_hip as bh

@@ -1,7 +1,7 @@
 /* A plain C11 caller of the drop-in boundary, built by tests/test_c_caller.py
  * with gcc -std=c11 -pedantic -Wall -Wextra -Werror against
  * include/blst_hip.h and libblst_hip.so, as a C caller of the reference
- * library would be.
+ * library would be: the same calls, only the library linked differs.
  *
  *   c_caller checks   the scratch sizes of the reference's window heuristic
  *                     (bindings/blst_scalar.h:535-537) and the argument checks
