@@ -63,7 +63,7 @@ def load_traffic():
 
 def cpu_baseline(points, scalars, n, gpu_compressed, samples, jac=None, wbits_case=None, g2_case=None):
     """The oracle (C port of blst_p1s_mult_pippenger) on the host cores, and
-    the checker of sampled batched results; with jac, also the oracle's batch
+    what checks sampled batched results; with jac, also the oracle's batch
     normalisation on a 2^18-point sample of it; with wbits_case, the oracle's
     table MSM on the same table and scalars."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))

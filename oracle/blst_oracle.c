@@ -884,8 +884,8 @@ void or_msm_pippenger_mt(u64 ret[18], const u64 *points, const uint8_t *scalars,
 
 /* Many independent MSMs (or_msm_pippenger each) handed out to threads, the
  * largest first: the reference library is reentrant and a batch of MSMs is
- * independent calls (README.md:39-40), so this is the checker for batched
- * GPU launches.  rets: njobs x 18 limbs. */
+ * independent calls (README.md:39-40), so this is what batched GPU launches
+ * are checked against.  rets: njobs x 18 limbs. */
 typedef struct {
     u64 *rets;
     const u64 *const *points;

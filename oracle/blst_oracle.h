@@ -3,8 +3,8 @@
  *
  * A plain-C CPU restatement of category-blst's G1 multi-scalar
  * multiplication path (blst_p1s_mult_pippenger and what it calls), used by
- * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
- * checker.  Nothing in the product library includes, links or calls this.
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg to check
+ * results.  Nothing in the product library includes, links or calls this.
  *
  * Data layout matches the reference ABI (bindings/blst_types.h:95-152):
  *   Fp element      6 x uint64 little-endian limbs, Montgomery form, R=2^384

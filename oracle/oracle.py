@@ -3,7 +3,7 @@
 The oracle is a CPU restatement of category-blst's G1 MSM path (see
 blst_oracle.c for the file:line each function follows).  Only tests/,
 __graft_entry__.smoke() and bench.py's cpu_baseline leg import this module,
-and only as the checker; the product library never touches it.
+and only to check results; the product library never touches it.
 """
 import ctypes
 import os

@@ -12,9 +12,9 @@ against the oracle (byte-exact 48-byte compressed encodings):
 
 The reference's batched form is the CUDA pipeline of
 blst/cuda/curve/multi_scalar_par_impl.h:1774-2085, tested against
-scalar::msm_mult_pippenger in test/Msm_tests.cpp:709-880; the checker here is
-the oracle's job-parallel MSM (independent MSMs on 16 host threads).  Inputs
-come from the seeded generators the benchmark uses (xorshift64*)."""
+scalar::msm_mult_pippenger in test/Msm_tests.cpp:709-880; here results are
+checked against the oracle's job-parallel MSM (independent MSMs on 16 host
+threads).  Inputs come from the seeded generators the benchmark uses (xorshift64*)."""
 import threading
 import time
 

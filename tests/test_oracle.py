@@ -121,8 +121,8 @@ def test_msm_against_textbook(orc, n, nbits):
     assert orc.compress(orc.msm_pippenger(A, S, n, nbits, threads=3)) == want
 
 
-def test_msm_batch_checker_against_textbook(orc):
-    """The job-parallel checker used for batched GPU launches: every job of a
+def test_msm_batch_against_textbook(orc):
+    """The job-parallel MSM that batched GPU launches are checked against: every job of a
     ragged batch (empty, one point, short and long scalars) equals the
     textbook sum, in any job order and thread count."""
     specs = [(12, 255), (0, 255), (1, 256), (5, 1), (9, 64), (7, 17), (11, 255), (3, 8)]
