@@ -33,8 +33,10 @@ int blst_hip_fp_ops(int op, void *out, const void *a, const void *b, const void 
  * set), 7 whether a is zero modulo p (1 or 0 in the first word), 8, 9, 10
  * the limbs of w 2^8, w 2^4, w where w is the 384-bit number in a's first
  * twelve words, 11 a's value as twelve 32-bit words (then two zero words),
- * 12 4a reduced.  Inputs must respect each operation's stated limb and value
- * bounds (fq.hip.h).  All four inputs are read (pass a for unused ones).
+ * 12 4a reduced, 13 a normalized value of any size reduced below 1.03p,
+ * 14 the canonical value in [0, p) of a normalized value below 2p.  Inputs
+ * must respect each operation's stated limb and value bounds (fq.hip.h).
+ * All four inputs are read (pass a for unused ones).
  * Returns 0 on success, -1 on invalid arguments or a device error. */
 int blst_hip_fq_ops(int op, uint32_t *out, const uint32_t *a, const uint32_t *b, const uint32_t *c,
                     const uint32_t *d, size_t n);
@@ -86,7 +88,9 @@ int blst_hip_booth_digits(int32_t *out, const blst_byte *scalars, size_t npoints
  * reduction stages against the one-lane formulas on the affine points
  * (device memory, 96 bytes each, on the curve): for each point P and its
  * successor Q, 2P + 2Q, 2P + 2P in two frames, 2P - 2P, sums with infinity
- * and doublings.  Returns the number of mismatching coordinates, -1 on error. */
+ * and doublings.  The same cases over the 28-bit limb field, one-lane and
+ * four-lane, must give the same elements.  Returns the number of mismatching
+ * coordinates, -1 on error. */
 int blst_hip_point_ops_check(const void *device_points, size_t npoints);
 
 /* [k_i]G for each scalar (generator multiples, affine, all-zero for k = 0),

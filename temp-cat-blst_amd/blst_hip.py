@@ -517,7 +517,8 @@ def fp_ops(op, a, b=None, c=None, d=None):
 
 
 FQ_OPS = {"mul": 0, "sqr": 1, "mul_sum": 2, "reduce": 3, "sub_lazy": 4, "neg_lazy": 5, "x3": 6, "is_zero": 7,
-          "from_words8": 8, "from_words4": 9, "from_words0": 10, "to_words": 11, "times4": 12}
+          "from_words8": 8, "from_words4": 9, "from_words0": 10, "to_words": 11, "times4": 12,
+          "reduce_wide": 13, "canon": 14}
 
 
 def fq_ops(op, a, b=None, c=None, d=None):

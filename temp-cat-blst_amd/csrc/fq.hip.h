@@ -223,6 +223,38 @@ BLS_FQ void fq_reduce(fq& r, const fq& x) {
     }
 }
 
+// r = x mod p up to one p for a normalized x of any value: q =
+// floor(x_13 / (p_13 + 1)) is below 2^12 and q p <= q (p_13 + 1) 2^364 <= V,
+// while V < (q + 1)(p_13 + 1) 2^364 and q p >= q p_13 2^364 leave
+// V - q p < (p_13 + 1 + q) 2^364 < 1.03 p.  So r is normalized, below 1.03p
+// (below 1.01p for V below 2^390, where q < 631).  q p_i < 2^40 needs 64-bit
+// differences.
+BLS_FQ void fq_reduce_wide(fq& r, const fq& x) {
+    const uint32_t q = x.l[13] / (fq_p(13) + 1);
+    int64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+        const int64_t t = (int64_t)x.l[i] - (int64_t)((uint64_t)q * fq_p(i)) + c;
+        r.l[i] = (uint32_t)t & kFqMask;
+        c = t >> 28;
+    }
+}
+
+// the canonical value in [0, p) of a normalized value below 2p
+BLS_FQ void fq_canon(fq& r, const fq& a) {
+    fq t;
+    int32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < 14; i++) {
+        const int32_t v = (int32_t)a.l[i] - (int32_t)fq_p(i) + c;
+        t.l[i] = (uint32_t)v & kFqMask;
+        c = v >> 28;
+    }
+    const bool ge = c >= 0;
+#pragma unroll
+    for (int i = 0; i < 14; i++) r.l[i] = ge ? t.l[i] : a.l[i];
+}
+
 // X3 = R2 - (PPP + 2Q) (or R2 - 2Q for a doubling) reduced to [0, 2p):
 // R2, PPP, Q normalized and below 1.1p; R2 + 6p - S has limbs below 2^30.4
 // and value below 7.1p

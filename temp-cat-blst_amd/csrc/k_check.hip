@@ -2,8 +2,11 @@
 // of an addition (general, equal points in different frames, opposite
 // points, infinity on either side) and of a doubling must give the same
 // coordinates (as field elements) as the one-lane formulas of ec.hip.h.
+// The same formulas over the 28-bit limb field (fq_point.hip.h), one-lane
+// and four-lane, on the same points loaded from their 32-bit word images,
+// must give the same elements again.
 #include "msm_common.hip.h"
-#include "ec_quad.hip.h"
+#include "fq_point.hip.h"
 #include "launch.h"
 
 namespace bls {
@@ -25,6 +28,31 @@ __device__ __forceinline__ uint32_t fp_differs(const fp& a, const fp& b) {
 #pragma unroll
     for (int k = 0; k < 12; k++) d |= ca.l[k] ^ cb.l[k];
     return d != 0;
+}
+
+// an fq value against the element of a 32-bit value
+__device__ __forceinline__ uint32_t fq_differs(const fq& a, const fp& b) {
+    fq ca, cb;
+    fq_load_words(cb, b.l);
+    fq_canon(cb, cb);
+    fq_canon(ca, a);
+    uint32_t d = 0;
+#pragma unroll
+    for (int k = 0; k < 14; k++) d |= ca.l[k] ^ cb.l[k];
+    return d != 0;
+}
+
+__device__ __forceinline__ void xyzz_to_fq(xyzz_t<fq>& r, const xyzz& p) {
+    fq_load_words(r.x, p.x.l);
+    fq_load_words(r.y, p.y.l);
+    fq_load_words(r.zz, p.zz.l);
+    fq_load_words(r.zzz, p.zzz.l);
+}
+
+__device__ __forceinline__ void xyzz_coord(fq& r, const xyzz_t<fq>& p, uint32_t ln) {
+    fp_sel(r, ln == 0, p.x, p.y);
+    fp_sel(r, ln == 2, p.zz, r);
+    fp_sel(r, ln == 3, p.zzz, r);
 }
 
 // one quad per point index i: P = pts[i], Q = pts[i + 1 mod n]
@@ -67,6 +95,17 @@ k_quad_check(const uint4* __restrict__ pts, uint32_t n, uint32_t* __restrict__ b
         quad_add(r, xl, yl);
         xyzz_coord(sl, s, ln);
         miss += fp_differs(r, sl);
+        xyzz_t<fq> xq, yq, sq;
+        xyzz_to_fq(xq, x);
+        xyzz_to_fq(yq, y);
+        xyzz_add(sq, xq, yq);
+        fq xlq, ylq, rq, slq;
+        xyzz_coord(slq, sq, ln);
+        miss += fq_differs(slq, sl);
+        xyzz_coord(xlq, xq, ln);
+        xyzz_coord(ylq, yq, ln);
+        quad_add(rq, xlq, ylq);
+        miss += fq_differs(rq, sl);
     }
 #pragma unroll 1
     for (int k = 0; k < 3; k++) {
@@ -77,6 +116,15 @@ k_quad_check(const uint4* __restrict__ pts, uint32_t n, uint32_t* __restrict__ b
         quad_dbl(r, xl);
         xyzz_coord(sl, s, ln);
         miss += fp_differs(r, sl);
+        xyzz_t<fq> xq, sq;
+        xyzz_to_fq(xq, x);
+        xyzz_dbl_inl(sq, xq);
+        fq xlq, rq, slq;
+        xyzz_coord(slq, sq, ln);
+        miss += fq_differs(slq, sl);
+        xyzz_coord(xlq, xq, ln);
+        quad_dbl(rq, xlq);
+        miss += fq_differs(rq, sl);
     }
     if (miss) atomicAdd(bad, miss);
 }

@@ -11,7 +11,7 @@
 // so the root's A is the window sum.  The merges run as a tree of four-lane
 // additions: per workgroup 64 nodes in LDS, as many passes as P needs.
 #include "msm_common.hip.h"
-#include "ec_quad.hip.h"
+#include "fq_point.hip.h"
 #include "launch.h"
 
 namespace bls {
@@ -30,7 +30,7 @@ k_reduce_segments(const group_t* __restrict__ groups, const uint32_t* __restrict
     const group_t G = groups[g];
     const uint32_t s = t - G.sb;
     const size_t base = (size_t)G.kb + (size_t)s * G.L;
-    xyzz_t<F> acc, tot, x, y;
+    xyzz_t<tail_t<F>> acc, tot, x, y;
     xyzz_set_inf(acc);
     xyzz_set_inf(tot);
 #pragma unroll 1
@@ -90,11 +90,17 @@ k_reduce_segments_quad(const group_t* __restrict__ groups, const uint32_t* __res
 // than four merges repeats them on four quads (16 lanes) and stores one.
 // tn[g] = (first input node, input nodes, first output node); blk[g] = first
 // block of group g.  last: store the root's A to W[g] instead.
+// four workgroups (four waves per SIMD) for G1, so that a first pass of
+// 1024 workgroups is resident at once
+template <class F> constexpr int kTreeBlocksPerCU = 1;
+template <> constexpr int kTreeBlocksPerCU<fp> = 4;
+
 template <class F>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(256, kTreeBlocksPerCU<F>)
 k_reduce_tree(const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, uint32_t ngroups,
               const xyzz_t<F>* __restrict__ in, xyzz_t<F>* __restrict__ out, xyzz_t<F>* __restrict__ W, uint32_t last) {
-    __shared__ xyzz_t<F> sA[kTreeNodes], sV[kTreeNodes];
+    using E = tail_t<F>;
+    __shared__ E sA[kTreeNodes][4], sV[kTreeNodes][4];
     const uint32_t g = find_owner(blk, ngroups, blockIdx.x);
     const uint32_t j = blockIdx.x - blk[g];
     const uint4 t = tn[g];
@@ -104,44 +110,44 @@ k_reduce_tree(const uint32_t* __restrict__ blk, const uint4* __restrict__ tn, ui
     const bool beta = u >= 32;
     const uint32_t v = u & 31;
     if (u < n) {
-        F a, b;
+        E a, b;
         quad_load(a, &in[2 * (size_t)(first + u)]);
         quad_load(b, &in[2 * (size_t)(first + u) + 1]);
-        reinterpret_cast<F*>(&sA[u])[ln] = a;
-        reinterpret_cast<F*>(&sV[u])[ln] = b;
+        sA[u][ln] = a;
+        sV[u][ln] = b;
     }
     __syncthreads();
     for (uint32_t m = n >> 1; m >= 1; m >>= 1) {
-        F r;
+        E r;
         if (v < max(m, 4u)) {
             const uint32_t mm = v % m;
-            F x, y;
+            E x, y;
             if (!beta) {
-                x = reinterpret_cast<const F*>(&sA[2 * mm + 1])[ln];
-                y = reinterpret_cast<const F*>(&sV[2 * mm + 1])[ln];
+                x = sA[2 * mm + 1][ln];
+                y = sV[2 * mm + 1][ln];
                 quad_add(x, x, y);
-                y = reinterpret_cast<const F*>(&sA[2 * mm])[ln];
+                y = sA[2 * mm][ln];
                 quad_add(r, y, x);
             } else {
-                x = reinterpret_cast<const F*>(&sV[2 * mm])[ln];
-                y = reinterpret_cast<const F*>(&sV[2 * mm + 1])[ln];
+                x = sV[2 * mm][ln];
+                y = sV[2 * mm + 1][ln];
                 quad_add(x, x, y);
                 quad_dbl(r, x);
             }
         }
         __syncthreads();
-        if (v < m) reinterpret_cast<F*>(beta ? &sV[v] : &sA[v])[ln] = r;
+        if (v < m) (beta ? sV[v] : sA[v])[ln] = r;
         __syncthreads();
     }
     if (last) {
         if (u == 0) {
-            F c;
-            fp_canon(c, reinterpret_cast<const F*>(&sA[0])[ln]);
+            E c;
+            fp_canon(c, sA[0][ln]);
             quad_store(&W[g], c);
         }
     } else if (u == 0) {
-        quad_store(&out[2 * (size_t)(t.z + j)], reinterpret_cast<const F*>(&sA[0])[ln]);
-        quad_store(&out[2 * (size_t)(t.z + j) + 1], reinterpret_cast<const F*>(&sV[0])[ln]);
+        quad_store(&out[2 * (size_t)(t.z + j)], sA[0][ln]);
+        quad_store(&out[2 * (size_t)(t.z + j) + 1], sV[0][ln]);
     }
 }
 

@@ -80,7 +80,7 @@ k_fp_ops(uint32_t op, uint4* __restrict__ out, const uint4* __restrict__ A, cons
 // operations of the 28-bit limb field, selected by op; rows of 14 words
 enum : uint32_t {
     kFqMul = 0, kFqSqr, kFqMulSum, kFqReduce, kFqSubLazy, kFqNegLazy, kFqX3, kFqIsZero, kFqFromWords8, kFqFromWords4,
-    kFqFromWords0, kFqToWords, kFqTimes4, kFqCount
+    kFqFromWords0, kFqToWords, kFqTimes4, kFqReduceWide, kFqCanon, kFqCount
 };
 
 __global__ void __launch_bounds__(256)
@@ -110,7 +110,9 @@ k_fq_ops(uint32_t op, uint32_t* __restrict__ out, const uint32_t* __restrict__ A
     case kFqFromWords4: fq_from_words<4>(r, a.l); break;
     case kFqFromWords0: fq_from_words<0>(r, a.l); break;
     case kFqToWords: fq_to_words(r.l, a); break;
-    default: fq_times4(r, a); break;
+    case kFqTimes4: fq_times4(r, a); break;
+    case kFqReduceWide: fq_reduce_wide(r, a); break;
+    default: fq_canon(r, a); break;
     }
 #pragma unroll
     for (int k = 0; k < 14; k++) out[14 * i + k] = r.l[k];

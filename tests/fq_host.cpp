@@ -32,7 +32,9 @@ void fqh_ops(int op, uint32_t* out, const uint32_t* A, const uint32_t* B, const 
         case 9: fq_from_words<4>(r, a.l); break;
         case 10: fq_from_words<0>(r, a.l); break;
         case 11: fq_to_words(r.l, a); break;
-        default: fq_times4(r, a); break;
+        case 12: fq_times4(r, a); break;
+        case 13: fq_reduce_wide(r, a); break;
+        default: fq_canon(r, a); break;
         }
         memcpy(out + 14 * i, r.l, 56);
     }

@@ -22,7 +22,8 @@ W, N = 28, 14
 MASK = (1 << W) - 1
 RINV = pow(1 << 392, -1, P)
 OPS = {"mul": 0, "sqr": 1, "mul_sum": 2, "reduce": 3, "sub_lazy": 4, "neg_lazy": 5, "x3": 6, "is_zero": 7,
-       "from_words8": 8, "from_words4": 9, "from_words0": 10, "to_words": 11, "times4": 12}
+       "from_words8": 8, "from_words4": 9, "from_words0": 10, "to_words": 11, "times4": 12,
+       "reduce_wide": 13, "canon": 14}
 
 
 def limbs(x):
@@ -99,6 +100,13 @@ def op_cases(rnd, count):
     def chk_zero(a, b, c, d, r):
         return r[0] == (1 if val(a) % P == 0 else 0)
 
+    def chk_wide(a, b, c, d, r):
+        bound = P * 101 // 100 if val(a) < 1 << 390 else P * 103 // 100
+        return val(r) % P == val(a) % P and normal(r, bound)
+
+    def chk_canon(a, b, c, d, r):
+        return [int(x) for x in r] == limbs(val(a) % P)
+
     for k in range(count):
         edge = k % 4 == 0
         # products: normalized, lazy, and the loaded x (below 2^390) against a value below 2p
@@ -124,6 +132,14 @@ def op_cases(rnd, count):
         cases.append(("x3", r2, ppp, q, dbl, chk_x3))
         z = [0, P, 2 * P - 1, 1, rnd.randrange(2 * P)][k % 5]
         cases.append(("is_zero", limbs(z), limbs(z), limbs(z), limbs(z), chk_zero))
+        # any normalized value: the largest, near multiples of p, the exact
+        # images of 32-bit words (w 2^8, w below 2p) and random ones
+        m = rnd.randrange(1, (1 << 392) // P)
+        v = [(1 << 392) - 1, m * P - 1, m * P, m * P + 1, rnd.randrange(2 * P) << 8,
+             rnd.randrange(1 << 392)][k % 6]
+        cases.append(("reduce_wide", limbs(v), limbs(v), limbs(v), limbs(v), chk_wide))
+        v = [0, 1, P - 1, P, P + 1, 2 * P - 1, rnd.randrange(2 * P)][k % 7]
+        cases.append(("canon", limbs(v), limbs(v), limbs(v), limbs(v), chk_canon))
     return cases
 
 
