@@ -1,6 +1,8 @@
-// The latency-bound G1 stages (partial merges, segment sums, reduction
-// trees) compute in the 28-bit limb field of fq.hip.h: a product there is
-// 493 VALU instructions against 624, a square 414 against 624.
+// The G1 bucket reduction (segment sums and trees, k_reduce.hip) computes in
+// the 28-bit limb field of fq.hip.h: a product there is 493 VALU
+// instructions against 624, a square 414 against 624.  The partial merges
+// stay in the 32-bit form: most of them add two stored pieces once, and the
+// conversions below would cost about what the field saves.
 //
 // The XYZZ formulas of ec.hip.h and ec_quad.hip.h are templates over the
 // coordinate type and call fp_mul, fp_sub, ... by overload.  The overloads

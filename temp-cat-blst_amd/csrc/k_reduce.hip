@@ -10,6 +10,8 @@
 //     A = A_left + (A_right + V_right),   V = 2 (V_left + V_right),
 // so the root's A is the window sum.  The merges run as a tree of four-lane
 // additions: per workgroup 64 nodes in LDS, as many passes as P needs.
+// G1 computes in the 28-bit limb field (tail_t<fp> = fq, fq_point.hip.h),
+// loading and storing the 32-bit layout; G2 in Fp2.
 #include "msm_common.hip.h"
 #include "fq_point.hip.h"
 #include "launch.h"
