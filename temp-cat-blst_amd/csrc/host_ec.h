@@ -10,6 +10,10 @@
 #pragma once
 #include <stdint.h>
 #include <string.h>
+#if defined(__x86_64__) && !defined(__HIP_DEVICE_COMPILE__)
+#include <cpuid.h>
+#define HOSTEC_X86_ADX 1
+#endif
 
 namespace hostec {
 
@@ -58,7 +62,7 @@ inline void mac(acc192& s, uint64_t x, uint64_t y) {
 // the low word of column k < 6 fixes m_k = lo (-1/p) mod 2^64 (which clears
 // it), and columns 6..11 are the result words, less p if needed.  Inputs
 // below 2p give a canonical result (a b + m p < 4p^2 + 2^384 p < 2^384 2p).
-inline void mul(fp6& r, const fp6& a, const fp6& b) {
+inline void mul_portable(fp6& r, const fp6& a, const fp6& b) {
     uint64_t m[6], t[7];
     acc192 s = {0, 0, 0};
     for (int k = 0; k < 6; k++) {
@@ -82,6 +86,71 @@ inline void mul(fp6& r, const fp6& a, const fp6& b) {
     t[5] = s.lo;
     t[6] = s.hi;
     sub_p_if_ge(r, t);
+}
+
+#ifdef HOSTEC_X86_ADX
+// The same product on x86-64 with BMI2 and ADX, operand scanning (CIOS) in
+// six rounds over a window of seven words L0..L6.  Round i adds a b_i with
+// mulx, the low words along the CF chain (adcx) and the high words along the
+// OF chain (adox), so the two carry chains run side by side; then it adds
+// m p with m = L0 (-1/p) mod 2^64 the same way, which clears L0.  The next
+// round's window is L1..L6 and the cleared word, so the rounds name the seven
+// registers (%0..%6) rotated by one.  p's top word is below 2^62, so with
+// inputs below 2p the running value stays below 2^383 and the window never
+// needs an eighth word; the result is below a b / 2^384 + p < 1.4p and one
+// subtraction of p makes it canonical, as above.
+// Operands: %7, %8 the product halves, %9 a, %10 b, %11 p, %12 -1/p mod 2^64.
+#define HOSTEC_MAC(OFF, SRC, LO, HI)       \
+    "mulxq " #OFF "(%" #SRC "), %7, %8\n\t" \
+    "adcxq %7, %" #LO "\n\t"                \
+    "adoxq %8, %" #HI "\n\t"
+#define HOSTEC_ROUND(I, L0, L1, L2, L3, L4, L5, L6)                                            \
+    "movq " #I "*8(%10), %%rdx\n\t"                                                            \
+    "xorq %" #L6 ", %" #L6 "\n\t"                                                              \
+    HOSTEC_MAC(0, 9, L0, L1) HOSTEC_MAC(8, 9, L1, L2) HOSTEC_MAC(16, 9, L2, L3)                  \
+    HOSTEC_MAC(24, 9, L3, L4) HOSTEC_MAC(32, 9, L4, L5) HOSTEC_MAC(40, 9, L5, L6)                \
+    "adcq $0, %" #L6 "\n\t"                                                                    \
+    "movq %" #L0 ", %%rdx\n\t"                                                                 \
+    "imulq %12, %%rdx\n\t"                                                                     \
+    "xorq %7, %7\n\t"                                                                          \
+    HOSTEC_MAC(0, 11, L0, L1) HOSTEC_MAC(8, 11, L1, L2) HOSTEC_MAC(16, 11, L2, L3)               \
+    HOSTEC_MAC(24, 11, L3, L4) HOSTEC_MAC(32, 11, L4, L5) HOSTEC_MAC(40, 11, L5, L6)             \
+    "adcq $0, %" #L6 "\n\t"
+
+inline void mul_adx(fp6& r, const fp6& a, const fp6& b) {
+    uint64_t w0 = 0, w1 = 0, w2 = 0, w3 = 0, w4 = 0, w5 = 0, w6, lo, hi;
+    asm(HOSTEC_ROUND(0, 0, 1, 2, 3, 4, 5, 6) HOSTEC_ROUND(1, 1, 2, 3, 4, 5, 6, 0)
+        HOSTEC_ROUND(2, 2, 3, 4, 5, 6, 0, 1) HOSTEC_ROUND(3, 3, 4, 5, 6, 0, 1, 2)
+        HOSTEC_ROUND(4, 4, 5, 6, 0, 1, 2, 3) HOSTEC_ROUND(5, 5, 6, 0, 1, 2, 3, 4)
+        : "+&r"(w0), "+&r"(w1), "+&r"(w2), "+&r"(w3), "+&r"(w4), "+&r"(w5), "=&r"(w6), "=&r"(lo), "=&r"(hi)
+        : "r"(a.l), "r"(b.l), "r"(kP), "m"(kN0)
+        : "rdx", "cc", "memory");
+    const uint64_t t[7] = {w6, w0, w1, w2, w3, w4, 0};
+    sub_p_if_ge(r, t);
+}
+#undef HOSTEC_ROUND
+#undef HOSTEC_MAC
+
+// whether this CPU has BMI2 (mulx) and ADX (adcx, adox): CPUID leaf 7,
+// EBX bits 8 and 19
+inline bool have_adx() {
+    static const bool v = [] {
+        unsigned a, b, c, d;
+        return __get_cpuid_count(7, 0, &a, &b, &c, &d) && ((b >> 8) & 1) && ((b >> 19) & 1);
+    }();
+    return v;
+}
+#endif
+
+// r = a b 2^-384 mod p, canonical for inputs below 2p
+inline void mul(fp6& r, const fp6& a, const fp6& b) {
+#ifdef HOSTEC_X86_ADX
+    if (have_adx()) {
+        mul_adx(r, a, b);
+        return;
+    }
+#endif
+    mul_portable(r, a, b);
 }
 
 inline void add(fp6& r, const fp6& a, const fp6& b) {

@@ -1,7 +1,8 @@
 // Host code of the product checked against the oracle, for the sanitizer
 // builds of tests/test_host_sanitizers.py:
 //   host_ec.h    G1 window fold, XYZZ -> Jacobian, compression, batch
-//                normalisation and inversion (the host tail of every G1 MSM)
+//                normalisation and inversion (the host tail of every G1 MSM),
+//                and its Montgomery product in both forms
 //   host_ec2.h   the G2 window fold and XYZZ -> Jacobian
 //   fq.hip.h     the G1 accumulation's 28-bit limb field and its bucket sum
 //   fp_inv.hip.h the divstep inversion
@@ -125,6 +126,54 @@ void check_g1_fold() {
                 expect(std::memcmp(got, want, 48) == 0, "g1 fold", (int)n, (int)(nbits * 100 + c));
             }
         }
+    }
+}
+
+// the host Montgomery product: the portable form against the oracle's
+// (inputs below p) and, where the CPU has BMI2 and ADX, the mulx/adcx/adox
+// form against the portable one on inputs below 2p, edge values included
+void check_host_products() {
+    const uint64_t twop_m1[6] = {0x73fdffffffff5555ULL, 0x3d57fffd62a7ffffULL, 0xce61a541ed61ec48ULL,
+                                 0xc8ee9709e70a257eULL, 0x96374f6c869759aeULL, 0x340223d472ffcd34ULL};
+    auto edge = [&](uint64_t r[6], int k) {
+        std::memset(r, 0, 48);
+        if (k == 1) r[0] = 1;
+        if (k == 2) { std::memcpy(r, kP, 48); r[0] -= 1; }
+        if (k == 3) std::memcpy(r, kP, 48);
+        if (k == 4) std::memcpy(r, twop_m1, 48);
+    };
+    for (int t = 0; t < 20000; t++) {
+        uint64_t a[6], b[6], want[6];
+        rand_fp(a);
+        rand_fp(b);
+        if (t < 25) {
+            edge(a, t % 5);
+            edge(b, t / 5);
+        }
+        hostec::fp6 x, y, r1, r2;
+        std::memcpy(x.l, a, 48);
+        std::memcpy(y.l, b, 48);
+        hostec::mul_portable(r1, x, y);
+        const bool below_p = t >= 25 || (t % 5 < 3 && t / 5 < 3);
+        if (below_p) {
+            or_fp_mul(want, a, b);
+            expect(std::memcmp(r1.l, want, 48) == 0, "host product", t, 0);
+        }
+#ifdef HOSTEC_X86_ADX
+        if (hostec::have_adx()) {
+            if (t >= 25 && (t & 1)) {   // lift into [p, 2p) where that stays below 2p
+                uint64_t c = 0;
+                for (int i = 0; i < 6; i++) {
+                    const unsigned __int128 v = (unsigned __int128)x.l[i] + kP[i] + c;
+                    x.l[i] = (uint64_t)v;
+                    c = (uint64_t)(v >> 64);
+                }
+                hostec::mul_portable(r1, x, y);
+            }
+            hostec::mul_adx(r2, x, y);
+            expect(std::memcmp(r1.l, r2.l, 48) == 0, "host product, adx", t, 0);
+        }
+#endif
     }
 }
 
@@ -386,6 +435,7 @@ int main(int argc, char** argv) {
     if (tsan) {
         check_threads();
     } else {
+        check_host_products();
         check_g1_fold();
         check_g1_affine_and_inverse();
         check_g2_fold();
