@@ -204,8 +204,9 @@ int blst_hip_msm_device(blst_p1 *ret, const void *device_points, const void *dev
 /* No counterpart in the reference.  blst_hip_msm_device for callers that
  * cannot allocate (code under HIP graph capture, tight serving loops,
  * harnesses that forbid hipMalloc): the same MSM and result for at most
- * 65536 points (nbits <= 256, tuned window), computed entirely in a fixed
- * 96 MiB device array that each GPU the library runs on holds for the life of
+ * 65536 points (nbits <= 256, tuned window; a wider accumulation tile where
+ * the tuned one would not fit), computed entirely in a fixed 96 MiB device
+ * array that each GPU the library runs on holds for the life of
  * the process.  The call allocates no device or page-locked memory; the
  * first call on a device takes the array's device address (and sets up the
  * library's device context if no other entry point has).  Every call on a

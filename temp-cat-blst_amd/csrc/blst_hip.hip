@@ -1657,7 +1657,7 @@ size_t comb_table_bytes(size_t npoints, size_t wbits, size_t nbits) {
 // address of each GPU's copy comes from hipGetSymbolAddress, once per device
 // with that device current, and only that address is handed to kernels.
 constexpr size_t kNoAllocMaxPoints = (size_t)1 << 16;
-constexpr size_t kNoAllocScratch = (size_t)96 << 20;   // the committed tuning table needs at most 90.0 MiB (65535 points, 256 bits)
+constexpr size_t kNoAllocScratch = (size_t)96 << 20;   // a size class whose tuned tile needs more takes a wider tile (noalloc_msm)
 // host staging of the layout tables, then the window sums
 constexpr size_t kNoAllocHostBytes = (size_t)1 << 20;
 
@@ -1699,12 +1699,19 @@ NoAllocDevice& noalloc_device() {
 
 // The MSM of blst_hip_msm_device (tuned window, one pipeline: every size up
 // to kNoAllocMaxPoints runs as one window group) in this device's scratch.
+// Where the tuned tile's partial-sum slots (two per tile) would not fit the
+// fixed region, the tile doubles until they do: the tile sets only how the
+// sorted entries are cut among lanes, never the result.
 void noalloc_msm(const void* device_points, const void* device_scalars, size_t npoints, size_t nbits,
                  blst_p1* ret) {
     NoAllocDevice& d = noalloc_device();
     std::lock_guard<std::mutex> lk(d.mu);
-    const JobSpec js = make_spec(device_points, device_scalars, npoints, nbits, 0, false);
-    const Layout L = make_layout(std::vector<JobSpec>(1, js));
+    JobSpec js = make_spec(device_points, device_scalars, npoints, nbits, 0, false);
+    Layout L = make_layout(std::vector<JobSpec>(1, js));
+    while (carved_bytes(L) > kNoAllocScratch && js.T < 4096) {
+        js.T *= 2;
+        L = make_layout(std::vector<JobSpec>(1, js));
+    }
     carve_workspace(d.ws, L, d.base, kNoAllocScratch);
     const size_t nsums = L.groups.size();
     if (nsums * sizeof(hostec::xyzz6) > kNoAllocHostBytes / 2) throw HipFail{"too many windows"};
