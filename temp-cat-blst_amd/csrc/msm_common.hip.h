@@ -12,14 +12,17 @@
 //               histogram with per-entry ranks instead: digits, scatter)
 //   accumulate  fixed-length chunks of the sorted list per lane, XYZZ mixed
 //               additions of gathered affine points (the bucket fill of
-//               points_tile_pippenger, multi_scalar.cpp:806-865); buckets that
-//               cross chunk boundaries leave partial sums in two slots/chunk
+//               points_tile_pippenger, multi_scalar.cpp:806-865), G1 in the
+//               28-bit limb field; buckets that cross chunk boundaries leave
+//               partial sums in two slots/chunk
 //   partials    buckets crossing one chunk boundary: one lane adds the two
-//               partials; longer spans: one workgroup per bucket, four-lane
-//               additions in strides, then a tree in LDS
+//               partials; three or four pieces one lane, up to kWaveSpan one
+//               wave, longer spans one workgroup per bucket (four-lane
+//               additions in strides, then a tree), all in one launch
 //   reduce      sum_b b * S_b per window (point_integrate_buckets,
 //               multi_scalar.cpp:730-751): running sums over segments of L
-//               buckets, then a tree of four-lane merges per window
+//               buckets, then a tree of four-lane merges per window, G1 in
+//               the 28-bit limb field
 //
 // Keys are (window * B + bucket) with B = 2^(c-1) buckets per window, so all
 // windows go through one sort and one accumulate launch.
