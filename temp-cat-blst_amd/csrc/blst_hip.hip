@@ -12,6 +12,7 @@
 
 #include <dlfcn.h>
 #include <condition_variable>
+#include <exception>
 
 #include <algorithm>
 #include <cstdio>
@@ -126,6 +127,7 @@ struct Stager {
     static constexpr size_t kMin = (size_t)8 << 20;   // smaller copies go direct
     uint8_t* ring[kRing] = {};
     hipEvent_t ev[kRing] = {};
+    uint64_t next_slot = 0;   // round robin over the ring across calls
     std::unique_ptr<CopyPool> pool;
     ~Stager() {
         for (int k = 0; k < kRing; k++) {
@@ -158,8 +160,10 @@ struct Stager {
         init();
         const uint8_t* s = static_cast<const uint8_t*>(src);
         uint8_t* d = static_cast<uint8_t*>(dst);
-        for (size_t off = 0, k = 0; off < bytes; off += kChunk, k++) {
-            const int slot = (int)(k % kRing);
+        // the slots continue from the previous call's, so the first chunk of a
+        // call does not wait for the DMA of the previous call's first chunk
+        for (size_t off = 0; off < bytes; off += kChunk) {
+            const int slot = (int)(next_slot++ % kRing);
             HIPCK(hipEventSynchronize(ev[slot]));   // the slot's previous DMA is done
             const size_t len = std::min(kChunk, bytes - off);
             pool->copy(ring[slot], s + off, len);
@@ -986,6 +990,60 @@ void device_msm(Ctx& cx, const JobSpec& js, blst_p1* ret) {
     fold_groups(cx, G, nwin, js.c, outs, nsums, ret);
 }
 
+// The staging of a split MSM's point ranges on a thread of its own:
+// stage(r, lo, m) (host copies into the page-locked ring and their DMAs, on
+// cx.stream) for r = 0 .. K-1 in turn, each followed by cx.range_ready[r]
+// recorded on cx.stream.  The caller's thread queues the kernels of range r
+// once wait(r) returns, so the copies of the later ranges run while the
+// earlier ranges' kernels are being queued (on one thread the two took turns:
+// a range's copies began only after the previous range's launches).
+// wait(r) and join() rethrow what stopped the staging; the destructor only
+// joins (a caller unwinding from its own error waits for the copies queued).
+class RangeStager {
+public:
+    template <class Stage>
+    RangeStager(Ctx& cx, const std::vector<size_t>& lo, Stage& stage) : K_((uint32_t)lo.size() - 1) {
+        thread_ = std::thread([this, &cx, &lo, &stage] {
+            uint32_t r = 0;
+            try {
+                HIPCK(hipSetDevice(cx.device));
+                for (; r < K_; r++) {
+                    stage(r, lo[r], lo[r + 1] - lo[r]);
+                    HIPCK(hipEventRecord(cx.range_ready[r], cx.stream));
+                    std::lock_guard<std::mutex> lk(mu_);
+                    recorded_ = r + 1;
+                    cv_.notify_all();
+                }
+            } catch (...) {
+                std::lock_guard<std::mutex> lk(mu_);
+                error_ = std::current_exception();
+                recorded_ = K_;
+                cv_.notify_all();
+            }
+        });
+    }
+    ~RangeStager() {
+        if (thread_.joinable()) thread_.join();
+    }
+    void wait(uint32_t r) {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return recorded_ > r; });
+        if (error_) std::rethrow_exception(error_);
+    }
+    void join() {
+        if (thread_.joinable()) thread_.join();
+        if (error_) std::rethrow_exception(error_);
+    }
+
+private:
+    const uint32_t K_;
+    std::thread thread_;
+    std::mutex mu_;
+    std::condition_variable cv_;
+    uint32_t recorded_ = 0;
+    std::exception_ptr error_;
+};
+
 // An MSM in K point ranges whose buckets are shared: each window-group
 // pipeline sorts and accumulates range r (the accumulation continues the
 // bucket sums of the earlier ranges, their entry counts summed alongside)
@@ -1015,6 +1073,8 @@ void ranged_msm(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stage&&
     std::vector<size_t> lo(K + 1);
     for (uint32_t k = 0; k < K; k++) lo[k] = (n * k / K) & ~(size_t)63;
     lo[K] = n;
+    // the first range's copies start while the layouts are built
+    RangeStager staging(cx, lo, stage);
     // layouts of every (group, range), buffers and tables first (allocation
     // may synchronise); each range's tables in a region of their own
     std::vector<Layout> Ls(G * K);
@@ -1045,8 +1105,7 @@ void ranged_msm(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stage&&
         outs[g] = static_cast<hostec::xyzz6*>(w.host_buf(nsums[g] * sizeof(hostec::xyzz6)));
     }
     for (uint32_t r = 0; r < K; r++) {
-        stage(r, lo[r], lo[r + 1] - lo[r]);
-        HIPCK(hipEventRecord(cx.range_ready[r], cx.stream));
+        staging.wait(r);
         for (uint32_t g = 0; g < G; g++) {
             Workspace& w = *cx.pipe[g];
             HIPCK(hipStreamWaitEvent(w.stream, cx.range_ready[r], 0));
@@ -1057,6 +1116,7 @@ void ranged_msm(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stage&&
             enqueue_batch(w, Ls[g * K + r], Ds[g * K + r], true, outs[g], g == 0 && r + 1 == K, role);
         }
     }
+    staging.join();
     fold_groups(cx, G, nwin, full.c, outs, nsums, ret);
 }
 
@@ -1112,9 +1172,8 @@ const uint8_t* resolve(const T* const items[], size_t n, size_t stride, std::vec
     bool contiguous = true, ended = false;
     const uint8_t* prev = head;
     for (size_t i = 1; i < n && contiguous; i++) {
-        const uint8_t* cur;
-        if (!ended && items[i]) cur = reinterpret_cast<const uint8_t*>(items[i]);
-        else { ended = true; cur = prev + stride; }
+        if (!items[i]) break;   // the rest continues contiguously: {base, NULL} is one check
+        const uint8_t* cur = reinterpret_cast<const uint8_t*>(items[i]);
         if (cur != prev + stride) contiguous = false;
         prev = cur;
     }
@@ -1197,6 +1256,7 @@ void ranged_msm_g2(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stag
     std::vector<size_t> lo(K + 1);
     for (uint32_t k = 0; k < K; k++) lo[k] = (n * k / K) & ~(size_t)63;
     lo[K] = n;
+    RangeStager staging(cx, lo, stage);   // copies start while the layouts are built
     std::vector<Layout> Ls(K);
     std::vector<DevLayout> Ds(K);
     size_t stride = 0;
@@ -1218,8 +1278,7 @@ void ranged_msm_g2(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stag
     const uint32_t nsum = (uint32_t)Ls[0].groups.size();
     auto* out = static_cast<hostec2::xyzz12*>(w.host_buf(nsum * sizeof(hostec2::xyzz12)));
     for (uint32_t r = 0; r < K; r++) {
-        stage(r, lo[r], lo[r + 1] - lo[r]);
-        HIPCK(hipEventRecord(cx.range_ready[r], cx.stream));
+        staging.wait(r);
         HIPCK(hipStreamWaitEvent(w.stream, cx.range_ready[r], 0));
         RangeRole role;
         role.prev = r ? w.cnt_sum.as<const uint32_t>() : nullptr;
@@ -1227,6 +1286,7 @@ void ranged_msm_g2(Ctx& cx, const JobSpec& full, size_t nbytes, uint32_t K, Stag
         role.reduce = r + 1 == K;
         enqueue_batch(w, Ls[r], Ds[r], true, out, r + 1 == K, role);
     }
+    staging.join();
     HIPCK(hipStreamSynchronize(w.stream));
     record_phases(w);
     hostec2::xyzz12 res;
