@@ -78,6 +78,13 @@ int blst_hip_gather_probe(const void *device_points, size_t stride_bytes, const 
  * blst_hip_last_error(), with later calls unaffected once the cap is lifted. */
 void blst_hip_set_alloc_cap(size_t bytes);
 
+/* Makes the staging thread of every later host-buffer MSM split into point
+ * ranges (2^19 points or more) fail before it stages point range `range`
+ * (-1: never), while the caller's thread waits for that range: the call must
+ * come back as infinity with a message in blst_hip_last_error(), and the next
+ * call, with the hook off, must be correct. */
+void blst_hip_set_stage_failure(int range);
+
 /* Test hook: the signed Booth digits the device computes for window size
  * `window` starting at bit `bit0` (one int32 per scalar, |d| <= 2^(window-1)),
  * written to out[npoints] (host).  Returns 0 on success. */

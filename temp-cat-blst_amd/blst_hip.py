@@ -87,6 +87,7 @@ TESTKIT_SIGNATURES = {
     "blst_hip_fp_ops": ([ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _sz], ctypes.c_int),
     "blst_hip_fq_ops": ([ctypes.c_int, _vp, _vp, _vp, _vp, _vp, _sz], ctypes.c_int),
     "blst_hip_set_alloc_cap": ([_sz], None),
+    "blst_hip_set_stage_failure": ([ctypes.c_int], None),
     "blst_hip_mad_issue_rate": ([_vp], ctypes.c_int),
     "blst_hip_batch_affine_probe": ([_vp, _vp, _sz, ctypes.c_int, _vp, _vp, _vp], ctypes.c_int),
     "blst_hip_gather_probe": ([_vp, _sz, _vp, _sz, _vp, _vp], ctypes.c_int),
@@ -584,6 +585,12 @@ def batch_affine_probe(d_points_ptr, npairs, pairs_per_lane, d_index_ptr=None, r
 def set_alloc_cap(nbytes):
     """Device allocations of the product above nbytes fail (0: no cap)."""
     testkit().blst_hip_set_alloc_cap(nbytes)
+
+
+def set_stage_failure(range_index):
+    """The staging thread of split host-buffer MSMs fails before this point
+    range (-1: never)."""
+    testkit().blst_hip_set_stage_failure(range_index)
 
 
 def booth_digits(scalars, nbits, bit0, window):

@@ -35,6 +35,7 @@ namespace host {
 thread_local std::string g_err;
 thread_local double g_phase[7];
 std::atomic<size_t> g_alloc_cap{0};
+std::atomic<int> g_stage_fail_range{-1};
 
 // the message of the exception being handled (called inside a catch block):
 // no exception crosses the C ABI, every one becomes blst_hip_last_error()
@@ -1008,6 +1009,8 @@ public:
             try {
                 HIPCK(hipSetDevice(cx.device));
                 for (; r < K_; r++) {
+                    if ((int)r == g_stage_fail_range.load(std::memory_order_relaxed))
+                        throw HipFail{"staging of point range " + std::to_string(r) + " refused (test hook)"};
                     stage(r, lo[r], lo[r + 1] - lo[r]);
                     HIPCK(hipEventRecord(cx.range_ready[r], cx.stream));
                     std::lock_guard<std::mutex> lk(mu_);

@@ -30,6 +30,9 @@ struct HipFail {
 // memory would (0: no cap).  Only the test kit sets it, to drive the
 // fail-closed paths of the entry points from the host.
 extern std::atomic<size_t> g_alloc_cap;
+// The staging thread of a split host-buffer MSM fails before this point
+// range (-1: never); only the test kit sets it.
+extern std::atomic<int> g_stage_fail_range;
 
 inline void device_malloc(void** p, size_t bytes) {
     const size_t cap = g_alloc_cap.load(std::memory_order_relaxed);

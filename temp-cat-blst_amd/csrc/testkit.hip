@@ -230,6 +230,8 @@ extern "C" {
 
 void blst_hip_set_alloc_cap(size_t bytes) { g_alloc_cap.store(bytes); }
 
+void blst_hip_set_stage_failure(int range) { g_stage_fail_range.store(range); }
+
 int blst_hip_fp_ops(int op, void* out, const void* a, const void* b, const void* c, const void* d, size_t n) {
     if (n == 0) { g_err.clear(); return 0; }
     if (op < 0 || op >= (int)kOpCount || !out || !a || !b || n >= (1u << 26) ||

@@ -164,6 +164,24 @@ def test_allocation_failure_in_the_drop_in_entry(hip, orc, inputs, tmp_path):
     assert bytes.fromhex(second) == orc.compress(orc.msm_pippenger(pts[:n], sc[:n], n, 255))
 
 
+def test_staging_failure_in_a_split_host_msm(hip, orc, inputs):
+    """A host-buffer MSM of 2^19 + 77 points runs as point ranges staged by a
+    thread of the library's own while the caller's thread queues each range's
+    kernels.  With that thread made to fail before its second range, the call
+    writes infinity and sets blst_hip_last_error(); the next call is correct."""
+    pts, sc = inputs
+    n = pts.shape[0]
+    hip.set_stage_failure(1)
+    try:
+        print("drop-in MSM of %d points with the staging of range 1 refused" % n, flush=True)
+        with pytest.raises(RuntimeError, match="staging of point range 1 refused"):
+            hip.p1s_mult_pippenger(pts, sc, 255)
+    finally:
+        hip.set_stage_failure(-1)
+    got = hip.compress(hip.p1s_mult_pippenger(pts, sc, 255))
+    assert got == orc.compress(orc.msm_pippenger(pts, sc, n, 255, threads=16))
+
+
 def test_server_rejects_what_it_cannot_run(hip, inputs):
     pts, sc = inputs
     srv = hip.Server(hbm_budget=1 << 20)
