@@ -808,9 +808,6 @@ void enqueue_batch(Workspace& cx, const Layout& L, const DevLayout& D, bool host
                    bool timing, const RangeRole& range) {
     const uint32_t nkeys = L.nkeys;
     const uint32_t nent = L.total_entries;
-    const uint32_t T = L.T;
-    const uint32_t nchunks = std::max<uint32_t>(div_up(nent, T), 1);
-    const uint32_t nslots = 2 * nchunks;
     const uint32_t ngroups = (uint32_t)L.groups.size();
     const uint32_t njobs = (uint32_t)L.jobs.size();
     hipStream_t st = cx.stream;
