@@ -295,6 +295,9 @@ def serving(rate=None, duration=2.0, load=0.9, check_every=97, burst_rate=2800.0
         "plain_latency_ms": {"p50": _pct(lat_plain, 50), "p99": _pct(lat_plain, 99), "max": max(lat_plain)},
         "speedup": plain_s / served_s,
         "capacity_speedup": capacity / (pts_total / plain_s),
+        "speedup_basis": "speedup: points/s served at the offered load over the plain path's points/s on the same "
+                         "trace (the plain path cannot keep up, so its rate is its maximum); capacity_speedup: the "
+                         "served form's maximum (the burst above) over the same plain rate",
         "server": stats,
         "served_equals_plain_sampled": same,
     }
