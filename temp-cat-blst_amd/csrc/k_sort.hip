@@ -159,7 +159,8 @@ k_place(const job_t* __restrict__ jobs, const sgroup_t* __restrict__ sg, const u
 //                bin's run written contiguously into tmp
 //   k_bin_fine   per bin: its buckets counted in LDS (the per-bucket count
 //                and offset tables of the later stages) and its entries
-//                placed in bucket order inside the bin's region
+//                placed in bucket order inside the bin's region; a bin of up
+//                to 4096 entries is read from HBM once, into LDS
 // Keys of a binned layout start every window at a multiple of 2^kBinBits,
 // so the global bin of a key is key >> kBinBits.
 
@@ -271,7 +272,11 @@ k_bin_fine(const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, ui
            const uint32_t* __restrict__ pos, const uint2* __restrict__ tmp, uint2* __restrict__ sorted,
            uint32_t* __restrict__ cnt, uint32_t* __restrict__ off, uint32_t nkeys, uint32_t ncounts) {
     constexpr uint32_t F = 1u << kBinBits;
+    // a bin of up to kFineStage entries (about twice the average at 2^20) is
+    // read from HBM once, into LDS, and counted and placed from there
+    constexpr uint32_t kFineStage = 4096;
     __shared__ uint32_t fcount[F], cursor[F];
+    __shared__ uint2 stage[kFineStage];
     const uint32_t key0 = blockIdx.x << kBinBits;
     const uint32_t g = find_owner(gkb, ngroups, key0);
     const sgroup_t S = sg[g];
@@ -280,7 +285,16 @@ k_bin_fine(const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, ui
     if (threadIdx.x < F) fcount[threadIdx.x] = 0;
     if (blockIdx.x == 0 && threadIdx.x == 0) off[nkeys] = pos[ncounts];   // total entries
     __syncthreads();
-    for (uint32_t e = lo + threadIdx.x; e < hi; e += blockDim.x) atomicAdd(&fcount[tmp[e].x & (F - 1)], 1u);
+    const bool staged = hi - lo <= kFineStage;   // uniform over the block
+    if (staged) {
+        for (uint32_t e = threadIdx.x; e < hi - lo; e += blockDim.x) {
+            const uint2 v = tmp[lo + e];
+            stage[e] = v;
+            atomicAdd(&fcount[v.x & (F - 1)], 1u);
+        }
+    } else {
+        for (uint32_t e = lo + threadIdx.x; e < hi; e += blockDim.x) atomicAdd(&fcount[tmp[e].x & (F - 1)], 1u);
+    }
     __syncthreads();
     if (threadIdx.x < 64) {
         // F == 64: one wave scans the bin's bucket counts
@@ -296,9 +310,16 @@ k_bin_fine(const sgroup_t* __restrict__ sg, const uint32_t* __restrict__ gkb, ui
         off[key0 + threadIdx.x] = lo + x - v;
     }
     __syncthreads();
-    for (uint32_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
-        const uint2 v = tmp[e];
-        sorted[lo + atomicAdd(&cursor[v.x & (F - 1)], 1u)] = v;
+    if (staged) {
+        for (uint32_t e = threadIdx.x; e < hi - lo; e += blockDim.x) {
+            const uint2 v = stage[e];
+            sorted[lo + atomicAdd(&cursor[v.x & (F - 1)], 1u)] = v;
+        }
+    } else {
+        for (uint32_t e = lo + threadIdx.x; e < hi; e += blockDim.x) {
+            const uint2 v = tmp[e];
+            sorted[lo + atomicAdd(&cursor[v.x & (F - 1)], 1u)] = v;
+        }
     }
 }
 static_assert((1u << kBinBits) == 64, "k_bin_fine scans one bin with one wave");
