@@ -695,7 +695,7 @@ void for_each_buffer(Workspace& cx, const Layout& L, Fn&& f) {
     f(cx.buckets, (size_t)nkeys * pb);
     f(cx.slot_key, (size_t)nslots * 4);
     f(cx.slots, (size_t)nslots * pb);
-    f(cx.heavy, (3 * (size_t)(nslots / 2) + 64) * 4);   // list lengths, then three lists of chunk capacity
+    f(cx.heavy, (3 * (size_t)(nslots / 2) + kListHeadWords) * 4);   // list lengths, part counters, three lists
     f(cx.H, L.tree_nodes * 2 * pb);
     f(cx.W, (size_t)L.groups.size() * pb);
     f(cx.res, (size_t)L.jobs.size() * pb);
@@ -757,14 +757,15 @@ void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool hos
     // buckets are not cleared: every bucket with entries is written below,
     // and the reduction skips the empty ones by their count.  The
     // accumulation lists the buckets of more than two pieces for the merge
-    // (list lengths cleared first).
+    // (list lengths and the heavy merge's part counters cleared first).
     uint32_t* heavy = cx.heavy.as<uint32_t>();
     merge_lists_t lists;
     lists.counts = heavy;
-    lists.wave = heavy + 64;
+    lists.parts = heavy + 64;
+    lists.wave = heavy + kListHeadWords;
     lists.group = lists.wave + nchunks;
     lists.few = lists.group + nchunks;
-    fill_words(cx, heavy, 3, 0);
+    fill_words(cx, heavy, kListHeadWords, 0);
     if (timing) HIPCK(hipEventRecord(cx.ev[2], st));
     launch_k_accumulate(dim3(div_up(nchunks, 256)), dim3(256), st, cx.sorted.as<const uint2>(), off + nkeys, D.jobs,
                         D.kbase, njobs, cx.buckets.as<P>(), cx.slot_key.as<uint32_t>(), cx.slots.as<P>(), T,
@@ -773,7 +774,7 @@ void enqueue_points(Workspace& cx, const Layout& L, const DevLayout& D, bool hos
     if (timing) HIPCK(hipEventRecord(cx.ev[3], st));
 
     // buckets spanning chunks: every kind of merge in one launch
-    launch_k_partial_merge(st, cx.slot_key.as<const uint32_t>(), cx.slots.as<const P>(), off, hist, T, nchunks,
+    launch_k_partial_merge(st, cx.slot_key.as<const uint32_t>(), cx.slots.as<P>(), off, hist, T, nchunks,
                            cx.buckets.as<P>(), lists);
     HIPCK(hipGetLastError());
     if (timing) HIPCK(hipEventRecord(cx.ev[4], st));

@@ -41,12 +41,22 @@ constexpr uint32_t kInvalid = 0xFFFFFFFFu;
 constexpr uint32_t kLightSpan = 4;
 constexpr uint32_t kWaveSpan = 128;
 
+// A bucket of more than kWaveSpan pieces is merged by workgroups; when fewer
+// such buckets are listed than the merge has workgroups for them, each
+// bucket's pieces are cut into up to kHeavyParts parts (never more than its
+// pieces), and parts[h] counts the finished parts of listed bucket h.
+constexpr uint32_t kHeavyParts = 128;
+static_assert(kHeavyParts <= kWaveSpan, "every part of a listed bucket holds a piece");
+
+// list lengths (counts[0..2]), part counters, then the three lists
 struct merge_lists_t {
     uint32_t* counts;
+    uint32_t* parts;
     uint32_t* few;
     uint32_t* wave;
     uint32_t* group;
 };
+constexpr uint32_t kListHeadWords = 64 + kHeavyParts;
 
 __device__ __forceinline__ void list_bucket(const merge_lists_t& L, uint32_t m, uint32_t key) {
     if (m <= kLightSpan) L.few[atomicAdd(&L.counts[2], 1u)] = key;

@@ -149,6 +149,24 @@ def test_heavy_bucket_long_span(hip, orc, base_points):
     assert orc.compress(hip.p1s_mult_pippenger(pts, sc, 255)) == want
 
 
+@pytest.mark.parametrize("k", [1, 0x1234567890ABCDEF1122334455667788])
+def test_heavy_bucket_merged_in_parts(hip, orc, base_points, k):
+    # 2^18 equal scalars: every window's one bucket holds all entries, 8192
+    # tile pieces, merged by several workgroups in parts whose sums the last
+    # one to finish adds (k = 1: a single bucket in all)
+    n = 1 << 18
+    pts = np.ascontiguousarray(np.tile(base_points, (n // base_points.shape[0], 1)))
+    sc = _scal([k] * n)
+    want = orc.compress(orc.msm_pippenger(pts, sc, n, 255))
+    dp, ds = hip.DeviceBuffer(pts), hip.DeviceBuffer(sc)
+    try:
+        for _ in range(2):   # the part counters start from zero on every call
+            assert hip.compress(hip.msm_device(dp.ptr, ds.ptr, n, 255)) == want
+    finally:
+        dp.close()
+        ds.close()
+
+
 def test_window_group_pipelines(hip, orc, base_points):
     # the windows split over 1..4 concurrent pipelines give one result
     n = 1 << 17
