@@ -317,6 +317,48 @@ def gather_probe(n=1 << 20, windows=16):
                                        "ns_per_entry_128B": ns128, "same_sums": bool(np.array_equal(s96, s128))}}))
 
 
+def skewed_scalars(n=1 << 20, reps=5):
+    """Lone MSMs of n device-resident points under scalar mixes that crowd
+    the buckets: uniform, 16 distinct values, half the scalars equal, all
+    equal, all 1 (every entry in one bucket of one window).  Median and best
+    of reps timed calls after one untimed call.  Timing only: results under
+    such mixes are checked against the oracle in tests/test_msm_gpu.py
+    (heavy-bucket tests) and tests/test_random_sweep_gpu.py."""
+    import blst_hip as bh
+
+    pts = bh.synth_points(n, SEED ^ 3)
+    uni = bh.synth_scalars(n, SEED ^ 4)
+    one_val = np.frombuffer((0x1234567890ABCDEF1122334455667788).to_bytes(32, "little"), np.uint8)
+    unit = np.frombuffer((1).to_bytes(32, "little"), np.uint8)
+    rng = np.random.default_rng(SEED & 0xFFFFFFFF)
+    mixes = {
+        "uniform": uni,
+        "16 distinct": uni[rng.integers(0, 16, n)],
+        "half equal": np.where((rng.random(n) < 0.5)[:, None], one_val[None, :], uni),
+        "all equal": np.repeat(one_val[None, :], n, axis=0),
+        "all one": np.repeat(unit[None, :], n, axis=0),
+    }
+    out = {}
+    d_p = bh.DeviceBuffer(pts)
+    try:
+        for name, s in mixes.items():
+            d_s = bh.DeviceBuffer(np.ascontiguousarray(s, np.uint8))
+            try:
+                bh.msm_device(d_p.ptr, d_s.ptr, n, 255)
+                ts = []
+                for _ in range(reps):
+                    t = time.perf_counter()
+                    bh.msm_device(d_p.ptr, d_s.ptr, n, 255)
+                    ts.append(time.perf_counter() - t)
+            finally:
+                d_s.close()
+            out[name] = {"median_ms": 1e3 * float(np.median(ts)), "best_ms": 1e3 * min(ts)}
+            print("%-12s %.2f ms" % (name, out[name]["median_ms"]), file=sys.stderr, flush=True)
+    finally:
+        d_p.close()
+    print(json.dumps({"skewed_scalars": {"points": n, "nbits": 255, "ms": out}}))
+
+
 def copy_calibration(nbytes, reps=3):
     """Device-to-device copies of a known size: the HIP runtime's blit kernel
     reads and writes nbytes each, so FETCH_SIZE / WRITE_SIZE of these
@@ -358,6 +400,8 @@ def main():
                     help="only run the batch-affine against XYZZ addition probe")
     ap.add_argument("--gather-probe", action="store_true",
                     help="only run the point-gather probe at 96- and 128-byte point strides")
+    ap.add_argument("--skewed-scalars", action="store_true",
+                    help="only time lone 2^20 MSMs under scalar mixes that crowd the buckets")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run device-to-device copies of BYTES (HBM counter calibration)")
     args = ap.parse_args()
@@ -369,6 +413,9 @@ def main():
         return
     if args.gather_probe:
         gather_probe()
+        return
+    if args.skewed_scalars:
+        skewed_scalars()
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the MSM backend runs on a single device "
